@@ -1,0 +1,134 @@
+/*
+ * paddi.h -- C ABI of the MI355X (gfx950) discord engine.
+ *
+ * One GPU's share of PADDi's discord search: for every subsequence length m in
+ * minL..maxL (step `step`), the top-k subsequences whose nearest
+ * non-overlapping neighbour is farthest away under z-normalised Euclidean
+ * distance.  All distances crossing this boundary are SQUARED z-normalised
+ * distances, as in the reference (struct Discord, src/PALMAD_types.hpp:7-11;
+ * r is passed squared, src/main.cu:144).
+ *
+ * Conventions (shared by every entry point):
+ *   - plain pointers and sizes only; host pointers unless the name says
+ *     "device"; the library owns its device scratch;
+ *   - calls block until the result is on the host and return a status code
+ *     (never exit(); the reference's CHECK() does, src/common.h:6-16);
+ *   - a context may be used by one host thread at a time; separate contexts
+ *     are independent;
+ *   - row i and column j form a valid pair iff |i - 1 - j| >= m
+ *     (src/cpu_kernels/cpu_kernels.cpp:57); a row is pruned by a valid pair
+ *     with ED^2 <= r2 (cpu_kernels.cpp:59-72); a window with zero standard
+ *     deviation has ED^2 = 2m to everything (src/PD3_kernels.cuh:63-64).
+ */
+#ifndef PADDI_H
+#define PADDI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t paddi_status;
+enum {
+    PADDI_OK = 0,
+    PADDI_ERR_INVALID = 1,      /* bad argument (sizes, ranges, null pointers) */
+    PADDI_ERR_DEVICE = 2,       /* HIP runtime error */
+    PADDI_ERR_NOMEM = 3,        /* device allocation failed */
+    PADDI_ERR_CAPACITY = 4,     /* output buffer too small; *count says how many */
+    PADDI_ERR_UNSUPPORTED = 5   /* request outside the engine's limits (e.g. m > 4096) */
+};
+
+/* Output record: struct Discord {int ind; DATA_TYPE dist;} (src/PALMAD_types.hpp:7-11)
+ * plus the length it was found at and, for batches, the series it belongs to. */
+typedef struct {
+    int64_t index;      /* 0-based start of the subsequence in its series */
+    int32_t length;     /* m */
+    int32_t series;     /* series number in a batch call, 0 otherwise */
+    float nn_dist2;     /* squared z-normalised distance to the nearest neighbour */
+    float range2;       /* squared range r of the attempt that returned it */
+} paddi_discord;
+
+enum {
+    PADDI_MODE_PLAIN = 0,    /* per-length launches, host r-descent (main.cu:136-233) */
+    PADDI_MODE_BATCHED = 1,  /* all lengths in one persistent launch (hipGraph) */
+    PADDI_MODE_LEARNED = 2   /* batched + learned r proposal, plain descent as fallback */
+};
+
+typedef struct {
+    int32_t mode;            /* PADDI_MODE_* */
+    int32_t max_attempts;    /* r-descent attempts per length before r = 0 (default 64) */
+    int32_t flags;           /* reserved, 0 */
+    int32_t reserved;
+    const char *model_path;  /* learned mode: weights file, NULL = built-in default */
+} paddi_opts;
+
+/* Per-call counters (optional output). */
+typedef struct {
+    int64_t attempts;        /* (m, r) passes run, summed over lengths */
+    int64_t survivors;       /* rows that survived selection, summed */
+    int64_t exact_refines;   /* candidates refined in double precision */
+    int64_t pairs;           /* pair evaluations counted on the device (estimate) */
+    double seconds;          /* wall time of the call */
+    double kernel_ms[4];     /* stats, select, refine, exact (ms, hipEvent) */
+} paddi_stats;
+
+typedef struct paddi_ctx paddi_ctx;
+typedef struct paddi_series paddi_series;
+
+const char *paddi_status_str(paddi_status s);
+const char *paddi_version(void);
+
+/* Context on HIP device `device`.  Replaces the per-GPU allocation block of
+ * src/main.cu:54-125 (the library owns all device buffers). */
+paddi_status paddi_create(int32_t device, paddi_ctx **out);
+void paddi_destroy(paddi_ctx *ctx);
+
+/* Sliding mean and standard deviation of the n-m+1 windows of T.
+ * Replaces compute_statistics / compute_statistics_cpu
+ * (src/preprocessing.cuh:68-76, src/cpu_kernels/cpu_kernels.cpp:7-31);
+ * computed from double-precision prefix sums, returned as float. */
+paddi_status paddi_compute_statistics(paddi_ctx *ctx, const float *T, int64_t n, int32_t m,
+                                      float *mean, float *std);
+
+/* Range discords at squared range r2 over the n-m+1 windows of T.
+ * Replaces find_discords_cpu (src/cpu_kernels/cpu_kernels.cpp:33-78) and the
+ * single-GPU do_PD3 + refine pipeline (src/PD3.cuh:15-61):
+ * is_discord[i] = 1 iff every valid pair of row i has ED^2 > r2, and then
+ * nn_dist2[i] is row i's nearest-neighbour ED^2 (FLT_MAX when row i has no
+ * valid pair).  nn_dist2 of rows with is_discord[i] = 0 is unspecified. */
+paddi_status paddi_range_discords(paddi_ctx *ctx, const float *T, int64_t n, int32_t m, float r2,
+                                  int32_t *is_discord, float *nn_dist2);
+
+/* Top-k discords for every length m = minL, minL+step, .., <= maxL.
+ * Replaces the PALMAD length loop of src/main.cu:136-233 (top-1 there; for
+ * k > 1 discords are ranked by (nn_dist2 desc, index asc) and accepted when
+ * |i - a| >= m for every accepted a).  out[] receives up to top_k records per
+ * length, lengths ascending, each length's records in rank order; *count is
+ * the number written.  opts and stats may be NULL. */
+paddi_status paddi_discords(paddi_ctx *ctx, const float *T, int64_t n, int32_t minL, int32_t maxL,
+                            int32_t step, int32_t top_k, const paddi_opts *opts,
+                            paddi_discord *out, int64_t cap, int64_t *count, paddi_stats *stats);
+
+/* Many independent series (uneven lengths) in one call; records carry
+ * `series`.  Same per-series semantics as paddi_discords. */
+paddi_status paddi_discords_batch(paddi_ctx *ctx, const float *const *T, const int64_t *n,
+                                  int32_t nseries, int32_t minL, int32_t maxL, int32_t step,
+                                  int32_t top_k, const paddi_opts *opts, paddi_discord *out,
+                                  int64_t cap, int64_t *count, paddi_stats *stats);
+
+/* Device-resident series: upload once, search many times (the bench times
+ * paddi_discords_series with the series already in HBM).  These have no
+ * counterpart in the reference, whose driver copies the series once per run
+ * (src/main.cu:129-133). */
+paddi_status paddi_series_upload(paddi_ctx *ctx, const float *T, int64_t n, paddi_series **out);
+void paddi_series_free(paddi_series *s);
+paddi_status paddi_discords_series(paddi_ctx *ctx, paddi_series *const *series, int32_t nseries,
+                                   int32_t minL, int32_t maxL, int32_t step, int32_t top_k,
+                                   const paddi_opts *opts, paddi_discord *out, int64_t cap,
+                                   int64_t *count, paddi_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PADDI_H */

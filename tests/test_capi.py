@@ -1,0 +1,57 @@
+"""The C-ABI library loads on a machine without a GPU and exports every function that
+include/paddi.h declares (no compute calls here)."""
+import ctypes
+import os
+import re
+
+from paddi_amd import engine
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if not h.endswith(".h"):
+            continue
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        names += re.findall(r"\b(paddi_[a-z0-9_]+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def test_header_declares_the_runner_api():
+    names = declared_functions()
+    for n in ("paddi_create", "paddi_destroy", "paddi_range_discords", "paddi_discords",
+              "paddi_discords_batch", "paddi_compute_statistics", "paddi_status_str"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(engine.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_layer_binds_all_exports():
+    L = engine.load_library()
+    for n in engine.EXPORTS:
+        assert hasattr(L, n)
+    assert set(engine.EXPORTS) <= set(declared_functions())
+
+
+def test_status_strings_without_device():
+    L = engine.load_library()
+    assert L.paddi_status_str(0) == b"ok"
+    assert L.paddi_status_str(1) == b"invalid argument"
+    assert b"gfx950" in L.paddi_version()
+
+
+def test_create_fails_cleanly_without_device():
+    import torch
+    if torch.cuda.is_available():
+        return
+    L = engine.load_library()
+    p = ctypes.c_void_p()
+    st = L.paddi_create(0, ctypes.byref(p))
+    assert st != 0 and not p

@@ -1,0 +1,143 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle.
+
+Bar: discord indices and lengths bit-exact; squared NN distances within 1e-4 relative of
+the double-precision oracle (the engine reports the exact double-precision value for
+accepted discords, so the observed error is ~1e-7); range-discord flags bit-exact except
+for rows whose exact distance is within 1e-7 relative of r2, where two double-precision
+evaluations in different operation orders may legitimately disagree.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from paddi_amd import synth
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+RTOL = 1e-4
+
+
+def brute_cases():
+    z = np.load(os.path.join(GOLD, "brute_profiles.npz"))
+    names = sorted({k.split("__")[0] for k in z.files})
+    return [(n, z[f"{n}__x"], int(z[f"{n}__m"][0]), z[f"{n}__nn"]) for n in names]
+
+
+def expected(ref):
+    return [(int(i), r["m"], float(d)) for r in ref for i, d in zip(r["idx"], r["d2"])]
+
+
+def assert_same_discords(got, exp, tag=""):
+    assert len(got) == len(exp), (tag, [(g.index, g.length) for g in got], exp)
+    worst = 0.0
+    for g, (i, m, d) in zip(got, exp):
+        assert (g.index, g.length) == (i, m), (tag, g, i, m)
+        if d < 3e38:
+            rel = abs(g.nn_dist2 - d) / max(d, 1e-30)
+            worst = max(worst, rel)
+            assert rel <= RTOL, (tag, g, d, rel)
+    return worst
+
+
+def test_statistics_match_oracle(ctx):
+    for x, m in ((synth.ar2_mix(5000, 31), 50),
+                 (synth.rw_planted(1 << 20, 2, 4, 192, 24.0), 256),
+                 (synth.ecg_like(20000, 32, 1, 1), 100)):
+        mu, sd = ctx.compute_statistics(x, m)
+        mu_r, sd_r = O.stats(x, m)
+        np.testing.assert_allclose(mu, mu_r, rtol=2e-7, atol=1e-6)
+        np.testing.assert_allclose(sd, sd_r, rtol=2e-6)
+
+
+@pytest.mark.parametrize("case", brute_cases(), ids=lambda c: c[0])
+def test_range_discords_match_oracle(ctx, case):
+    name, x, m, nn_ref = case
+    for q in (0.0, 0.5, 0.9, 0.99, 0.999):
+        r2 = float(np.quantile(nn_ref, q)) if q > 0 else 0.0
+        flag, nn = ctx.range_discords(x, m, r2)
+        cand, nn_o = O.range_discords(x, m, r2)
+        near = np.abs(nn_ref - r2) <= 1e-7 * max(r2, 1e-3)
+        assert np.array_equal((flag == 1)[~near], (cand == 1)[~near]), (name, q)
+        sel = (flag == 1) & (cand == 1) & (nn_o < 1e300) & (nn_o > 1e-3)
+        if sel.any():
+            rel = np.abs(nn[sel] - nn_o[sel]) / nn_o[sel]
+            assert rel.max() <= RTOL, (name, q, rel.max())
+
+
+@pytest.mark.parametrize("kind,n,minL,maxL,step,k", [
+    ("rw", 4000, 40, 44, 1, 2),
+    ("rw", 9000, 64, 70, 3, 3),
+    ("ecg", 12000, 60, 66, 2, 3),
+    ("ar2", 8000, 30, 36, 1, 2),
+    ("rw", 2000, 100, 100, 1, 3),
+    ("ar2", 5000, 8, 12, 1, 4),
+])
+def test_discords_match_oracle(ctx, kind, n, minL, maxL, step, k):
+    x = {"rw": lambda: synth.rw_planted(n, 40 + n % 97, 2, max(minL, 16), 12.0),
+         "ecg": lambda: synth.ecg_like(n, 41, 1, 1),
+         "ar2": lambda: synth.ar2_mix(n, 42)}[kind]()
+    ref = O.palmad(x, minL, maxL, step, k)
+    got = ctx.discords(x, minL, maxL, step, k, mode="plain")
+    worst = assert_same_discords(got, expected(ref), kind)
+    print(f"{kind} n={n} m={minL}..{maxL}/{step} k={k}: max rel err vs FP64 = {worst:.2e}")
+    # the plain form follows the same r-schedule as the oracle driver
+    for r in ref:
+        gs = [g for g in got if g.length == r["m"]]
+        if gs:
+            assert gs[0].range2 == pytest.approx(r["r2"], rel=1e-5, abs=1e-6)
+
+
+def test_c1_matches_cached_oracle(ctx):
+    w = synth.workload_c1()
+    with open(os.path.join(GOLD, "palmad_c1.json")) as f:
+        ref = json.load(f)
+    got = ctx.discords(w["series"][0], w["minL"], w["maxL"], w["step"], w["k"], mode="plain")
+    worst = assert_same_discords(got, expected(ref), "c1")
+    print(f"c1: max rel err vs FP64 = {worst:.2e}")
+
+
+def test_edge_cases(ctx):
+    # fewer windows than one 64-row block; ragged last block; k larger than possible
+    for n, m, k in ((60, 8, 2), (200, 16, 3), (131, 10, 20), (1000, 31, 5)):
+        x = synth.ar2_mix(n, n)
+        ref = O.palmad(x, m, m, 1, k)
+        got = ctx.discords(x, m, m, 1, k, mode="plain")
+        assert_same_discords(got, expected(ref), f"n={n} m={m} k={k}")
+
+
+def test_degenerate_windows(ctx):
+    x = synth.rw_planted(3000, 50, 1, 32, 8.0).astype(np.float64)
+    x[1000:1200] = 2.5  # flat: zero standard deviation, ED^2 = 2m to everything
+    x[2000:2100] = 0.0
+    x = x.astype(np.float32)
+    for m in (24, 40):
+        ref = O.palmad(x, m, m, 1, 3)
+        got = ctx.discords(x, m, m, 1, 3, mode="plain")
+        assert_same_discords(got, expected(ref), f"flat m={m}")
+
+
+def test_deterministic_replay(ctx):
+    x = synth.rw_planted(30000, 60, 3, 64, 16.0)
+    a = ctx.discords(x, 60, 66, 1, 3)
+    b = ctx.discords(x, 60, 66, 1, 3)
+    assert [(d.index, d.length, d.nn_dist2) for d in a] == \
+        [(d.index, d.length, d.nn_dist2) for d in b]
+
+
+def test_certificate_at_scale(ctx):
+    """262,144-point random walk: the top-3 per length is certified by the oracle run at
+    r2' = (1 - 1e-3) * d_3: if the oracle's range set at r2' yields the same greedy top-3,
+    the engine's answer is the exact one (the top-k is r-independent below d_k)."""
+    x = synth.rw_planted(262144, 70, 6, 160, 24.0)
+    got = ctx.discords(x, 128, 130, 1, 3)
+    for m in (128, 130):
+        gs = [g for g in got if g.length == m]
+        assert len(gs) == 3
+        r2 = (1 - 1e-3) * gs[-1].nn_dist2
+        cand, nn = O.range_discords(x, m, r2)
+        idx, d, nd = O.topk(cand, nn, r2, m, 3)
+        assert [g.index for g in gs] == list(idx), (m, gs, idx)
+        np.testing.assert_allclose([g.nn_dist2 for g in gs], d, rtol=RTOL)
