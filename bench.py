@@ -1,0 +1,185 @@
+"""Throughput of the discord engine on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3] [--lengths L]
+
+A step is one complete discord search over the workload (every length minL..maxL, top-k
+per length), from the host call to the returned discord set, with the series already
+resident in HBM.  The default workload is the 8,388,608-point random walk with 20 planted
+bursts, m = 128..256 (129 lengths), top-3.  Both GPU forms run: the plain form (per-length
+launches, host r-descent as src/main.cu:136-233) and the primary form (batched launches
+with the learned r proposal when built); `value` is the primary form's throughput and
+both must return the identical discord set.
+
+Printed: one JSON line with the metric, the roofline of the dominant kernel (algorithmic
+bytes per launch / hipEvent launch time, against 8 TB/s HBM3E) and the CPU oracle timed on
+a bounded sample on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK_OPS = 256 * 4 * 16 * 2.4e9  # CUs x SIMDs x lanes x clock: one VALU op per lane
+OPS_PER_PAIR = 5.5                     # recurrence + compare + max per pair (scan inner loop)
+
+WORKLOAD_NAMES = {
+    "c1": "rw65k_m128_top3",
+    "c2": "ecg1m_m64-128_top3",
+    "c3": "rw8m_m128-256_top3",
+    "c4": "ar2mix2m_m256-512s4_top3",
+}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=2)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--workload", default="c3", choices=sorted(WORKLOAD_NAMES))
+    p.add_argument("--lengths", type=int, default=0, help="limit the number of lengths (debug)")
+    p.add_argument("--primary", default="auto", help="plain|batched|learned|auto")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-points", type=int, default=65536)
+    return p.parse_args()
+
+
+def same_set(a, b, rtol=1e-4):
+    if len(a) != len(b):
+        return False
+    for x, y in zip(a, b):
+        if (x.index, x.length, x.series) != (y.index, y.length, y.series):
+            return False
+        if abs(x.nn_dist2 - y.nn_dist2) > rtol * max(abs(y.nn_dist2), 1e-30):
+            return False
+    return True
+
+
+def run_form(ctx, handles, w, maxL, mode, steps, warmup):
+    for _ in range(warmup):
+        ctx.discords_series(handles, w["minL"], maxL, w["step"], w["k"], mode=mode)
+    times, stats, recs = [], [], None
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        recs, st = ctx.discords_series(handles, w["minL"], maxL, w["step"], w["k"], mode=mode,
+                                       with_stats=True)
+        times.append(time.perf_counter() - t0)
+        stats.append(st)
+    return recs, times, stats
+
+
+def cpu_baseline(x, w, npts):
+    """The oracle (C restatement of the reference CPU path, OpenMP over rows) on the first
+    `npts` points at the first length, with the main.cu r-schedule."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    xs = x[:npts]
+    t0 = time.perf_counter()
+    O.palmad(xs, w["minL"], w["minL"], 1, w["k"], nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": npts * 1 / dt, "unit": "points*lengths/s", "cores": threads,
+            "kind": "port", "seconds": dt,
+            "sample": f"first {npts} points of the series, m={w['minL']} only, top-{w['k']}, "
+                      f"plain r-descent, FP64 oracle, {threads} threads"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
+              "multi-GPU sharding is design-only", file=sys.stderr)
+        sys.exit(2)
+    from paddi_amd import synth
+    from paddi_amd.engine import Context, load_library
+
+    w = synth.WORKLOADS[a.workload]()
+    series = w["series"]
+    nlen_full = (w["maxL"] - w["minL"]) // w["step"] + 1
+    nlen = min(nlen_full, a.lengths) if a.lengths > 0 else nlen_full
+    maxL = w["minL"] + (nlen - 1) * w["step"]
+    points = sum(len(x) for x in series)
+    units = points * nlen
+
+    L = load_library()
+    ctx = Context(0)
+    handles = [ctx.upload(x) for x in series]
+
+    primary = a.primary
+    if primary == "auto":
+        primary = "learned" if getattr(L, "paddi_has_mode", None) and L.paddi_has_mode(2) else \
+            ("batched" if getattr(L, "paddi_has_mode", None) and L.paddi_has_mode(1) else "plain")
+
+    print(f"[bench] workload {a.workload}: {len(series)} series, {points} points, "
+          f"{nlen} lengths, top-{w['k']}; primary form: {primary}", file=sys.stderr, flush=True)
+    plain_recs, plain_t, plain_st = run_form(ctx, handles, w, maxL, "plain", a.steps if
+                                             primary == "plain" else 1,
+                                             a.warmup if primary == "plain" else 0)
+    print(f"[bench] plain: {[round(t, 3) for t in plain_t]} s", file=sys.stderr, flush=True)
+    if primary == "plain":
+        recs, times, stats = plain_recs, plain_t, plain_st
+    else:
+        recs, times, stats = run_form(ctx, handles, w, maxL, primary, a.steps, a.warmup)
+        print(f"[bench] {primary}: {[round(t, 3) for t in times]} s", file=sys.stderr,
+              flush=True)
+    identical = same_set(recs, plain_recs)
+
+    t_step = sum(times) / len(times)
+    value = units / t_step
+    plain_value = units / (sum(plain_t) / len(plain_t))
+
+    # dominant kernel: the one with the most summed time across the timed steps
+    names = ("derive", "select", "refine", "exact")
+    tot = {k: sum(s["kernel_ms"][k] for s in stats) for k in names}
+    nl = {k: sum(s["launches"][k] for s in stats) for k in names}
+    dom = max(names, key=lambda k: tot[k])
+    avg_ms = tot[dom] / max(nl[dom], 1)
+    # algorithmic bytes per launch: 16 B per (point x length); one launch covers one length
+    bytes_per_launch = 16.0 * points
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    pairs = sum(s["pairs"] for s in stats) / len(stats)
+    scan_ms = (tot["select"] + tot["refine"]) / len(stats)
+    valu = (pairs * OPS_PER_PAIR) / (scan_ms * 1e-3) / VALU_PEAK_OPS if scan_ms > 0 else 0.0
+
+    out = {
+        "metric": "points×lengths/sec, batched+learned vs plain GPU PALMAD, identical discord set",
+        "value": value,
+        "unit": "points*lengths/s",
+        "n_gpus": 1,
+        "steps": len(times),
+        "warmup": a.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD_NAMES[a.workload], "points": points,
+                   "series": len(series), "lengths": nlen, "m": f"{w['minL']}..{maxL}",
+                   "step": w["step"], "top_k": w["k"], "form": primary},
+        "plain": {"value": plain_value, "ms_per_step": 1e3 * sum(plain_t) / len(plain_t),
+                  "attempts": plain_st[-1]["attempts"]},
+        "speedup_vs_plain": value / plain_value,
+        "identical_discord_set": identical,
+        "attempts": stats[-1]["attempts"],
+        "survivors": stats[-1]["survivors"],
+        "exact_refines": stats[-1]["exact_refines"],
+        "kernel_ms_per_step": {k: tot[k] / len(stats) for k in names},
+        "roofline": {"bound": "hbm", "kernel": f"k_scan_{dom}" if dom in ("select", "refine")
+                     else f"k_{dom}", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": avg_ms, "launches": nl[dom],
+                     "bytes_per_launch": bytes_per_launch},
+        "valu": {"pairs_per_step": pairs, "ops_per_pair": OPS_PER_PAIR,
+                 "frac_of_peak": valu},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(series[0], w, a.cpu_points)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

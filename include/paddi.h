@@ -70,7 +70,8 @@ typedef struct {
     int64_t exact_refines;   /* candidates refined in double precision */
     int64_t pairs;           /* pair evaluations counted on the device (estimate) */
     double seconds;          /* wall time of the call */
-    double kernel_ms[4];     /* stats, select, refine, exact (ms, hipEvent) */
+    double kernel_ms[4];     /* derive, select, refine, exact: summed kernel time (ms, hipEvent) */
+    int64_t launches[4];     /* launches of the same four kernels */
 } paddi_stats;
 
 typedef struct paddi_ctx paddi_ctx;
@@ -78,6 +79,9 @@ typedef struct paddi_series paddi_series;
 
 const char *paddi_status_str(paddi_status s);
 const char *paddi_version(void);
+/* 1 if this build supports PADDI_MODE_* `mode` (the learned mode needs its model).
+ * No counterpart in the reference, which has a single form. */
+int32_t paddi_has_mode(int32_t mode);
 
 /* Context on HIP device `device`.  Replaces the per-GPU allocation block of
  * src/main.cu:54-125 (the library owns all device buffers). */

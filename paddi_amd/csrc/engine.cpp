@@ -1,20 +1,30 @@
 // Host side of the discord engine and its C ABI (include/paddi.h).
 //
-// Per-(m, r) pipeline (the single-GPU slice of the reference's missing
+// Per-(m, r) pass over a batch of series (the single-GPU slice of the reference's missing
 // do_PALMAD_one_length, called at src/main.cu:144,176,219):
 //   k_derive        window statistics of length m from the series' prefix sums
-//   k_scan_select   64-row blocks scan a bounded window to their right (PD3 phase 1)
+//   k_scan_select   64-row blocks scan their witness tiles and a bounded window to their right
 //   k_list_blocks   blocks with live rows
-//   k_scan_refine   live blocks x column chunks over all remaining columns (PD3 phase 2)
+//   k_scan_refine   live blocks x column chunks over all remaining columns
 //   k_survivors     surviving rows with their single-precision NN distance
-//   host            exact top-k: interval ordering + k_exact (double) where it matters
+//   TopK            exact top-k: interval ordering + k_exact (double) where it matters
 //
 // Exactness.  The scans prune at ED^2 <= r2 - delta, where delta bounds the error of the
 // single-precision distance, so the surviving set contains every row whose exact NN distance
 // exceeds r2, and each survivor's single-precision distance a_i is within delta of the exact
-// one.  select_topk() ranks survivors by exact distance, computing the exact (double)
-// distance only for candidates whose rank the +-delta intervals leave undecided, and for
-// every accepted discord (whose reported distance is then the double-precision value).
+// one.  TopK ranks survivors by exact distance, computing the exact (double) distance only
+// for candidates whose rank the +-delta intervals leave undecided and for every accepted
+// discord (whose reported distance is then the double-precision value).
+//
+// Forms.
+//   plain   each series alone, each length with the r-descent of src/main.cu:136-233.
+//   fast    all series of the call in every launch; per length, r comes from the exact
+//           double-precision NN distances (at the new length) of the previous length's
+//           discords -- a lower bound of the new k-th distance whenever they stay mutually
+//           non-overlapping -- or, for the first length, from the learned proposal (or a
+//           geometric probe descent without a model); blocks first rescan the tiles that
+//           pruned their rows at the previous length.  The result is the same discord set:
+//           the top-k does not depend on r once r is below the k-th distance.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,8 +38,7 @@
 #include "../../include/paddi.h"
 #include "engine.h"
 #include "kernels.h"
-
-using namespace paddi;
+#include "learned.h"
 
 namespace paddi {
 
@@ -78,11 +87,88 @@ void DevBuf::release()
 double delta_ed2(int m)
 {
     // Bound on |single - exact| squared distance: correlation error of a seed (m-term FP32
-    // dot product on locally centred data) plus up to 63 recurrence steps, with a 4x
-    // safety factor over the worst case measured on the test workloads.
+    // dot product on locally centred data) plus up to 63 recurrence steps, with a wide
+    // safety factor over the largest error measured on the test workloads.
     const double rho_err = 4.0e-5 + 2.0e-7 * m;
     return 2.0 * m * rho_err;
 }
+
+static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+static inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+// ------------------------------------------------------------------------------------------
+// Exact greedy top-k over the survivors of one series.
+//
+// Every survivor c has an interval [lo_c, hi_c] = [a_c - delta, a_c + delta] holding its
+// exact distance (a point once refined).  T = argmax hi; the exact maximum lies in
+// U = {c : hi_c >= lo_T}.  Once U is all exact, its maximum (ties: lower index) is the next
+// candidate in exact order; it is accepted when its distance exceeds r2, and every
+// candidate overlapping it (|i - a| < m) is dropped, since all of them rank below it.
+
+void TopK::reset(std::vector<Cand> &&c, double r2_, int m_, int k_)
+{
+    cands = std::move(c);
+    gone.assign(cands.size(), 0);
+    acc.clear();
+    r2 = r2_;
+    m = m_;
+    k = k_;
+    delta = delta_ed2(m_);
+    done = false;
+}
+
+bool TopK::advance(std::vector<size_t> &need)
+{
+    need.clear();
+    auto lo = [&](const Cand &c) {
+        return c.exact ? c.v : (c.a >= 3.0e38f ? INFINITY : (double)c.a - delta);
+    };
+    auto hi = [&](const Cand &c) {
+        return c.exact ? c.v : (c.a >= 3.0e38f ? INFINITY : (double)c.a + delta);
+    };
+    while (!done) {
+        if ((int)acc.size() >= k) { done = true; break; }
+        int64_t T = -1;
+        double hT = -INFINITY;
+        for (size_t q = 0; q < cands.size(); q++) {
+            if (gone[q]) continue;
+            const double h = hi(cands[q]);
+            if (T < 0 || h > hT || (h == hT && cands[q].i < cands[(size_t)T].i)) {
+                T = (int64_t)q;
+                hT = h;
+            }
+        }
+        if (T < 0) { done = true; break; }
+        const double loT = lo(cands[(size_t)T]);
+        std::vector<size_t> U;
+        for (size_t q = 0; q < cands.size(); q++)
+            if (!gone[q] && hi(cands[q]) >= loT) U.push_back(q);
+        if (U.empty()) U.push_back((size_t)T);  // only when distances are not ordered (NaN)
+        for (size_t q : U)
+            if (!cands[q].exact) need.push_back(q);
+        if (!need.empty()) {
+            if (need.size() > 64) {
+                std::partial_sort(need.begin(), need.begin() + 64, need.end(),
+                                  [&](size_t a, size_t b) { return hi(cands[a]) > hi(cands[b]); });
+                need.resize(64);
+            }
+            return true;
+        }
+        size_t B = U[0];
+        for (size_t q : U) {
+            const Cand &a = cands[q], &b = cands[B];
+            if (a.v > b.v || (a.v == b.v && a.i < b.i)) B = q;
+        }
+        if (!(cands[B].v > r2)) { done = true; break; }
+        acc.push_back(cands[B]);
+        const int64_t bi = cands[B].i;
+        for (size_t q = 0; q < cands.size(); q++)
+            if (!gone[q] && std::llabs(cands[q].i - bi) < m) gone[q] = 1;
+    }
+    return false;
+}
+
+// ------------------------------------------------------------------------------------------
 
 Engine::Engine(int device) : device_(device)
 {
@@ -95,9 +181,6 @@ Engine::Engine(int device) : device_(device)
 Engine::~Engine()
 {
     (void)hipSetDevice(device_);
-    for (DevBuf *b : {&colv_, &st64_, &alive_, &best_, &blist_, &cnt_, &sidx_, &sd2_, &cand_,
-                      &okey_})
-        b->release();
     for (auto &e : ev_) (void)hipEventDestroy(e);
     if (pin_) (void)hipHostFree(pin_);
     (void)hipStreamDestroy(stream_);
@@ -106,206 +189,318 @@ Engine::~Engine()
 Series *Engine::upload(const float *T, int64_t n)
 {
     HIPC(hipSetDevice(device_));
-    Series *s = new Series();
+    std::unique_ptr<Series> s(new Series());
     s->n = n;
-    s->nb = (n + PB - 1) / PB + 1;
-    try {
-        s->x.reserve((size_t)(n + XPAD) * sizeof(float));
-        s->P.reserve((size_t)s->nb * (PB + 1) * sizeof(double2));
-        s->refs.reserve((size_t)s->nb * sizeof(double));
-        HIPC(hipMemsetAsync(s->x.p, 0, (size_t)(n + XPAD) * sizeof(float), stream_));
-        HIPC(hipMemcpyAsync(s->x.p, T, (size_t)n * sizeof(float), hipMemcpyHostToDevice,
-                            stream_));
-        HIPC(hipStreamSynchronize(stream_));
-    } catch (...) {
-        delete s;
-        throw;
+    s->x.reserve((size_t)n * sizeof(float));
+    HIPC(hipMemcpyAsync(s->x.p, T, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream_));
+    HIPC(hipStreamSynchronize(stream_));
+    return s.release();
+}
+
+void Engine::build_batch(Batch &B, const std::vector<const Series *> &ss, int minL)
+{
+    B.nser = (int)ss.size();
+    B.minL = minL;
+    B.n.assign(ss.size(), 0);
+    B.xoff.assign(ss.size(), 0);
+    B.roff.assign(ss.size(), 0);
+    B.pboff.assign(ss.size(), 0);
+    int64_t xo = 0, ro = 0, po = 0;
+    for (size_t q = 0; q < ss.size(); q++) {
+        const int64_t n = ss[q]->n;
+        B.n[q] = n;
+        B.xoff[q] = xo;
+        B.roff[q] = ro;
+        B.pboff[q] = po;
+        xo += round_up(n + XPAD, 64);
+        ro += round_up(n - minL + 1, 64) + 128;
+        po += (n + PB - 1) / PB + 1;
     }
-    return s;
+    B.xtotal = xo;
+    B.rows = ro;
+    B.nblocks = ro / 64;
+    B.npb = po;
+    B.x.reserve((size_t)xo * sizeof(float));
+    B.P.reserve((size_t)po * (PB + 1) * sizeof(double2));
+    B.refs.reserve((size_t)po * sizeof(double));
+    B.ser.reserve(ss.size() * sizeof(SerDev) * 4);
+    B.bser.reserve((size_t)B.nblocks * sizeof(int));
+    B.pser.reserve((size_t)po * sizeof(int));
+    HIPC(hipMemsetAsync(B.x.p, 0, (size_t)xo * sizeof(float), stream_));
+    for (size_t q = 0; q < ss.size(); q++)
+        HIPC(hipMemcpyAsync(B.x.as<float>() + B.xoff[q], ss[q]->x.p, (size_t)B.n[q] * 4,
+                            hipMemcpyDeviceToDevice, stream_));
+    B.bser_host.assign((size_t)B.nblocks, -1);
+    std::vector<int> pser((size_t)po, 0);
+    for (size_t q = 0; q < ss.size(); q++) {
+        const int64_t nb = (B.n[q] - minL + 1 + 63) / 64;
+        for (int64_t b = 0; b < nb; b++) B.bser_host[(size_t)(B.roff[q] / 64 + b)] = (int)q;
+        const int64_t np = (B.n[q] + PB - 1) / PB + 1;
+        for (int64_t b = 0; b < np; b++) pser[(size_t)(B.pboff[q] + b)] = (int)q;
+    }
+    HIPC(hipMemcpyAsync(B.bser.p, B.bser_host.data(), (size_t)B.nblocks * 4,
+                        hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(B.pser.p, pser.data(), (size_t)po * 4, hipMemcpyHostToDevice, stream_));
+    B.host.assign(ss.size(), SerDev{});
+    for (size_t q = 0; q < ss.size(); q++) {
+        SerDev &S = B.host[q];
+        S.xoff = B.xoff[q];
+        S.n = B.n[q];
+        S.roff = B.roff[q];
+        S.pboff = B.pboff[q];
+        S.N = B.n[q] - minL + 1;
+        S.thr = 1.0f;
+    }
+    upload_ser(B);
+    HIPC(hipStreamSynchronize(stream_));  // host vectors above must outlive the copies
 }
 
-void Engine::prefix(Series &s)
+void Engine::upload_ser(Batch &B)
 {
-    if (s.prefix_ready) return;
-    hipLaunchKernelGGL(k_prefix, dim3((unsigned)s.nb), dim3(256), 0, stream_, s.xp(), s.n,
-                       (double2 *)s.P.p, (double *)s.refs.p);
+    // synchronous with respect to the stream: the host array may change right after
+    HIPC(hipMemcpyAsync(B.ser.p, B.host.data(), B.host.size() * sizeof(SerDev),
+                        hipMemcpyHostToDevice, stream_));
+    HIPC(hipStreamSynchronize(stream_));
+}
+
+void Engine::prefix(Batch &B)
+{
+    hipLaunchKernelGGL(k_prefix, dim3((unsigned)B.npb), dim3(256), 0, stream_,
+                       B.x.as<float>(), B.ser.as<SerDev>(), B.pser.as<int>(), B.P.as<double2>(),
+                       B.refs.as<double>());
     HIPC(hipGetLastError());
-    s.prefix_ready = true;
 }
 
-void Engine::derive(Series &s, int m)
+void Engine::derive(Batch &B, int m, paddi_stats *st)
 {
-    const int64_t N = s.n - m + 1;
-    const int64_t Npad = ((N + 63) / 64) * 64 + 128;
-    colv_.reserve((size_t)Npad * sizeof(float4));
-    st64_.reserve((size_t)Npad * sizeof(double2));
-    hipLaunchKernelGGL(k_derive, dim3((unsigned)((Npad + 255) / 256)), dim3(256), 0, stream_,
-                       s.xp(), (const double2 *)s.P.p, (const double *)s.refs.p, N, Npad, m,
-                       (float4 *)colv_.p, (double2 *)st64_.p);
+    for (int q = 0; q < B.nser; q++) B.host[(size_t)q].N = B.n[(size_t)q] - m + 1;
+    upload_ser(B);
+    colv_.reserve((size_t)B.rows * sizeof(float4));
+    st64_.reserve((size_t)B.rows * sizeof(double2));
+    HIPC(hipEventRecord(ev_[4], stream_));
+    hipLaunchKernelGGL(k_derive, dim3(grid1(B.rows, 256)), dim3(256), 0, stream_,
+                       B.x.as<float>(), B.ser.as<SerDev>(), B.bser.as<int>(), B.P.as<double2>(),
+                       B.refs.as<double>(), B.rows, m, colv_.as<float4>(), st64_.as<double2>());
     HIPC(hipGetLastError());
+    HIPC(hipEventRecord(ev_[5], stream_));
+    if (st) {
+        HIPC(hipEventSynchronize(ev_[5]));
+        float a = 0.f;
+        HIPC(hipEventElapsedTime(&a, ev_[4], ev_[5]));
+        st->kernel_ms[0] += a;
+        st->launches[0] += 1;
+    }
 }
 
-static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
-
-// One (m, r2) pass: returns the survivors (index, single-precision squared NN distance).
-void Engine::pass(Series &s, int m, double r2, std::vector<Cand> &out, paddi_stats *st)
+void Engine::copy_st64(double2 *dst, int64_t N)
 {
-    const int64_t N = s.n - m + 1;
-    const int64_t nwords = (N + 63) / 64;
-    alive_.reserve((size_t)nwords * 8);
-    best_.reserve((size_t)N * 4);
-    blist_.reserve((size_t)nwords * 4);
-    cnt_.reserve(16);
-    sidx_.reserve((size_t)N * 8);
-    sd2_.reserve((size_t)N * 4);
+    HIPC(hipMemcpy(dst, st64_.p, (size_t)N * sizeof(double2), hipMemcpyDeviceToHost));
+}
+
+// One (m, r2_s) pass: survivors per series.  Series with r2 < 0 are skipped (no rows).
+void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
+                  std::vector<std::vector<Cand>> &out, paddi_stats *st)
+{
     const double delta = delta_ed2(m);
-    const float thr_rho = (float)(1.0 - (r2 - delta) / (2.0 * m));
-    const int64_t WA = select_window_;
-    const int64_t CH = refine_chunk_;
+    for (int q = 0; q < B.nser; q++) {
+        SerDev &S = B.host[(size_t)q];
+        S.N = r2[(size_t)q] < 0.0 ? 0 : B.n[(size_t)q] - m + 1;
+        S.thr = (float)(1.0 - (r2[(size_t)q] - delta) / (2.0 * m));
+    }
+    upload_ser(B);
+    const int64_t rows = B.rows, nwords = B.nblocks;
+    alive_.reserve((size_t)nwords * 8);
+    best_.reserve((size_t)rows * 4);
+    blist_.reserve((size_t)nwords * 4);
+    cnt_.reserve(32);
+    sidx_.reserve((size_t)rows * 8);
+    sd2_.reserve((size_t)rows * 4);
+    int *wread = nullptr, *wwrite = nullptr;
+    if (witness) {
+        wit_[0].reserve((size_t)rows * 4);
+        wit_[1].reserve((size_t)rows * 4);
+        wread = wit_[witcur_ ^ 1].as<int>();
+        wwrite = wit_[witcur_].as<int>();
+        hipLaunchKernelGGL(k_fill_int, dim3(grid1(rows, 256)), dim3(256), 0, stream_, wwrite,
+                           rows, -1);
+    }
+    ScanArgs A{};
+    A.x = B.x.as<float>();
+    A.ser = B.ser.as<SerDev>();
+    A.bser = B.bser.as<int>();
+    A.colv = colv_.as<float4>();
+    A.alive = alive_.as<uint64_t>();
+    A.best = best_.as<int>();
+    A.wit = wwrite;
+    A.wit_read = wread_valid_ ? wread : nullptr;
+    A.pairs = (unsigned long long *)((char *)cnt_.p + 16);
+    A.WA = select_window_;
+    A.CH = refine_chunk_;
+    A.m = m;
     const size_t lds = scan_lds_bytes(m);
 
-    HIPC(hipEventRecord(ev_[0], stream_));
-    hipLaunchKernelGGL(k_fill_alive, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
-                       (uint64_t *)alive_.p, N, nwords);
-    hipLaunchKernelGGL(k_fill_int, dim3(grid1(N, 256)), dim3(256), 0, stream_, (int *)best_.p, N,
+    hipLaunchKernelGGL(k_fill_alive, dim3(grid1(nwords, 256)), dim3(256), 0, stream_, A.alive,
+                       A.ser, A.bser, nwords);
+    hipLaunchKernelGGL(k_fill_int, dim3(grid1(rows, 256)), dim3(256), 0, stream_, A.best, rows,
                        (int)0x807fffff /* ord_key(-inf) */);
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, stream_, (int *)cnt_.p, 4, 0);
-    hipLaunchKernelGGL(k_scan_select, dim3((unsigned)nwords), dim3(64), lds, stream_, s.xp(),
-                       (const float4 *)colv_.p, (uint64_t *)alive_.p, (int *)best_.p, N, m,
-                       thr_rho, WA);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, stream_, cnt_.as<int>(), 8, 0);
+    HIPC(hipEventRecord(ev_[0], stream_));
+    hipLaunchKernelGGL(k_scan_select, dim3((unsigned)nwords), dim3(64), lds, stream_, A);
     HIPC(hipGetLastError());
-    hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
-                       (const uint64_t *)alive_.p, nwords, (int *)blist_.p, (int *)cnt_.p);
-    HIPC(hipMemcpyAsync(pin_, cnt_.p, 4, hipMemcpyDeviceToHost, stream_));
     HIPC(hipEventRecord(ev_[1], stream_));
+    hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
+                       (const uint64_t *)A.alive, nwords, blist_.as<int>(), cnt_.as<int>());
+    HIPC(hipMemcpyAsync(pin_, cnt_.p, 4, hipMemcpyDeviceToHost, stream_));
     HIPC(hipStreamSynchronize(stream_));
     const int nblk = ((int *)pin_)[0];
-    const int nchunks = (int)((N + CH - 1) / CH);
+    int64_t nmax = 0;
+    for (int q = 0; q < B.nser; q++) nmax = std::max(nmax, B.host[(size_t)q].N);
+    const int nchunks = (int)std::max<int64_t>(1, (nmax + A.CH - 1) / A.CH);
+    HIPC(hipEventRecord(ev_[2], stream_));
     if (nblk > 0) {
         hipLaunchKernelGGL(k_scan_refine, dim3((unsigned)((int64_t)nblk * nchunks)), dim3(64), lds,
-                           stream_, s.xp(), (const float4 *)colv_.p, (uint64_t *)alive_.p,
-                           (int *)best_.p, (const int *)blist_.p, N, m, thr_rho, WA, CH, nchunks);
+                           stream_, A, (const int *)blist_.p, nchunks);
         HIPC(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_survivors, dim3(grid1(N, 256)), dim3(256), 0, stream_,
-                       (const uint64_t *)alive_.p, (const int *)best_.p, N, m, (int *)cnt_.p + 1,
-                       (int64_t *)sidx_.p, (float *)sd2_.p, N);
+    HIPC(hipEventRecord(ev_[3], stream_));
+    hipLaunchKernelGGL(k_survivors, dim3(grid1(rows, 256)), dim3(256), 0, stream_,
+                       (const uint64_t *)A.alive, (const int *)A.best, rows, m,
+                       cnt_.as<int>() + 1, sidx_.as<int64_t>(), sd2_.as<float>(), rows);
     HIPC(hipGetLastError());
-    HIPC(hipMemcpyAsync((int *)pin_ + 1, (int *)cnt_.p + 1, 4, hipMemcpyDeviceToHost, stream_));
-    HIPC(hipEventRecord(ev_[2], stream_));
+    HIPC(hipMemcpyAsync(pin_, cnt_.p, 32, hipMemcpyDeviceToHost, stream_));
     HIPC(hipStreamSynchronize(stream_));
     const int ns = ((int *)pin_)[1];
-    out.resize((size_t)ns);
+    out.assign((size_t)B.nser, {});
     if (ns > 0) {
         std::vector<int64_t> ix((size_t)ns);
         std::vector<float> dd((size_t)ns);
         HIPC(hipMemcpyAsync(ix.data(), sidx_.p, (size_t)ns * 8, hipMemcpyDeviceToHost, stream_));
         HIPC(hipMemcpyAsync(dd.data(), sd2_.p, (size_t)ns * 4, hipMemcpyDeviceToHost, stream_));
         HIPC(hipStreamSynchronize(stream_));
-        for (int q = 0; q < ns; q++) out[(size_t)q] = Cand{ix[(size_t)q], dd[(size_t)q], 0.0, false};
+        for (int q = 0; q < ns; q++) {
+            const int64_t g = ix[(size_t)q];
+            const int sid = B.bser_host[(size_t)(g >> 6)];
+            out[(size_t)sid].push_back(Cand{g - B.roff[(size_t)sid], dd[(size_t)q], 0.0, false});
+        }
     }
     if (st) {
         float a = 0.f, b = 0.f;
         HIPC(hipEventElapsedTime(&a, ev_[0], ev_[1]));
-        HIPC(hipEventElapsedTime(&b, ev_[1], ev_[2]));
+        HIPC(hipEventElapsedTime(&b, ev_[2], ev_[3]));
         st->kernel_ms[1] += a;
-        st->kernel_ms[2] += b;
+        st->launches[1] += 1;
+        if (nblk > 0) {
+            st->kernel_ms[2] += b;
+            st->launches[2] += 1;
+        }
         st->attempts += 1;
         st->survivors += ns;
+        st->pairs += (int64_t)((unsigned long long *)pin_)[2];
     }
 }
 
-// Exact (double) NN distance of the given candidates.
-void Engine::exact(Series &s, int m, const std::vector<int64_t> &idx, std::vector<double> &out,
-                   paddi_stats *st)
+// Exact (double) NN distance of (series, local row) requests.
+void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &req,
+                   std::vector<double> &out, paddi_stats *st)
 {
-    const int64_t N = s.n - m + 1;
-    out.assign(idx.size(), 0.0);
+    out.assign(req.size(), 0.0);
+    if (req.empty()) return;
+    // restore N of every series (a pass may have switched series off)
+    for (int q = 0; q < B.nser; q++) B.host[(size_t)q].N = B.n[(size_t)q] - m + 1;
+    upload_ser(B);
     const int umax = exact_max_u(m);
-    cand_.reserve(MAXU * 8);
-    okey_.reserve(MAXU * 8);
-    for (size_t b0 = 0; b0 < idx.size(); b0 += (size_t)umax) {
-        const int nU = (int)std::min<size_t>((size_t)umax, idx.size() - b0);
-        HIPC(hipMemcpyAsync(cand_.p, idx.data() + b0, (size_t)nU * 8, hipMemcpyHostToDevice,
-                            stream_));
-        HIPC(hipMemsetAsync(okey_.p, 0xff, (size_t)nU * 8, stream_));
-        HIPC(hipEventRecord(ev_[0], stream_));
-        hipLaunchKernelGGL(k_exact, dim3(grid1(N, 256)), dim3(256), exact_lds_bytes(m, nU),
-                           stream_, s.xp(), (const double2 *)st64_.p, N, m,
-                           (const int64_t *)cand_.p, nU, (unsigned long long *)okey_.p);
-        HIPC(hipGetLastError());
-        HIPC(hipEventRecord(ev_[1], stream_));
-        unsigned long long keys[MAXU];
-        HIPC(hipMemcpyAsync(keys, okey_.p, (size_t)nU * 8, hipMemcpyDeviceToHost, stream_));
-        HIPC(hipStreamSynchronize(stream_));
-        for (int u = 0; u < nU; u++) {
-            unsigned long long k = keys[u];
-            unsigned long long bits = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
-            double d;
-            std::memcpy(&d, &bits, 8);
-            out[b0 + (size_t)u] = d;
+    std::vector<size_t> order(req.size());
+    for (size_t q = 0; q < req.size(); q++) order[q] = q;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return req[a].first < req[b].first; });
+    std::vector<int64_t> cand;
+    std::vector<int> gfirst, gser;
+    std::vector<int64_t> goff;
+    int64_t blocks = 0;
+    for (size_t q = 0; q < order.size();) {
+        const int sid = req[order[q]].first;
+        gfirst.push_back((int)cand.size());
+        gser.push_back(sid);
+        goff.push_back(blocks);
+        blocks += (B.n[(size_t)sid] - m + 1 + 255) / 256;
+        int u = 0;
+        while (q < order.size() && req[order[q]].first == sid && u < umax) {
+            cand.push_back(req[order[q]].second);
+            q++;
+            u++;
         }
-        if (st) {
-            float a = 0.f;
-            HIPC(hipEventElapsedTime(&a, ev_[0], ev_[1]));
-            st->kernel_ms[3] += a;
-            st->exact_refines += nU;
-        }
+    }
+    gfirst.push_back((int)cand.size());
+    const int ng = (int)gser.size();
+    cand_.reserve(cand.size() * 8);
+    okey_.reserve(cand.size() * 8);
+    gfirst_.reserve(gfirst.size() * 4);
+    gser_.reserve(gser.size() * 4);
+    goff_.reserve(goff.size() * 8);
+    HIPC(hipMemcpyAsync(cand_.p, cand.data(), cand.size() * 8, hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(gfirst_.p, gfirst.data(), gfirst.size() * 4, hipMemcpyHostToDevice,
+                        stream_));
+    HIPC(hipMemcpyAsync(gser_.p, gser.data(), gser.size() * 4, hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(goff_.p, goff.data(), goff.size() * 8, hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemsetAsync(okey_.p, 0xff, cand.size() * 8, stream_));
+    ExactArgs E{};
+    E.x = B.x.as<float>();
+    E.ser = B.ser.as<SerDev>();
+    E.st64 = st64_.as<double2>();
+    E.cand = cand_.as<int64_t>();
+    E.gfirst = gfirst_.as<int>();
+    E.gser = gser_.as<int>();
+    E.goff = goff_.as<int64_t>();
+    E.outkey = okey_.as<unsigned long long>();
+    E.ngroups = ng;
+    E.m = m;
+    HIPC(hipEventRecord(ev_[6], stream_));
+    hipLaunchKernelGGL(k_exact, dim3((unsigned)blocks), dim3(256), exact_lds_bytes(m, umax),
+                       stream_, E);
+    HIPC(hipGetLastError());
+    HIPC(hipEventRecord(ev_[7], stream_));
+    std::vector<unsigned long long> keys(cand.size());
+    HIPC(hipMemcpyAsync(keys.data(), okey_.p, cand.size() * 8, hipMemcpyDeviceToHost, stream_));
+    HIPC(hipStreamSynchronize(stream_));
+    for (size_t q = 0; q < order.size(); q++) {
+        const unsigned long long k = keys[q];
+        const unsigned long long bits = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+        double d;
+        std::memcpy(&d, &bits, 8);
+        out[order[q]] = d;
+    }
+    if (st) {
+        float a = 0.f;
+        HIPC(hipEventElapsedTime(&a, ev_[6], ev_[7]));
+        st->kernel_ms[3] += a;
+        st->launches[3] += 1;
+        st->exact_refines += (int64_t)cand.size();
     }
 }
 
-// Exact greedy top-k over the survivors (see file header).  Returns the accepted discords in
-// rank order; stops early when the next exact distance is <= r2.
-void Engine::select_topk(Series &s, int m, double r2, int k, std::vector<Cand> &cands,
-                         std::vector<Cand> &acc, paddi_stats *st)
+void Engine::topk_all(Batch &B, int m, std::vector<TopK> &tk, paddi_stats *st)
 {
-    const double delta = delta_ed2(m);
-    acc.clear();
-    std::vector<char> gone(cands.size(), 0);
-    auto lo = [&](const Cand &c) { return c.exact ? c.v : (double)c.a - delta; };
-    auto hi = [&](const Cand &c) { return c.exact ? c.v : (double)c.a + delta; };
-    auto is_inf = [&](const Cand &c) { return !c.exact && c.a >= 3.0e38f; };
-    while ((int)acc.size() < k) {
-        // top by upper bound (ties: smaller index)
-        int64_t T = -1;
-        for (size_t q = 0; q < cands.size(); q++) {
-            if (gone[q]) continue;
-            if (T < 0) { T = (int64_t)q; continue; }
-            const Cand &a = cands[q], &b = cands[(size_t)T];
-            const double ha = is_inf(a) ? INFINITY : hi(a), hb = is_inf(b) ? INFINITY : hi(b);
-            if (ha > hb || (ha == hb && a.i < b.i)) T = (int64_t)q;
+    for (;;) {
+        std::vector<std::pair<int, int64_t>> req;
+        std::vector<std::pair<int, size_t>> who;
+        std::vector<size_t> need;
+        for (int q = 0; q < B.nser; q++) {
+            TopK &T = tk[(size_t)q];
+            if (T.done) continue;
+            if (T.advance(need))
+                for (size_t c : need) {
+                    req.emplace_back(q, T.cands[c].i);
+                    who.emplace_back(q, c);
+                }
         }
-        if (T < 0) break;
-        const double loT = is_inf(cands[(size_t)T]) ? INFINITY : lo(cands[(size_t)T]);
-        std::vector<size_t> U;
-        for (size_t q = 0; q < cands.size(); q++) {
-            if (gone[q]) continue;
-            const double h = is_inf(cands[q]) ? INFINITY : hi(cands[q]);
-            if (h >= loT) U.push_back(q);
+        if (req.empty()) break;
+        std::vector<double> v;
+        exact(B, m, req, v, st);
+        for (size_t u = 0; u < who.size(); u++) {
+            Cand &c = tk[(size_t)who[u].first].cands[who[u].second];
+            c.exact = true;
+            c.v = v[u];
         }
-        std::vector<int64_t> need;
-        std::vector<size_t> needq;
-        for (size_t q : U)
-            if (!cands[q].exact) { need.push_back(cands[q].i); needq.push_back(q); }
-        if (!need.empty()) {
-            std::vector<double> v;
-            exact(s, m, need, v, st);
-            for (size_t u = 0; u < need.size(); u++) {
-                cands[needq[u]].exact = true;
-                cands[needq[u]].v = v[u];
-            }
-            continue;
-        }
-        size_t B = U[0];
-        for (size_t q : U) {
-            const Cand &a = cands[q], &b = cands[B];
-            if (a.v > b.v || (a.v == b.v && a.i < b.i)) B = q;
-        }
-        if (!(cands[B].v > r2)) break;
-        acc.push_back(cands[B]);
-        const int64_t bi = cands[B].i;
-        for (size_t q = 0; q < cands.size(); q++)
-            if (!gone[q] && std::llabs(cands[q].i - bi) < m) gone[q] = 1;
     }
 }
 
@@ -317,18 +512,28 @@ static double next_r_after_fail(int j, double r, double S)
     return 0.99 * r;
 }
 
-// PALMAD length loop with the r-schedule of src/main.cu:136-233 (guards as in the oracle).
-void Engine::palmad(Series &s, int minL, int maxL, int step, int k, const paddi_opts &o,
-                    std::vector<paddi_discord> &out, paddi_stats *st, int series_id)
+static void emit(std::vector<paddi_discord> &out, const std::vector<Cand> &acc, int m, int sid,
+                 double r2)
 {
-    prefix(s);
+    for (const Cand &c : acc)
+        out.push_back(paddi_discord{c.i, m, sid, c.v >= 3.0e38 ? 3.4028235e38f : (float)c.v,
+                                    (float)r2});
+}
+
+// The plain form for series sid: src/main.cu:136-233 (guards as in the oracle).
+void Engine::plain(Batch &B, int sid, int minL, int maxL, int step, int k, const paddi_opts &o,
+                   std::vector<paddi_discord> &out, paddi_stats *st)
+{
     const int nlen = (maxL - minL) / step + 1;
     std::vector<double> dk((size_t)nlen, 0.0);
-    std::vector<Cand> cands, acc;
+    std::vector<std::vector<Cand>> surv;
+    std::vector<TopK> tk((size_t)B.nser);
+    std::vector<double> r2v((size_t)B.nser, -1.0);
     const int max_att = o.max_attempts > 0 ? o.max_attempts : 64;
+    wread_valid_ = false;
     for (int j = 0; j < nlen; j++) {
         const int m = minL + j * step;
-        derive(s, m);
+        derive(B, m, st);
         double r, S = 0.0;
         if (j == 0) {
             r = 2.0 * std::sqrt((double)minL);
@@ -350,20 +555,127 @@ void Engine::palmad(Series &s, int minL, int maxL, int step, int k, const paddi_
             if (r < 0.0) r = 0.0;
             if (att >= max_att) r = 0.0;
             const double r2 = r * r;
-            pass(s, m, r2, cands, st);
-            select_topk(s, m, r2, k, cands, acc, st);
+            r2v[(size_t)sid] = r2;
+            pass(B, m, r2v, false, surv, st);
+            tk[(size_t)sid].reset(std::move(surv[(size_t)sid]), r2, m, k);
+            topk_all(B, m, tk, st);
             att++;
+            const std::vector<Cand> &acc = tk[(size_t)sid].acc;
             if ((int)acc.size() >= k || r == 0.0) {
-                for (const Cand &c : acc)
-                    out.push_back(paddi_discord{c.i, m, series_id,
-                                                c.v >= 3.0e38 ? 3.4028235e38f : (float)c.v,
-                                                (float)r2});
+                emit(out, acc, m, sid, r2);
                 dk[(size_t)j] = acc.empty() ? 0.0 : acc.back().v;
                 break;
             }
             r = next_r_after_fail(j, r, S);
         }
     }
+}
+
+// The fast form: every series in every launch (see file header).
+void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opts &o,
+                  std::vector<paddi_discord> &out, paddi_stats *st)
+{
+    const int nlen = (maxL - minL) / step + 1;
+    const int ns = B.nser;
+    const int max_att = o.max_attempts > 0 ? o.max_attempts : 64;
+    const bool learned = o.mode == PADDI_MODE_LEARNED;
+    std::vector<std::vector<int64_t>> prev((size_t)ns);
+    std::vector<std::vector<paddi_discord>> res((size_t)ns);
+    std::vector<std::vector<Cand>> surv;
+    std::vector<TopK> tk((size_t)ns);
+    wread_valid_ = false;
+    for (int j = 0; j < nlen; j++) {
+        const int m = minL + j * step;
+        derive(B, m, st);
+        std::vector<double> r2((size_t)ns, 0.0);
+        std::vector<char> from_prev((size_t)ns, 0);
+        // r from the previous length's discords, evaluated exactly at this length
+        {
+            std::vector<std::pair<int, int64_t>> req;
+            for (int q = 0; q < ns; q++)
+                if ((int)prev[(size_t)q].size() >= k)
+                    for (int64_t i : prev[(size_t)q])
+                        if (i <= B.n[(size_t)q] - m) req.emplace_back(q, i);
+            std::vector<double> v;
+            exact(B, m, req, v, st);
+            std::vector<double> mn((size_t)ns, INFINITY);
+            std::vector<int> cnt((size_t)ns, 0);
+            for (size_t u = 0; u < req.size(); u++) {
+                mn[(size_t)req[u].first] = std::min(mn[(size_t)req[u].first], v[u]);
+                cnt[(size_t)req[u].first]++;
+            }
+            for (int q = 0; q < ns; q++)
+                if (cnt[(size_t)q] >= k && std::isfinite(mn[(size_t)q]) && mn[(size_t)q] > 0.0) {
+                    r2[(size_t)q] = mn[(size_t)q] * (1.0 - 1e-6);
+                    from_prev[(size_t)q] = 1;
+                }
+        }
+        // first length (or no usable previous result): learned proposal or probe descent
+        std::vector<double> prop((size_t)ns, 0.0);
+        bool need_prop = false;
+        for (int q = 0; q < ns; q++) need_prop |= !from_prev[(size_t)q];
+        if (need_prop && learned) propose_r2(B, m, k, prop);
+        for (int q = 0; q < ns; q++)
+            if (!from_prev[(size_t)q])
+                r2[(size_t)q] = (learned && prop[(size_t)q] > 0.0) ? prop[(size_t)q] : 2.0 * m;
+        std::vector<char> active((size_t)ns, 1);
+        std::vector<int> att((size_t)ns, 0);
+        for (;;) {
+            std::vector<double> r2p((size_t)ns, -1.0);
+            bool any = false;
+            for (int q = 0; q < ns; q++)
+                if (active[(size_t)q]) {
+                    if (att[(size_t)q] >= max_att) r2[(size_t)q] = 0.0;
+                    r2p[(size_t)q] = r2[(size_t)q];
+                    any = true;
+                }
+            if (!any) break;
+            pass(B, m, r2p, true, surv, st);
+            for (int q = 0; q < ns; q++)
+                if (active[(size_t)q]) tk[(size_t)q].reset(std::move(surv[(size_t)q]), r2[(size_t)q], m, k);
+                else tk[(size_t)q].done = true;
+            topk_all(B, m, tk, st);
+            for (int q = 0; q < ns; q++) {
+                if (!active[(size_t)q]) continue;
+                att[(size_t)q]++;
+                const std::vector<Cand> &acc = tk[(size_t)q].acc;
+                if ((int)acc.size() >= k || r2[(size_t)q] <= 0.0) {
+                    emit(res[(size_t)q], acc, m, q, r2[(size_t)q]);
+                    prev[(size_t)q].clear();
+                    for (const Cand &c : acc) prev[(size_t)q].push_back(c.i);
+                    active[(size_t)q] = 0;
+                } else {
+                    // below the k-th distance the answer is the same: step r down
+                    r2[(size_t)q] *= from_prev[(size_t)q] ? 0.9 : 0.85;
+                    if (r2[(size_t)q] < 1e-9) r2[(size_t)q] = 0.0;
+                }
+            }
+        }
+        // this length's witnesses become the next length's read side
+        wread_valid_ = true;
+        witcur_ ^= 1;
+    }
+    for (int q = 0; q < ns; q++) out.insert(out.end(), res[(size_t)q].begin(), res[(size_t)q].end());
+}
+
+void Engine::run(const std::vector<const Series *> &ss, int minL, int maxL, int step, int k,
+                 const paddi_opts &o, std::vector<paddi_discord> &out, paddi_stats *st)
+{
+    if (o.mode == PADDI_MODE_PLAIN) {
+        // the reference's method looped over series, each alone
+        for (size_t q = 0; q < ss.size(); q++) {
+            build_batch(batch_, {ss[q]}, minL);
+            prefix(batch_);
+            std::vector<paddi_discord> r;
+            plain(batch_, 0, minL, maxL, step, k, o, r, st);
+            for (paddi_discord &d : r) d.series = (int32_t)q;
+            out.insert(out.end(), r.begin(), r.end());
+        }
+        return;
+    }
+    build_batch(batch_, ss, minL);
+    prefix(batch_);
+    fast(batch_, minL, maxL, step, k, o, out, st);
 }
 
 }  // namespace paddi
@@ -401,7 +713,7 @@ static bool valid_lengths(int64_t n, int32_t minL, int32_t maxL, int32_t step, i
 extern "C" {
 
 const char *paddi_status_str(paddi_status s) { return paddi::status_str(s); }
-const char *paddi_version(void) { return "paddi-mi355x 0.1 (gfx950)"; }
+const char *paddi_version(void) { return "paddi-mi355x 0.2 (gfx950)"; }
 
 paddi_status paddi_create(int32_t device, paddi_ctx **out)
 {
@@ -455,8 +767,10 @@ paddi_status paddi_compute_statistics(paddi_ctx *ctx, const float *T, int64_t n,
     return guarded(ctx, [&] {
         paddi::Engine &E = *ctx->eng;
         std::unique_ptr<paddi::Series> s(E.upload(T, n));
-        E.prefix(*s);
-        E.derive(*s, m);
+        paddi::Batch B;
+        E.build_batch(B, {s.get()}, m);
+        E.prefix(B);
+        E.derive(B, m, nullptr);
         const int64_t N = n - m + 1;
         std::vector<double2> st((size_t)N);
         E.copy_st64(st.data(), N);
@@ -478,30 +792,32 @@ paddi_status paddi_range_discords(paddi_ctx *ctx, const float *T, int64_t n, int
     return guarded(ctx, [&] {
         paddi::Engine &E = *ctx->eng;
         std::unique_ptr<paddi::Series> s(E.upload(T, n));
-        E.prefix(*s);
-        E.derive(*s, m);
+        paddi::Batch B;
+        E.build_batch(B, {s.get()}, m);
+        E.prefix(B);
+        E.derive(B, m, nullptr);
         const int64_t N = n - m + 1;
-        std::vector<paddi::Cand> cands;
-        E.pass(*s, m, (double)r2, cands, nullptr);
+        std::vector<std::vector<paddi::Cand>> surv;
+        E.pass(B, m, {(double)r2}, false, surv, nullptr);
         std::fill(is_discord, is_discord + N, 0);
         std::fill(nn_dist2, nn_dist2 + N, 3.4028235e38f);
         const double delta = paddi::delta_ed2(m);
-        std::vector<int64_t> amb;
-        for (const paddi::Cand &c : cands) {
+        std::vector<std::pair<int, int64_t>> amb;
+        for (const paddi::Cand &c : surv[0]) {
             if (c.a >= 3.0e38f || (double)c.a > (double)r2 + delta) {
                 is_discord[c.i] = 1;
                 nn_dist2[c.i] = c.a;
             } else if ((double)c.a > (double)r2 - delta) {
-                amb.push_back(c.i);
+                amb.emplace_back(0, c.i);
             }
         }
         if (!amb.empty()) {
             std::vector<double> v;
-            E.exact(*s, m, amb, v, nullptr);
+            E.exact(B, m, amb, v, nullptr);
             for (size_t q = 0; q < amb.size(); q++)
                 if (v[q] > (double)r2) {
-                    is_discord[amb[q]] = 1;
-                    nn_dist2[amb[q]] = (float)v[q];
+                    is_discord[amb[q].second] = 1;
+                    nn_dist2[amb[q].second] = (float)v[q];
                 }
         }
         return PADDI_OK;
@@ -514,19 +830,22 @@ paddi_status paddi_discords_series(paddi_ctx *ctx, paddi_series *const *series, 
                                    int64_t *count, paddi_stats *stats)
 {
     if (!ctx || !series || nseries <= 0 || !count || (!out && cap > 0)) return PADDI_ERR_INVALID;
-    for (int32_t q = 0; q < nseries; q++)
+    std::vector<const paddi::Series *> ss;
+    for (int32_t q = 0; q < nseries; q++) {
         if (!series[q] || series[q]->ctx != ctx ||
             !valid_lengths(series[q]->s->n, minL, maxL, step, top_k))
             return PADDI_ERR_INVALID;
+        ss.push_back(series[q]->s);
+    }
     if (maxL > paddi::MAXM) return PADDI_ERR_UNSUPPORTED;
     paddi_opts o{};
     if (opts) o = *opts;
+    if (o.mode < PADDI_MODE_PLAIN || o.mode > PADDI_MODE_LEARNED) return PADDI_ERR_INVALID;
     if (stats) std::memset(stats, 0, sizeof(*stats));
     const auto t0 = std::chrono::steady_clock::now();
     paddi_status st = guarded(ctx, [&] {
         std::vector<paddi_discord> res;
-        for (int32_t q = 0; q < nseries; q++)
-            ctx->eng->run(*series[q]->s, minL, maxL, step, top_k, o, res, stats, q);
+        ctx->eng->run(ss, minL, maxL, step, top_k, o, res, stats);
         *count = (int64_t)res.size();
         if ((int64_t)res.size() > cap) return PADDI_ERR_CAPACITY;
         std::copy(res.begin(), res.end(), out);
@@ -568,6 +887,13 @@ paddi_status paddi_discords(paddi_ctx *ctx, const float *T, int64_t n, int32_t m
     const int64_t ns[1] = {n};
     return paddi_discords_batch(ctx, Ts, ns, 1, minL, maxL, step, top_k, opts, out, cap, count,
                                 stats);
+}
+
+int32_t paddi_has_mode(int32_t mode)
+{
+    if (mode == PADDI_MODE_PLAIN || mode == PADDI_MODE_BATCHED) return 1;
+    if (mode == PADDI_MODE_LEARNED) return paddi::learned_available() ? 1 : 0;
+    return 0;
 }
 
 }  // extern "C"

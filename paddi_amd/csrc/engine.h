@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/paddi.h"
+#include "kernels.h"
 
 namespace paddi {
 
@@ -19,18 +20,17 @@ struct DevBuf {
     size_t bytes = 0;
     void reserve(size_t nbytes);
     void release();
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
+    template <class T> T *as() const { return (T *)p; }
 };
 
-// A series resident in HBM with its prefix sums.
+// A series uploaded to HBM (paddi_series_upload).
 struct Series {
     int64_t n = 0;
-    int64_t nb = 0;            // prefix blocks
-    DevBuf x;                  // n + XPAD floats, zero padded
-    DevBuf P;                  // nb * (PB+1) double2
-    DevBuf refs;               // nb doubles
-    bool prefix_ready = false;
-    const float *xp() const { return (const float *)x.p; }
+    DevBuf x;  // n floats
 };
 
 // Surviving row: single-precision distance a, exact distance v once computed.
@@ -41,6 +41,31 @@ struct Cand {
     bool exact;
 };
 
+// Exact greedy top-k of one series at one (m, r2) (see engine.cpp).
+struct TopK {
+    std::vector<Cand> cands;
+    std::vector<char> gone;
+    std::vector<Cand> acc;
+    double r2 = 0.0, delta = 0.0;
+    int m = 0, k = 0;
+    bool done = false;
+    void reset(std::vector<Cand> &&c, double r2_, int m_, int k_);
+    // Advances the greedy; returns true with `need` (indices into cands) when exact values
+    // are required before it can continue.
+    bool advance(std::vector<size_t> &need);
+};
+
+// Series of one call laid out in a shared global row space.
+struct Batch {
+    int nser = 0;
+    int minL = 0;
+    std::vector<int64_t> n, xoff, roff, pboff;
+    int64_t xtotal = 0, rows = 0, nblocks = 0, npb = 0;
+    std::vector<SerDev> host;   // per series, N/thr updated per length
+    DevBuf x, P, refs, ser, bser, pser;
+    std::vector<int> bser_host;
+};
+
 class Engine {
 public:
     explicit Engine(int device);
@@ -49,34 +74,43 @@ public:
     Engine &operator=(const Engine &) = delete;
 
     Series *upload(const float *T, int64_t n);
-    void prefix(Series &s);
-    void derive(Series &s, int m);
-    void pass(Series &s, int m, double r2, std::vector<Cand> &out, paddi_stats *st);
-    void exact(Series &s, int m, const std::vector<int64_t> &idx, std::vector<double> &out,
-               paddi_stats *st);
-    void select_topk(Series &s, int m, double r2, int k, std::vector<Cand> &cands,
-                     std::vector<Cand> &acc, paddi_stats *st);
-    void palmad(Series &s, int minL, int maxL, int step, int k, const paddi_opts &o,
-                std::vector<paddi_discord> &out, paddi_stats *st, int series_id);
-    void run(Series &s, int minL, int maxL, int step, int k, const paddi_opts &o,
-             std::vector<paddi_discord> &out, paddi_stats *st, int series_id)
-    {
-        palmad(s, minL, maxL, step, k, o, out, st, series_id);
-    }
-    void copy_st64(double2 *dst, int64_t N)
-    {
-        (void)hipMemcpy(dst, st64_.p, (size_t)N * sizeof(double2), hipMemcpyDeviceToHost);
-    }
+    void build_batch(Batch &B, const std::vector<const Series *> &ss, int minL);
+    void prefix(Batch &B);
+    void derive(Batch &B, int m, paddi_stats *st);
+    // One (m, r2_s) pass over every series of the batch; survivors per series.
+    void pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
+              std::vector<std::vector<Cand>> &out, paddi_stats *st);
+    // Exact NN ED^2 of (series, local row) requests at length m (derive(m) must be current).
+    void exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &req,
+               std::vector<double> &out, paddi_stats *st);
+    // Run the exact top-k of every series to completion (batched exact refines).
+    void topk_all(Batch &B, int m, std::vector<TopK> &tk, paddi_stats *st);
+
+    void run(const std::vector<const Series *> &ss, int minL, int maxL, int step, int k,
+             const paddi_opts &o, std::vector<paddi_discord> &out, paddi_stats *st);
+    void copy_st64(double2 *dst, int64_t N);
 
     int64_t select_window_ = 8192;   // columns right of a block scanned by k_scan_select
     int64_t refine_chunk_ = 65536;   // columns per k_scan_refine work item
 
 private:
+    void plain(Batch &B, int sid, int minL, int maxL, int step, int k, const paddi_opts &o,
+               std::vector<paddi_discord> &out, paddi_stats *st);
+    void fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opts &o,
+              std::vector<paddi_discord> &out, paddi_stats *st);
+    void upload_ser(Batch &B);
+    // Learned proposal of the squared range per series at length m (0 = no proposal).
+    void propose_r2(Batch &B, int m, int k, std::vector<double> &out);
+
+    Batch batch_;
+    bool wread_valid_ = false;   // the read-side witness buffer holds the previous length
     int device_;
     hipStream_t stream_ = nullptr;
-    hipEvent_t ev_[4] = {};
+    hipEvent_t ev_[8] = {};
     void *pin_ = nullptr;
-    DevBuf colv_, st64_, alive_, best_, blist_, cnt_, sidx_, sd2_, cand_, okey_;
+    DevBuf colv_, st64_, alive_, best_, wit_[2], blist_, cnt_, sidx_, sd2_;
+    DevBuf cand_, okey_, gfirst_, gser_, goff_;
+    int witcur_ = 0;
 };
 
 }  // namespace paddi

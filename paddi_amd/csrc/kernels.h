@@ -1,4 +1,4 @@
-// Kernel declarations and engine-wide constants (gfx950).
+// Kernel declarations, argument blocks and engine-wide constants (gfx950).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -7,43 +7,78 @@ namespace paddi {
 
 constexpr int PB = 4096;       // prefix-sum block (points); also the largest supported m
 constexpr int MAXM = 2048;     // largest subsequence length the scan kernels accept
-constexpr int MAXU = 8;        // candidates per exact-refine launch
-constexpr int64_t XPAD = 1024; // zero padding after the series (floats)
+constexpr int MAXU = 8;        // candidates per exact-refine group
+constexpr int64_t XPAD = 1024; // zero padding after each series (floats)
 
-__global__ void k_prefix(const float *__restrict__ x, int64_t n, double2 *__restrict__ P,
+// One series of a batch, as the kernels see it (rebuilt per length: N and thr change).
+struct SerDev {
+    int64_t xoff;   // offset of x_s in the concatenated series buffer
+    int64_t n;      // points
+    int64_t roff;   // first global row (multiple of 64)
+    int64_t pboff;  // first prefix block
+    int64_t N;      // windows at this length, n - m + 1
+    float thr;      // prune when correlation >= thr (= 1 - (r2 - delta)/(2m))
+    int32_t pad;
+};
+
+struct ScanArgs {
+    const float *x;
+    const SerDev *ser;
+    const int *bser;          // global block -> series (-1 = padding block)
+    const float4 *colv;       // global rows: (df, dg, invn, mu) in single precision
+    uint64_t *alive;          // one word per global block
+    int *best;                // global rows: ord_key of the max correlation
+    int *wit;                 // global rows: pruning tile recorded at this length (or null)
+    const int *wit_read;      // global rows: tiles recorded at the previous length (or null)
+    unsigned long long *pairs;
+    int64_t WA;               // columns right of a block scanned by k_scan_select
+    int64_t CH;               // columns per k_scan_refine work item
+    int m;
+};
+
+struct ExactArgs {
+    const float *x;
+    const SerDev *ser;
+    const double2 *st64;      // global rows: (mu, invn) in double precision
+    const int64_t *cand;      // local row index of each candidate
+    const int *gfirst;        // group g holds candidates [gfirst[g], gfirst[g+1])
+    const int *gser;          // series of group g
+    const int64_t *goff;      // first block of group g (prefix sums of 256-column chunks)
+    unsigned long long *outkey;
+    int ngroups;
+    int m;
+};
+
+__global__ void k_prefix(const float *__restrict__ x, const SerDev *__restrict__ ser,
+                         const int *__restrict__ pser, double2 *__restrict__ P,
                          double *__restrict__ refs);
-__global__ void k_derive(const float *__restrict__ x, const double2 *__restrict__ P,
-                         const double *__restrict__ refs, int64_t N, int64_t Npad, int m,
+__global__ void k_derive(const float *__restrict__ x, const SerDev *__restrict__ ser,
+                         const int *__restrict__ bser, const double2 *__restrict__ P,
+                         const double *__restrict__ refs, int64_t rows, int m,
                          float4 *__restrict__ colv, double2 *__restrict__ st64);
-__global__ void k_scan_select(const float *__restrict__ x, const float4 *__restrict__ colv,
-                              uint64_t *__restrict__ alive, int *__restrict__ best, int64_t N,
-                              int m, float thr_rho, int64_t WA);
-__global__ void k_scan_refine(const float *__restrict__ x, const float4 *__restrict__ colv,
-                              uint64_t *__restrict__ alive, int *__restrict__ best,
-                              const int *__restrict__ blist, int64_t N, int m, float thr_rho,
-                              int64_t WA, int64_t CH, int nchunks);
-__global__ void k_fill_alive(uint64_t *__restrict__ alive, int64_t N, int64_t nwords);
+__global__ void k_scan_select(ScanArgs A);
+__global__ void k_scan_refine(ScanArgs A, const int *__restrict__ blist, int nchunks);
+__global__ void k_fill_alive(uint64_t *__restrict__ alive, const SerDev *__restrict__ ser,
+                             const int *__restrict__ bser, int64_t nwords);
 __global__ void k_fill_int(int *__restrict__ p, int64_t n, int v);
 __global__ void k_list_blocks(const uint64_t *__restrict__ alive, int64_t nwords,
                               int *__restrict__ list, int *__restrict__ count);
 __global__ void k_survivors(const uint64_t *__restrict__ alive, const int *__restrict__ best,
-                            int64_t N, int m, int *__restrict__ count, int64_t *__restrict__ idx,
-                            float *__restrict__ d2, int64_t cap);
-__global__ void k_exact(const float *__restrict__ x, const double2 *__restrict__ st64, int64_t N,
-                        int m, const int64_t *__restrict__ cand, int nU,
-                        unsigned long long *__restrict__ outkey);
+                            int64_t rows, int m, int *__restrict__ count,
+                            int64_t *__restrict__ idx, float *__restrict__ d2, int64_t cap);
+__global__ void k_exact(ExactArgs E);
 
 // LDS bytes of the scan kernels for length m.
 inline size_t scan_lds_bytes(int m)
 {
     return 64 * 16 + (size_t)((m + 3) & ~3) * 4 + (size_t)(64 + m + 8) * 4;
 }
-// LDS bytes of k_exact for nU candidates of length m.
+// LDS bytes of k_exact for groups of up to nU candidates of length m.
 inline size_t exact_lds_bytes(int m, int nU)
 {
     return ((size_t)nU * m + 256 + m + 8) * 8;
 }
-// Candidates per exact launch that fit 64 KiB of LDS at length m.
+// Candidates per exact group that fit 64 KiB of LDS at length m.
 inline int exact_max_u(int m)
 {
     int u = (int)((65536 / 8 - 264 - m) / m);

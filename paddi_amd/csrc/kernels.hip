@@ -1,18 +1,24 @@
 // Hand-written gfx950 kernels of the discord engine.
 //
-//   k_prefix        blocked, locally centred FP64 prefix sums of the series (once per series)
+//   k_prefix        blocked, locally centred FP64 prefix sums of every series (once per series)
 //   k_derive        per-length window statistics and recurrence terms from the prefix sums
 //                   (replaces gpu_compute_statistics / gpu_update_statistics,
 //                    src/preprocessing.cuh:12-65)
-//   k_scan_select   PD3 candidate selection over a bounded window right of each 64-row block
-//                   (replaces gpu_local_candidate_select + gpu_local_define_candidates,
-//                    src/PD3_kernels.cuh:9-274)
+//   k_scan_select   PD3 candidate selection: each 64-row block scans its witness tiles and a
+//                   bounded window to its right (replaces gpu_local_candidate_select +
+//                   gpu_local_define_candidates, src/PD3_kernels.cuh:9-274)
 //   k_scan_refine   exhaustive scan of surviving blocks over the remaining columns, split in
 //                   column chunks, early abandoning (replaces gpu_local_discords_refine,
 //                   src/PD3_kernels.cuh:276-528, and closes its left-of-segment gap)
 //   k_survivors     compaction of surviving rows (replaces sets_processing.hpp:49-113)
 //   k_exact         double-precision nearest-neighbour distance of a few candidates
 //                   (replaces gpu_multigpus_discords_refine, src/PD3_kernels.cuh:665-759)
+//
+// Batching.  A launch covers every series of a batch.  Series s owns the global rows
+// [roff_s, roff_s + Rpad_s) (roff_s a multiple of 64); its rows at or beyond N_s = n_s - m + 1
+// are padding with zero terms.  bser[] maps a global 64-row block to its series (-1 for a
+// padding block).  Per-row arrays (colv, st64, best, wit) and the alive bit words are indexed
+// by global row; inside a kernel everything is rebased to the series' local indices.
 //
 // Numerics.  The series is never used raw in single precision: every dot product runs on
 // data centred by a local reference, and the diagonal recurrence runs on the mean-centred
@@ -56,29 +62,35 @@ __device__ __forceinline__ unsigned long long ord_key64(double d)
 }
 
 // ---------------------------------------------------------------------------------------
-// Prefix sums.  Block b covers x[b*PB, (b+1)*PB); c_b = x[b*PB].  P[b*(PB+1) + k] holds
-// (sum, sum of squares) of (x - c_b) over x[b*PB, b*PB+k), k = 0..PB (k = PB is the total).
-// Points at or beyond n count as c_b (contribute 0).
+// Prefix sums.  Prefix block b of series s covers x_s[b*PB, (b+1)*PB); c_b = x_s[b*PB].
+// P[(pboff_s + b)*(PB+1) + k] holds (sum, sum of squares) of (x - c_b) over
+// x_s[b*PB, b*PB+k), k = 0..PB (k = PB is the total).  Points at or beyond n_s count as c_b.
 
-__global__ __launch_bounds__(256) void k_prefix(const float *__restrict__ x, int64_t n,
+__global__ __launch_bounds__(256) void k_prefix(const float *__restrict__ x,
+                                                const SerDev *__restrict__ ser,
+                                                const int *__restrict__ pser,
                                                 double2 *__restrict__ P, double *__restrict__ refs)
 {
     const int t = threadIdx.x;
-    const int64_t base = (int64_t)blockIdx.x * PB;
-    const double c = (base < n) ? (double)x[base] : 0.0;
+    const int gb = blockIdx.x;
+    const int sid = pser[gb];
+    const SerDev S = ser[sid];
+    const float *xs = x + S.xoff;
+    const int64_t n = S.n;
+    const int64_t base = (int64_t)(gb - S.pboff) * PB;
+    const double c = (base < n) ? (double)xs[base] : 0.0;
     constexpr int PER = PB / 256;
     double v1[PER], v2[PER];
     double s1 = 0.0, s2 = 0.0;
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         int64_t g = base + (int64_t)t * PER + q;
-        double y = (g < n) ? ((double)x[g] - c) : 0.0;
+        double y = (g < n) ? ((double)xs[g] - c) : 0.0;
         v1[q] = s1;
         v2[q] = s2;
         s1 += y;
         s2 += y * y;
     }
-    // inclusive scan of the thread totals across the wave
     const int lane = t & 63, w = t >> 6;
     double i1 = s1, i2 = s2;
 #pragma unroll
@@ -90,17 +102,17 @@ __global__ __launch_bounds__(256) void k_prefix(const float *__restrict__ x, int
     __shared__ double wt1[4], wt2[4];
     if (lane == 63) { wt1[w] = i1; wt2[w] = i2; }
     __syncthreads();
-    double o1 = i1 - s1, o2 = i2 - s2;  // exclusive within the wave
+    double o1 = i1 - s1, o2 = i2 - s2;
     for (int q = 0; q < w; q++) { o1 += wt1[q]; o2 += wt2[q]; }
-    double2 *Pb = P + (int64_t)blockIdx.x * (PB + 1);
+    double2 *Pb = P + (int64_t)gb * (PB + 1);
 #pragma unroll
     for (int q = 0; q < PER; q++)
         Pb[t * PER + q] = make_double2(o1 + v1[q], o2 + v2[q]);
     if (t == 255) Pb[PB] = make_double2(o1 + s1, o2 + s2);
-    if (t == 0) refs[blockIdx.x] = c;
+    if (t == 0) refs[gb] = c;
 }
 
-// (sum, sum of squares) of (x - c_b) over the window [i, i+m), b = i / PB, m <= PB.
+// (sum, sum of squares) of (x - c_b) over the window [i, i+m) of one series; m <= PB.
 __device__ __forceinline__ double2 window_sums(const double2 *__restrict__ P,
                                                const double *__restrict__ refs, int64_t i, int m,
                                                double *cb)
@@ -136,36 +148,49 @@ __device__ __forceinline__ void window_stats(const double2 *__restrict__ P,
     *invn = (M2 > 1e-13 * (s.y + q) && M2 > 0.0) ? 1.0 / sqrt(M2) : 0.0;
 }
 
-// Per-length terms for positions i in [0, Npad): rows/columns beyond N get zeros.
+// Per-length terms for every global row; padding rows get zeros.
 __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
+                                                const SerDev *__restrict__ ser,
+                                                const int *__restrict__ bser,
                                                 const double2 *__restrict__ P,
-                                                const double *__restrict__ refs, int64_t N,
-                                                int64_t Npad, int m, float4 *__restrict__ colv,
+                                                const double *__restrict__ refs, int64_t rows,
+                                                int m, float4 *__restrict__ colv,
                                                 double2 *__restrict__ st64)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Npad) return;
-    if (i >= N) {
-        colv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        st64[i] = make_double2(0.0, 0.0);
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= rows) return;
+    const int sid = bser[g >> 6];
+    if (sid < 0) {
+        colv[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+        st64[g] = make_double2(0.0, 0.0);
         return;
     }
+    const SerDev S = ser[sid];
+    const int64_t i = g - S.roff;
+    if (i >= S.N) {
+        colv[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+        st64[g] = make_double2(0.0, 0.0);
+        return;
+    }
+    const float *xs = x + S.xoff;
+    const double2 *Ps = P + S.pboff * (PB + 1);
+    const double *rs = refs + S.pboff;
     double mu, invn;
-    window_stats(P, refs, i, m, &mu, &invn);
+    window_stats(Ps, rs, i, m, &mu, &invn);
     double df = 0.0, dg = 0.0;
     if (i > 0) {
         double mup, invp;
-        window_stats(P, refs, i - 1, m, &mup, &invp);
-        const double xin = (double)x[i + m - 1], xout = (double)x[i - 1];
+        window_stats(Ps, rs, i - 1, m, &mup, &invp);
+        const double xin = (double)xs[i + m - 1], xout = (double)xs[i - 1];
         df = 0.5 * (xin - xout);
         dg = (xin - mu) + (xout - mup);
     }
-    colv[i] = make_float4((float)df, (float)dg, (float)invn, (float)mu);
-    st64[i] = make_double2(mu, invn);
+    colv[g] = make_float4((float)df, (float)dg, (float)invn, (float)mu);
+    st64[g] = make_double2(mu, invn);
 }
 
 // ---------------------------------------------------------------------------------------
-// Scan kernels.
+// Scan kernels (all indices below are local to one series).
 
 struct ScanLds {
     float *xs;    // [64 + m + 4]  x - ref over the current window
@@ -219,9 +244,21 @@ struct RowState {
     float thr;     // prune when cov*invn_j >= thr
     float mx;      // max over valid columns of cov*invn_j
     uint64_t alive;// wave-uniform alive mask of the 64 rows
+    uint32_t tiles;// 64x64 tiles evaluated (pair counter)
 };
 
-// climit: first column not to evaluate (min of the scan end and N); N: rows at or beyond N
+// One series' view of the batch.
+struct View {
+    const float *x;
+    const float4 *colv;
+    uint64_t *alive;
+    int *best;
+    int *wit;
+    int64_t N;
+    int m;
+};
+
+// climit: first column not to evaluate (min of the scan end and N); rows at or beyond N
 // are padding.
 template <bool MASKED>
 __device__ __forceinline__ void tile_steps(const ScanLds &L, RowState &R, int64_t s, int64_t c,
@@ -265,52 +302,55 @@ __device__ __forceinline__ void tile_steps(const ScanLds &L, RowState &R, int64_
 }
 
 // Scan columns [c0, c1) for the block of rows [s, s+64).  c0 % 64 == 0.  Returns false when
-// every row of the block is known dead.
-__device__ bool scan_range(const ScanLds &L, RowState &R, const float *__restrict__ x,
-                           const float4 *__restrict__ colv, uint64_t *__restrict__ alive,
-                           int64_t s, int64_t c0, int64_t c1, int64_t N, int m, float qrow_sum,
-                           bool qrow_staged)
+// every row of the block is known dead.  With V.wit set, rows pruned by this block record
+// the pruning tile and columns pruned by it record the block.
+__device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t s, int64_t c0,
+                           int64_t c1)
 {
     const int t = threadIdx.x;
+    const int m = V.m;
+    const int64_t N = V.N;
     // Chain initialisation: lane u holds cov(s+u, c0-1), or cov(s+u+1, 0) when c0 == 0
     // (position 0 has zero recurrence terms, so the first step reproduces it).
     {
         const int64_t cI = c0 > 0 ? c0 - 1 : 0;
         const int64_t rb = c0 > 0 ? s : s + 1;
-        const float muc = colv[cI].w;
-        const float qs = stage_centred(L, x, cI, m, muc);
-        R.cov = corr_lanes(L, x, rb, m, qs, colv[rb + t].w);
+        const float muc = V.colv[cI].w;
+        const float qs = stage_centred(L, V.x, cI, m, muc);
+        R.cov = corr_lanes(L, V.x, rb, m, qs, V.colv[rb + t].w);
     }
     // Seeds use the centred first row of the block.
-    const float murow = colv[s].w;
-    const float qsum = stage_centred(L, x, s, m, murow);
-    (void)qrow_sum;
-    (void)qrow_staged;
+    const float murow = V.colv[s].w;
+    const float qsum = stage_centred(L, V.x, s, m, murow);
     const uint64_t word = (uint64_t)(s >> 6);
     for (int64_t c = c0; c < c1; c += WAVE) {
-        const float4 cv = colv[c + t];
-        const float seed = corr_lanes(L, x, c, m, qsum, cv.w);
+        const float4 cv = V.colv[c + t];
+        const float seed = corr_lanes(L, V.x, c, m, qsum, cv.w);
         L.tile[t] = make_float4(cv.x, cv.y, cv.z, seed);
         __syncthreads();
+        R.tiles++;
         const bool masked = (c - s - 63 <= m && c - s + 63 >= -m) || (c + WAVE > N) ||
                             (s + WAVE > N) || (c + WAVE > c1);
         uint64_t rowhit = 0, colhit = 0;
         if (masked) {
-            // columns at or beyond c1 are outside this scan
             const int64_t lim = c1 < N ? c1 : N;
             tile_steps<true>(L, R, s, c, N, lim, m, rowhit, colhit);
         } else {
             tile_steps<false>(L, R, s, c, N, N, m, rowhit, colhit);
         }
         __syncthreads();
+        if (V.wit) {
+            if (((rowhit & R.alive) >> t) & 1ull) V.wit[s + t] = (int)c;
+            if ((colhit >> t) & 1ull) V.wit[c + t] = (int)s;
+        }
         if (t == 0 && colhit)
-            __hip_atomic_fetch_and(alive + (c >> 6), ~colhit, __ATOMIC_RELAXED,
+            __hip_atomic_fetch_and(V.alive + (c >> 6), ~colhit, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
         uint64_t cur;
         if (t == 0) {
-            cur = rowhit ? __hip_atomic_fetch_and(alive + word, ~rowhit, __ATOMIC_RELAXED,
+            cur = rowhit ? __hip_atomic_fetch_and(V.alive + word, ~rowhit, __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT)
-                         : __hip_atomic_load(alive + word, __ATOMIC_RELAXED,
+                         : __hip_atomic_load(V.alive + word, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
         }
         cur = __shfl(cur, 0, 64);
@@ -320,11 +360,10 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const float *__restric
     return true;
 }
 
-__device__ __forceinline__ void load_rows(RowState &R, const float4 *__restrict__ colv,
-                                          int64_t s, int m, float thr_rho)
+__device__ __forceinline__ void load_rows(RowState &R, const View &V, int64_t s, float thr_rho)
 {
     const int t = threadIdx.x;
-    const float4 rv = colv[s + t];
+    const float4 rv = V.colv[s + t];
     R.rdf = (t == 0) ? 0.f : rv.x;
     R.rdg = (t == 0) ? 0.f : rv.y;
     const float inv = rv.z;
@@ -333,16 +372,15 @@ __device__ __forceinline__ void load_rows(RowState &R, const float4 *__restrict_
     else
         R.thr = (thr_rho <= 0.f) ? -INFINITY : INFINITY;
     R.mx = -INFINITY;
-    (void)m;
+    R.tiles = 0;
 }
 
-__device__ __forceinline__ void flush_best(const RowState &R, const float4 *__restrict__ colv,
-                                           int *__restrict__ best, int64_t s, int64_t N)
+__device__ __forceinline__ void flush_best(const RowState &R, const View &V, int64_t s)
 {
     const int t = threadIdx.x;
-    if (((R.alive >> t) & 1ull) && (s + t) < N && R.mx > -INFINITY) {
-        const float rho = R.mx * colv[s + t].z;
-        atomicMax(best + s + t, ord_key(rho));
+    if (((R.alive >> t) & 1ull) && (s + t) < V.N && R.mx > -INFINITY) {
+        const float rho = R.mx * V.colv[s + t].z;
+        atomicMax(V.best + s + t, ord_key(rho));
     }
 }
 
@@ -356,74 +394,123 @@ __device__ __forceinline__ ScanLds carve_lds(int m)
     return L;
 }
 
-__global__ __launch_bounds__(64) void k_scan_select(const float *__restrict__ x,
-                                                    const float4 *__restrict__ colv,
-                                                    uint64_t *__restrict__ alive,
-                                                    int *__restrict__ best, int64_t N, int m,
-                                                    float thr_rho, int64_t WA)
+__device__ __forceinline__ void count_pairs(const RowState &R,
+                                            unsigned long long *__restrict__ pairs)
 {
-    const int64_t s = (int64_t)blockIdx.x * WAVE;
-    if (s >= N) return;
-    RowState R;
-    R.alive = __hip_atomic_load(alive + (s >> 6), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (R.alive == 0) return;
-    ScanLds L = carve_lds(m);
-    load_rows(R, colv, s, m, thr_rho);
-    const int64_t c1 = (s + WA < N) ? s + WA : N;
-    scan_range(L, R, x, colv, alive, s, s, c1, N, m, 0.f, false);
-    flush_best(R, colv, best, s, N);
+    if (threadIdx.x == 0 && R.tiles) atomicAdd(pairs, (unsigned long long)R.tiles * 4096ull);
 }
 
-// Work item = (surviving block, column chunk).  The chunk [q*CH, (q+1)*CH) minus the window
-// [s, s+WA) already scanned by k_scan_select.
-__global__ __launch_bounds__(64) void k_scan_refine(const float *__restrict__ x,
-                                                    const float4 *__restrict__ colv,
-                                                    uint64_t *__restrict__ alive,
-                                                    int *__restrict__ best,
-                                                    const int *__restrict__ blist, int64_t N,
-                                                    int m, float thr_rho, int64_t WA, int64_t CH,
+// Rebase the batch onto global block gb's series.  Returns the series id (-1: padding).
+__device__ __forceinline__ int make_view(const ScanArgs &A, int64_t gb, View &V, int64_t &s,
+                                         float &thr)
+{
+    const int sid = A.bser[gb];
+    if (sid < 0) return -1;
+    const SerDev S = A.ser[sid];
+    V.x = A.x + S.xoff;
+    V.colv = A.colv + S.roff;
+    V.alive = A.alive + (S.roff >> 6);
+    V.best = A.best + S.roff;
+    V.wit = A.wit ? A.wit + S.roff : nullptr;
+    V.N = S.N;
+    V.m = A.m;
+    s = gb * WAVE - S.roff;
+    thr = S.thr;
+    return sid;
+}
+
+__global__ __launch_bounds__(64) void k_scan_select(ScanArgs A)
+{
+    const int64_t gb = blockIdx.x;
+    View V;
+    int64_t s;
+    float thr_rho;
+    if (make_view(A, gb, V, s, thr_rho) < 0 || s >= V.N) return;
+    RowState R;
+    R.alive = __hip_atomic_load(V.alive + (s >> 6), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (R.alive == 0) return;
+    ScanLds L = carve_lds(V.m);
+    load_rows(R, V, s, thr_rho);
+    bool live = true;
+    if (A.wit_read) {
+        // Witness tiles: the tiles that pruned these rows at the previous length.  Scan up
+        // to four distinct ones first.
+        int w = A.wit_read[gb * WAVE + threadIdx.x];
+        const int64_t lim = V.N;
+        for (int rep = 0; rep < 4 && live; rep++) {
+            const uint64_t has = __builtin_amdgcn_ballot_w64(w >= 0 && w < lim && !(w & 63));
+            if (!has) break;
+            const int lane = __builtin_ctzll(has);
+            const int wt = __shfl(w, lane, 64);
+            if (w == wt) w = -1;
+            const int64_t e = (wt + WAVE < lim) ? wt + WAVE : lim;
+            live = scan_range(L, R, V, s, wt, e);
+        }
+    }
+    if (live) {
+        const int64_t c1 = (s + A.WA < V.N) ? s + A.WA : V.N;
+        live = scan_range(L, R, V, s, s, c1);
+    }
+    if (live) flush_best(R, V, s);
+    count_pairs(R, A.pairs);
+}
+
+// Work item = (surviving global block, column chunk q).  The chunk [q*CH, (q+1)*CH) minus
+// the window [s, s+WA) already scanned by k_scan_select.
+__global__ __launch_bounds__(64) void k_scan_refine(ScanArgs A, const int *__restrict__ blist,
                                                     int nchunks)
 {
     const int item = blockIdx.x;
-    const int64_t b = blist[item / nchunks];
+    const int64_t gb = blist[item / nchunks];
     const int64_t q = item % nchunks;
-    const int64_t s = b * WAVE;
-    RowState R;
-    R.alive = __hip_atomic_load(alive + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (R.alive == 0) return;
+    View V;
+    int64_t s;
+    float thr_rho;
+    if (make_view(A, gb, V, s, thr_rho) < 0) return;
+    const int64_t CH = A.CH;
+    const int64_t N = V.N;
     const int64_t a0 = q * CH;
+    if (a0 >= N) return;
+    RowState R;
+    R.alive = __hip_atomic_load(V.alive + (s >> 6), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (R.alive == 0) return;
     const int64_t a1 = ((q + 1) * CH < N) ? (q + 1) * CH : N;
-    const int64_t w0 = s, w1 = (s + WA < N) ? s + WA : N;
-    ScanLds L = carve_lds(m);
-    load_rows(R, colv, s, m, thr_rho);
+    const int64_t w0 = s, w1 = (s + A.WA < N) ? s + A.WA : N;
+    ScanLds L = carve_lds(V.m);
+    load_rows(R, V, s, thr_rho);
     bool live = true;
-    // part left of the window
     if (a0 < w0 && live) {
         const int64_t e = a1 < w0 ? a1 : w0;
-        live = scan_range(L, R, x, colv, alive, s, a0, e, N, m, 0.f, false);
+        live = scan_range(L, R, V, s, a0, e);
     }
-    // part right of the window
     if (a1 > w1 && live) {
         int64_t st = a0 > w1 ? a0 : w1;
         st &= ~(int64_t)63;  // align; columns re-scanned below w1 are harmless
-        live = scan_range(L, R, x, colv, alive, s, st, a1, N, m, 0.f, false);
+        live = scan_range(L, R, V, s, st, a1);
     }
-    if (live) flush_best(R, colv, best, s, N);
+    if (live) flush_best(R, V, s);
+    count_pairs(R, A.pairs);
 }
 
 // ---------------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_fill_alive(uint64_t *__restrict__ alive, int64_t N,
-                                                    int64_t nwords)
+// Alive bits of every global row: 1 for real rows (< N of their series), 0 for padding.
+__global__ __launch_bounds__(256) void k_fill_alive(uint64_t *__restrict__ alive,
+                                                    const SerDev *__restrict__ ser,
+                                                    const int *__restrict__ bser, int64_t nwords)
 {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nwords) return;
-    const int64_t r0 = w * 64;
+    const int sid = bser[w];
     uint64_t v = 0;
-    if (r0 + 64 <= N)
-        v = ~0ull;
-    else if (r0 < N)
-        v = (~0ull) >> (64 - (N - r0));
+    if (sid >= 0) {
+        const SerDev S = ser[sid];
+        const int64_t r0 = w * 64 - S.roff;
+        if (r0 + 64 <= S.N)
+            v = ~0ull;
+        else if (r0 < S.N)
+            v = (~0ull) >> (64 - (S.N - r0));
+    }
     alive[w] = v;
 }
 
@@ -445,17 +532,17 @@ __global__ __launch_bounds__(256) void k_list_blocks(const uint64_t *__restrict_
     }
 }
 
-// Surviving rows -> (index, FP32 squared distance); FLT_MAX when no valid pair was seen.
+// Surviving rows -> (global row, FP32 squared distance); FLT_MAX when no valid pair was seen.
 __global__ __launch_bounds__(256) void k_survivors(const uint64_t *__restrict__ alive,
-                                                   const int *__restrict__ best, int64_t N, int m,
-                                                   int *__restrict__ count,
+                                                   const int *__restrict__ best, int64_t rows,
+                                                   int m, int *__restrict__ count,
                                                    int64_t *__restrict__ idx,
                                                    float *__restrict__ d2, int64_t cap)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    if (!((alive[i >> 6] >> (i & 63)) & 1ull)) return;
-    const int k = best[i];
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= rows) return;
+    if (!((alive[g >> 6] >> (g & 63)) & 1ull)) return;
+    const int k = best[g];
     float d;
     if (k == ord_key(-INFINITY))
         d = FLT_MAX;
@@ -463,32 +550,44 @@ __global__ __launch_bounds__(256) void k_survivors(const uint64_t *__restrict__ 
         d = 2.0f * (float)m * (1.0f - ord_val(k));
     const int p = atomicAdd(count, 1);
     if (p < cap) {
-        idx[p] = i;
+        idx[p] = g;
         d2[p] = d;
     }
 }
 
 // ---------------------------------------------------------------------------------------
-// Exact (FP64) nearest-neighbour squared distance of up to MAXU candidates.  Each block
-// handles 256 columns; the candidates' centred windows live in LDS.  The pair value is
-// formed as sum_k (x[i+k]-mu_i)(x[j+k]-mu_j) in ascending k, times (invn_i*invn_j), which is
-// bitwise symmetric in (i, j): mutual nearest neighbours tie exactly and the tie goes to
-// the lower index.  outkey[u] receives the atomic minimum of ord_key64(ED^2).
+// Exact (FP64) nearest-neighbour squared distance of candidate groups.  A group is up to
+// MAXU candidates of one series; block = (group, 256-column chunk of that series), found by
+// binary search in goff (prefix sums of chunk counts).  The pair value is formed as
+// sum_k (x[i+k]-mu_i)(x[j+k]-mu_j) in ascending k, times (invn_i*invn_j), which is bitwise
+// symmetric in (i, j): mutual nearest neighbours tie exactly and the tie goes to the lower
+// index.  outkey[u] receives the atomic minimum of ord_key64(ED^2) of candidate u.
 
-__global__ __launch_bounds__(256) void k_exact(const float *__restrict__ x,
-                                               const double2 *__restrict__ st64, int64_t N, int m,
-                                               const int64_t *__restrict__ cand, int nU,
-                                               unsigned long long *__restrict__ outkey)
+__global__ __launch_bounds__(256) void k_exact(ExactArgs E)
 {
     extern __shared__ double dsh[];
+    const int m = E.m;
     double *xs = dsh;                   // [256 + m]   raw x as double
     double *A = dsh + (256 + m + 8);    // [nU][m]     centred candidate windows
     __shared__ double ainv[MAXU];
     __shared__ unsigned long long wmin[MAXU][4];
     const int t = threadIdx.x;
-    const int64_t c = (int64_t)blockIdx.x * 256;
+    int lo = 0, hi = E.ngroups - 1;
+    const int64_t item = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (E.goff[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const int grp = lo;
+    const int u0 = E.gfirst[grp];
+    const int nU = E.gfirst[grp + 1] - u0;
+    const SerDev S = E.ser[E.gser[grp]];
+    const float *x = E.x + S.xoff;
+    const double2 *st64 = E.st64 + S.roff;
+    const int64_t N = S.N;
+    const int64_t c = (item - E.goff[grp]) * 256;
     for (int u = 0; u < nU; u++) {
-        const int64_t i = cand[u];
+        const int64_t i = E.cand[u0 + u];
         const double2 st = st64[i];
         if (t == 0) ainv[u] = st.y;
         for (int k = t; k < m; k += 256) A[u * m + k] = (double)x[i + k] - st.x;
@@ -510,7 +609,7 @@ __global__ __launch_bounds__(256) void k_exact(const float *__restrict__ x,
 #pragma unroll
     for (int u = 0; u < MAXU; u++) {
         if (u >= nU) break;
-        const int64_t i = cand[u];
+        const int64_t i = E.cand[u0 + u];
         double ed = DBL_MAX;
         const bool valid = (j < N) && ((j - i >= m - 1) || (i - j >= m + 1));
         if (valid) {
@@ -531,7 +630,7 @@ __global__ __launch_bounds__(256) void k_exact(const float *__restrict__ x,
     if (t < nU) {
         unsigned long long k = wmin[t][0];
         for (int q = 1; q < 4; q++) k = wmin[t][q] < k ? wmin[t][q] : k;
-        atomicMin(outkey + t, k);
+        atomicMin(E.outkey + u0 + t, k);
     }
 }
 

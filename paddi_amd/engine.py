@@ -47,13 +47,15 @@ class Opts(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("attempts", C.c_int64), ("survivors", C.c_int64),
                 ("exact_refines", C.c_int64), ("pairs", C.c_int64), ("seconds", C.c_double),
-                ("kernel_ms", C.c_double * 4)]
+                ("kernel_ms", C.c_double * 4), ("launches", C.c_int64 * 4)]
+
+    KERNELS = ("derive", "select", "refine", "exact")
 
     def as_dict(self):
         return dict(attempts=self.attempts, survivors=self.survivors,
                     exact_refines=self.exact_refines, pairs=self.pairs, seconds=self.seconds,
-                    kernel_ms=dict(zip(("stats", "select", "refine", "exact"),
-                                       list(self.kernel_ms))))
+                    kernel_ms=dict(zip(self.KERNELS, list(self.kernel_ms))),
+                    launches=dict(zip(self.KERNELS, list(self.launches))))
 
 
 EXPORTS = ("paddi_status_str", "paddi_version", "paddi_create", "paddi_destroy",

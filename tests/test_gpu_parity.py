@@ -141,3 +141,48 @@ def test_certificate_at_scale(ctx):
         idx, d, nd = O.topk(cand, nn, r2, m, 3)
         assert [g.index for g in gs] == list(idx), (m, gs, idx)
         np.testing.assert_allclose([g.nn_dist2 for g in gs], d, rtol=RTOL)
+
+
+# --- the batched form (all series in every launch, tight per-length r, witness tiles) ------
+
+@pytest.mark.parametrize("kind,n,minL,maxL,step,k", [
+    ("rw", 9000, 64, 72, 1, 3),
+    ("ecg", 12000, 60, 66, 2, 3),
+    ("ar2", 5000, 8, 12, 1, 4),
+    ("rw", 30000, 100, 110, 5, 2),
+])
+def test_batched_matches_oracle(ctx, kind, n, minL, maxL, step, k):
+    x = {"rw": lambda: synth.rw_planted(n, 40 + n % 97, 2, max(minL, 16), 12.0),
+         "ecg": lambda: synth.ecg_like(n, 41, 1, 1),
+         "ar2": lambda: synth.ar2_mix(n, 42)}[kind]()
+    ref = O.palmad(x, minL, maxL, step, k)
+    got, st = ctx.discords(x, minL, maxL, step, k, mode="batched", with_stats=True)
+    worst = assert_same_discords(got, expected(ref), kind)
+    print(f"batched {kind} n={n}: max rel err {worst:.2e}, attempts {st['attempts']}")
+
+
+def test_batched_uneven_series_match_oracle(ctx):
+    """Many uneven series in one call: each series' discords equal the oracle's on it."""
+    rng = np.random.default_rng(8)
+    series = []
+    for q in range(12):
+        n = int(rng.integers(2000, 12000))
+        series.append(synth.rw_planted(n, 100 + q, 1, 40, 10.0) if q % 2 == 0
+                      else synth.ecg_like(n, 200 + q, 1, 0))
+    minL, maxL, k = 40, 44, 1
+    got = ctx.discords_batch(series, minL, maxL, 1, k, mode="batched")
+    plain = ctx.discords_batch(series, minL, maxL, 1, k, mode="plain")
+    assert [(g.series, g.index, g.length) for g in got] == \
+        [(g.series, g.index, g.length) for g in plain]
+    for q, x in enumerate(series):
+        ref = O.palmad(x, minL, maxL, 1, k)
+        mine = [g for g in got if g.series == q]
+        assert_same_discords(mine, expected(ref), f"series {q}")
+
+
+def test_batched_c1_matches_cached_oracle(ctx):
+    w = synth.workload_c1()
+    with open(os.path.join(GOLD, "palmad_c1.json")) as f:
+        ref = json.load(f)
+    got = ctx.discords(w["series"][0], w["minL"], w["maxL"], w["step"], w["k"], mode="batched")
+    assert_same_discords(got, expected(ref), "c1 batched")
