@@ -132,6 +132,27 @@ paddi_status paddi_discords_series(paddi_ctx *ctx, paddi_series *const *series, 
                                    const paddi_opts *opts, paddi_discord *out, int64_t cap,
                                    int64_t *count, paddi_stats *stats);
 
+/* ---- learned range proposal (no counterpart in the reference, whose range schedule is
+ * the blind r-descent of src/main.cu:136-233) ----------------------------------------
+ * Features of (series, m): m/512, log10(global std)/4 and the 32-bin histogram of
+ * ED^2/(2m) over 1024 seeded random non-overlapping window pairs.  Network:
+ * nfeat -> hidden (ReLU) -> hidden (ReLU) -> 1, single precision, parameters as one flat
+ * array W1[nfeat][hidden] b1 W2[hidden][hidden] b2 W3[hidden] b3; inputs standardised by
+ * (X - fmu) / fsd.  Loss: mean squared error. */
+paddi_status paddi_mlp_sizes(int32_t *nfeat, int32_t *hidden, int32_t *nparam);
+paddi_status paddi_features(paddi_ctx *ctx, paddi_series *const *series, int32_t nseries,
+                            int32_t m, float *out /* [nseries][nfeat] */);
+paddi_status paddi_mlp_forward(paddi_ctx *ctx, const float *params, const float *fmu,
+                               const float *fsd, const float *X, int32_t B, float *Y);
+paddi_status paddi_mlp_gradient(paddi_ctx *ctx, const float *params, const float *fmu,
+                                const float *fsd, const float *X, const float *T, int32_t B,
+                                float *grad /* [nparam] */, float *loss);
+/* Full-batch Adam on the device for `steps` steps; params updated in place; losses[steps]
+ * (optional) receives the loss before each step. */
+paddi_status paddi_mlp_train(paddi_ctx *ctx, float *params, const float *fmu, const float *fsd,
+                             const float *X, const float *T, int32_t B, int32_t steps, float lr,
+                             float weight_decay, float *losses);
+
 #ifdef __cplusplus
 }
 #endif

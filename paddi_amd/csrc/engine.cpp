@@ -673,6 +673,7 @@ void Engine::run(const std::vector<const Series *> &ss, int minL, int maxL, int 
         }
         return;
     }
+    if (o.mode == PADDI_MODE_LEARNED) set_model_path(o.model_path);
     build_batch(batch_, ss, minL);
     prefix(batch_);
     fast(batch_, minL, maxL, step, k, o, out, st);
@@ -682,15 +683,6 @@ void Engine::run(const std::vector<const Series *> &ss, int minL, int maxL, int 
 
 // ------------------------------------------------------------------------------------------
 // C ABI
-
-struct paddi_ctx {
-    paddi::Engine *eng;
-    std::string last_error;
-};
-struct paddi_series {
-    paddi_ctx *ctx;
-    paddi::Series *s;
-};
 
 template <class F>
 static paddi_status guarded(paddi_ctx *ctx, F &&f)

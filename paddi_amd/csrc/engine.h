@@ -6,8 +6,11 @@
 #include <stdint.h>
 #include <vector>
 
+#include <string>
+
 #include "../../include/paddi.h"
 #include "kernels.h"
+#include "mlp.h"
 
 namespace paddi {
 
@@ -98,10 +101,23 @@ private:
                std::vector<paddi_discord> &out, paddi_stats *st);
     void fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opts &o,
               std::vector<paddi_discord> &out, paddi_stats *st);
+public:
     void upload_ser(Batch &B);
     // Learned proposal of the squared range per series at length m (0 = no proposal).
     void propose_r2(Batch &B, int m, int k, std::vector<double> &out);
+    // Features of every series at length m (derive(m) must be current); NFEAT per series.
+    void features(Batch &B, int m, std::vector<float> &X);
+    void mlp_forward(const float *prm, const float *fmu, const float *fsd, const float *X,
+                     int Bn, float *Y);
+    void mlp_train(float *prm, const float *fmu, const float *fsd, const float *X,
+                   const float *T, int Bn, int steps, float lr, float wd, float *losses,
+                   float *grad_out);
+    void set_model_path(const char *p);
 
+private:
+    Model model_;
+    std::string model_path_;
+    DevBuf feat_, mlpbuf_;
     Batch batch_;
     bool wread_valid_ = false;   // the read-side witness buffer holds the previous length
     int device_;
@@ -114,3 +130,13 @@ private:
 };
 
 }  // namespace paddi
+
+// Opaque handles of the C ABI.
+struct paddi_ctx {
+    paddi::Engine *eng;
+    std::string last_error;
+};
+struct paddi_series {
+    paddi_ctx *ctx;
+    paddi::Series *s;
+};

@@ -61,7 +61,8 @@ class Stats(C.Structure):
 EXPORTS = ("paddi_status_str", "paddi_version", "paddi_create", "paddi_destroy",
            "paddi_compute_statistics", "paddi_range_discords", "paddi_discords",
            "paddi_discords_batch", "paddi_series_upload", "paddi_series_free",
-           "paddi_discords_series")
+           "paddi_discords_series", "paddi_has_mode", "paddi_mlp_sizes", "paddi_features",
+           "paddi_mlp_forward", "paddi_mlp_gradient", "paddi_mlp_train")
 
 _lib = None
 
@@ -92,8 +93,22 @@ def load_library(path: str = LIB_PATH):
     L.paddi_series_free.restype = None
     L.paddi_discords_series.argtypes = [P, P, i32, i32, i32, i32, i32, C.POINTER(Opts), P, i64,
                                         C.POINTER(i64), C.POINTER(Stats)]
+    L.paddi_has_mode.argtypes = [i32]
+    L.paddi_has_mode.restype = i32
+    L.paddi_mlp_sizes.argtypes = [P, P, P]
+    L.paddi_features.argtypes = [P, P, i32, i32, P]
+    L.paddi_mlp_forward.argtypes = [P, P, P, P, P, i32, P]
+    L.paddi_mlp_gradient.argtypes = [P, P, P, P, P, P, i32, P, P]
+    L.paddi_mlp_train.argtypes = [P, P, P, P, P, P, i32, i32, C.c_float, C.c_float, P]
     _lib = L
     return L
+
+
+def mlp_sizes():
+    L = load_library()
+    a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+    L.paddi_mlp_sizes(C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value
 
 
 def _f32(T):
@@ -219,6 +234,48 @@ class Context:
 
     def upload(self, T) -> SeriesHandle:
         return SeriesHandle(self, T)
+
+    # -- learned range proposal ---------------------------------------------------
+
+    def features(self, handles, m):
+        nf, _, _ = mlp_sizes()
+        hs = (C.c_void_p * len(handles))(*[h._h.value for h in handles])
+        out = np.empty((len(handles), nf), np.float32)
+        self._check(self._L.paddi_features(self._c, hs, len(handles), m, _ptr(out)),
+                    "paddi_features")
+        return out
+
+    def mlp_forward(self, params, fmu, fsd, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.empty(len(X), np.float32)
+        args = [np.ascontiguousarray(a, np.float32) for a in (params, fmu, fsd)]
+        self._check(self._L.paddi_mlp_forward(self._c, *[_ptr(a) for a in args], _ptr(X),
+                                              len(X), _ptr(Y)), "paddi_mlp_forward")
+        return Y
+
+    def mlp_gradient(self, params, fmu, fsd, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        args = [np.ascontiguousarray(a, np.float32) for a in (params, fmu, fsd)]
+        g = np.empty(len(args[0]), np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(self._L.paddi_mlp_gradient(self._c, *[_ptr(a) for a in args], _ptr(X),
+                                               _ptr(T), len(X), _ptr(g), _ptr(loss)),
+                    "paddi_mlp_gradient")
+        return g, float(loss[0])
+
+    def mlp_train(self, params, fmu, fsd, X, T, steps, lr=1e-3, weight_decay=0.0):
+        p = np.array(params, np.float32, copy=True)
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        fmu = np.ascontiguousarray(fmu, np.float32)
+        fsd = np.ascontiguousarray(fsd, np.float32)
+        losses = np.empty(steps, np.float32)
+        self._check(self._L.paddi_mlp_train(self._c, _ptr(p), _ptr(fmu), _ptr(fsd), _ptr(X),
+                                            _ptr(T), len(X), steps, C.c_float(lr),
+                                            C.c_float(weight_decay), _ptr(losses)),
+                    "paddi_mlp_train")
+        return p, losses
 
     def discords_series(self, handles, minL, maxL, step=1, top_k=1, mode="plain",
                         max_attempts=64, model_path=None, with_stats=False):

@@ -161,6 +161,36 @@ def test_batched_matches_oracle(ctx, kind, n, minL, maxL, step, k):
     print(f"batched {kind} n={n}: max rel err {worst:.2e}, attempts {st['attempts']}")
 
 
+@pytest.mark.parametrize("kind,n,minL,maxL,step,k", [
+    ("rw", 20000, 64, 70, 2, 3),
+    ("ecg", 15000, 100, 104, 1, 2),
+    ("ar2", 12000, 256, 264, 4, 3),
+])
+def test_learned_matches_oracle(ctx, kind, n, minL, maxL, step, k):
+    """The learned proposal changes only how many passes run, never the discord set."""
+    x = {"rw": lambda: synth.rw_planted(n, 50 + n % 89, 2, 64, 12.0),
+         "ecg": lambda: synth.ecg_like(n, 43, 1, 1),
+         "ar2": lambda: synth.ar2_mix(n, 44)}[kind]()
+    ref = O.palmad(x, minL, maxL, step, k)
+    got, st = ctx.discords(x, minL, maxL, step, k, mode="learned", with_stats=True)
+    assert_same_discords(got, expected(ref), f"learned {kind}")
+
+
+def test_learned_proposal_too_high_falls_back(ctx, tmp_path):
+    """A model whose margin is hugely negative proposes r far above every discord distance:
+    the descent must fall back and still return the oracle's set."""
+    from paddi_amd import model as Mdl
+    d = Mdl.load()
+    bad = tmp_path / "bad.bin"
+    Mdl.save(str(bad), d["fmu"], d["fsd"], d["params"], -3.0)
+    x = synth.rw_planted(20000, 77, 2, 64, 12.0)
+    ref = O.palmad(x, 64, 66, 1, 2)
+    got, st = ctx.discords(x, 64, 66, 1, 2, mode="learned", model_path=str(bad),
+                           with_stats=True)
+    assert_same_discords(got, expected(ref), "fallback")
+    assert st["attempts"] > 3
+
+
 def test_batched_uneven_series_match_oracle(ctx):
     """Many uneven series in one call: each series' discords equal the oracle's on it."""
     rng = np.random.default_rng(8)
