@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--lengths", type=int, default=0, help="limit the number of lengths (debug)")
     p.add_argument("--primary", default="auto", help="plain|batched|learned|auto")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-plain", action="store_true", help="skip the plain form (profiling)")
     p.add_argument("--cpu-points", type=int, default=65536)
     return p.parse_args()
 
@@ -115,16 +116,20 @@ def main():
 
     print(f"[bench] workload {a.workload}: {len(series)} series, {points} points, "
           f"{nlen} lengths, top-{w['k']}; primary form: {primary}", file=sys.stderr, flush=True)
-    plain_recs, plain_t, plain_st = run_form(ctx, handles, w, maxL, "plain", a.steps if
-                                             primary == "plain" else 1,
-                                             a.warmup if primary == "plain" else 0)
-    print(f"[bench] plain: {[round(t, 3) for t in plain_t]} s", file=sys.stderr, flush=True)
-    if primary == "plain":
-        recs, times, stats = plain_recs, plain_t, plain_st
-    else:
+    if primary != "plain":
         recs, times, stats = run_form(ctx, handles, w, maxL, primary, a.steps, a.warmup)
         print(f"[bench] {primary}: {[round(t, 3) for t in times]} s", file=sys.stderr,
               flush=True)
+    if a.no_plain and primary != "plain":
+        plain_recs, plain_t, plain_st = recs, [float("nan")], [stats[-1]]
+    else:
+        plain_recs, plain_t, plain_st = run_form(ctx, handles, w, maxL, "plain", a.steps if
+                                                 primary == "plain" else 1,
+                                                 a.warmup if primary == "plain" else 0)
+        print(f"[bench] plain: {[round(t, 3) for t in plain_t]} s", file=sys.stderr,
+              flush=True)
+    if primary == "plain":
+        recs, times, stats = plain_recs, plain_t, plain_st
     identical = same_set(recs, plain_recs)
 
     t_step = sum(times) / len(times)

@@ -31,6 +31,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -152,6 +153,19 @@ bool TopK::advance(std::vector<size_t> &need)
                                   [&](size_t a, size_t b) { return hi(cands[a]) > hi(cands[b]); });
                 need.resize(64);
             }
+            // Speculate: refine the next-ranked candidates in the same launch (they are
+            // usually the next ones decided), up to `spec` per round.
+            if ((int)need.size() < spec) {
+                std::vector<char> in(cands.size(), 0);
+                for (size_t q : need) in[q] = 1;
+                std::vector<size_t> extra;
+                for (size_t q = 0; q < cands.size(); q++)
+                    if (!gone[q] && !in[q] && !cands[q].exact) extra.push_back(q);
+                const size_t want = std::min(extra.size(), (size_t)spec - need.size());
+                std::partial_sort(extra.begin(), extra.begin() + want, extra.end(),
+                                  [&](size_t a, size_t b) { return hi(cands[a]) > hi(cands[b]); });
+                need.insert(need.end(), extra.begin(), extra.begin() + want);
+            }
             return true;
         }
         size_t B = U[0];
@@ -176,6 +190,8 @@ Engine::Engine(int device) : device_(device)
     HIPC(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (auto &e : ev_) HIPC(hipEventCreate(&e));
     HIPC(hipHostMalloc(&pin_, 64, hipHostMallocDefault));
+    const char *dbg = std::getenv("PADDI_DEBUG");
+    debug_ = dbg && dbg[0] == '1';
 }
 
 Engine::~Engine()
@@ -345,22 +361,46 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
                        (int)0x807fffff /* ord_key(-inf) */);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, stream_, cnt_.as<int>(), 8, 0);
     HIPC(hipEventRecord(ev_[0], stream_));
-    hipLaunchKernelGGL(k_scan_select, dim3((unsigned)nwords), dim3(64), lds, stream_, A);
+    launch_scan_select(A, (unsigned)nwords, lds, stream_);
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ev_[1], stream_));
     hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
                        (const uint64_t *)A.alive, nwords, blist_.as<int>(), cnt_.as<int>());
-    HIPC(hipMemcpyAsync(pin_, cnt_.p, 4, hipMemcpyDeviceToHost, stream_));
+    HIPC(hipMemcpyAsync(pin_, cnt_.p, 32, hipMemcpyDeviceToHost, stream_));
     HIPC(hipStreamSynchronize(stream_));
-    const int nblk = ((int *)pin_)[0];
+    int nblk = ((int *)pin_)[0];
+    const int nblk_select = nblk;
+    const unsigned long long pairs_select = ((unsigned long long *)pin_)[2];
     int64_t nmax = 0;
     for (int q = 0; q < B.nser; q++) nmax = std::max(nmax, B.host[(size_t)q].N);
-    const int nchunks = (int)std::max<int64_t>(1, (nmax + A.CH - 1) / A.CH);
+    // Refine rounds over cyclic offsets [WA, N) from each block, nearest first; between
+    // rounds the live blocks are listed again, so far columns are launched only for the
+    // few blocks the near ones did not kill.  Chunk size: about refine_waves_ work items
+    // per round (16 waves per SIMD), within [min_chunk_, refine_chunk_] columns.
+    const int64_t bounds[4] = {A.WA, A.WA + 2 * refine_chunk_, A.WA + 8 * refine_chunk_, nmax};
     HIPC(hipEventRecord(ev_[2], stream_));
-    if (nblk > 0) {
-        hipLaunchKernelGGL(k_scan_refine, dim3((unsigned)((int64_t)nblk * nchunks)), dim3(64), lds,
-                           stream_, A, (const int *)blist_.p, nchunks);
+    int rounds = 0;
+    for (int rd = 0; rd < 3 && nblk > 0; rd++) {
+        const int64_t ub = bounds[rd], ue = std::min(bounds[rd + 1], nmax);
+        if (ub >= ue) continue;
+        if (rounds > 0) {
+            hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, stream_, cnt_.as<int>(), 1, 0);
+            hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
+                               (const uint64_t *)A.alive, nwords, blist_.as<int>(),
+                               cnt_.as<int>());
+            HIPC(hipMemcpyAsync(pin_, cnt_.p, 4, hipMemcpyDeviceToHost, stream_));
+            HIPC(hipStreamSynchronize(stream_));
+            nblk = ((int *)pin_)[0];
+            if (nblk == 0) break;
+        }
+        const int64_t len = ue - ub;
+        int64_t ch = round_up((len * (int64_t)nblk + refine_waves_ - 1) / refine_waves_, 64);
+        A.CH = std::min<int64_t>(refine_chunk_, std::max<int64_t>(min_chunk_, ch));
+        const int nchunks = (int)((len + A.CH - 1) / A.CH);
+        launch_scan_refine(A, (const int *)blist_.p, nchunks, ub, ue,
+                           (unsigned)((int64_t)nblk * nchunks), lds, stream_);
         HIPC(hipGetLastError());
+        rounds++;
     }
     HIPC(hipEventRecord(ev_[3], stream_));
     hipLaunchKernelGGL(k_survivors, dim3(grid1(rows, 256)), dim3(256), 0, stream_,
@@ -370,6 +410,17 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
     HIPC(hipMemcpyAsync(pin_, cnt_.p, 32, hipMemcpyDeviceToHost, stream_));
     HIPC(hipStreamSynchronize(stream_));
     const int ns = ((int *)pin_)[1];
+    if (debug_) {
+        float a = 0.f, b = 0.f;
+        HIPC(hipEventElapsedTime(&a, ev_[0], ev_[1]));
+        HIPC(hipEventElapsedTime(&b, ev_[2], ev_[3]));
+        const unsigned long long pairs_all = ((unsigned long long *)pin_)[2];
+        std::fprintf(stderr,
+                     "[paddi] m=%d select %.2f ms %.3g pairs | refine blocks %d->%d rounds %d "
+                     "%.2f ms %.3g pairs | survivors %d\n",
+                     m, a, (double)pairs_select, nblk_select, nblk, rounds, b,
+                     (double)(pairs_all - pairs_select), ns);
+    }
     out.assign((size_t)B.nser, {});
     if (ns > 0) {
         std::vector<int64_t> ix((size_t)ns);
@@ -389,7 +440,7 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
         HIPC(hipEventElapsedTime(&b, ev_[2], ev_[3]));
         st->kernel_ms[1] += a;
         st->launches[1] += 1;
-        if (nblk > 0) {
+        if (rounds > 0) {
             st->kernel_ms[2] += b;
             st->launches[2] += 1;
         }
@@ -422,7 +473,7 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
         gfirst.push_back((int)cand.size());
         gser.push_back(sid);
         goff.push_back(blocks);
-        blocks += (B.n[(size_t)sid] - m + 1 + 255) / 256;
+        blocks += (B.n[(size_t)sid] - m + 1 + EXACT_COLS - 1) / EXACT_COLS;
         int u = 0;
         while (q < order.size() && req[order[q]].first == sid && u < umax) {
             cand.push_back(req[order[q]].second);

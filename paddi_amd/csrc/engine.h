@@ -51,6 +51,7 @@ struct TopK {
     std::vector<Cand> acc;
     double r2 = 0.0, delta = 0.0;
     int m = 0, k = 0;
+    int spec = 8;      // candidates refined per round (speculative beyond the undecided set)
     bool done = false;
     void reset(std::vector<Cand> &&c, double r2_, int m_, int k_);
     // Advances the greedy; returns true with `need` (indices into cands) when exact values
@@ -94,7 +95,10 @@ public:
     void copy_st64(double2 *dst, int64_t N);
 
     int64_t select_window_ = 8192;   // columns right of a block scanned by k_scan_select
-    int64_t refine_chunk_ = 65536;   // columns per k_scan_refine work item
+    int64_t refine_chunk_ = 65536;   // longest k_scan_refine work item (columns)
+    int64_t min_chunk_ = 2048;       // shortest k_scan_refine work item (columns)
+    int64_t refine_waves_ = 16384;   // target number of k_scan_refine work items
+    bool debug_ = false;             // per-pass trace on stderr (PADDI_DEBUG=1)
 
 private:
     void plain(Batch &B, int sid, int minL, int maxL, int step, int k, const paddi_opts &o,

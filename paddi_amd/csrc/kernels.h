@@ -8,6 +8,7 @@ namespace paddi {
 constexpr int PB = 4096;       // prefix-sum block (points); also the largest supported m
 constexpr int MAXM = 2048;     // largest subsequence length the scan kernels accept
 constexpr int MAXU = 8;        // candidates per exact-refine group
+constexpr int EXACT_COLS = 2048; // columns per k_exact block (8 sub-tiles of 256)
 constexpr int64_t XPAD = 1024; // zero padding after each series (floats)
 
 // One series of a batch, as the kernels see it (rebuilt per length: N and thr change).
@@ -56,8 +57,10 @@ __global__ void k_derive(const float *__restrict__ x, const SerDev *__restrict__
                          const int *__restrict__ bser, const double2 *__restrict__ P,
                          const double *__restrict__ refs, int64_t rows, int m,
                          float4 *__restrict__ colv, double2 *__restrict__ st64);
-__global__ void k_scan_select(ScanArgs A);
-__global__ void k_scan_refine(ScanArgs A, const int *__restrict__ blist, int nchunks);
+// k_scan_select / k_scan_refine are templates on the prefetch depth; these pick it from m.
+void launch_scan_select(const ScanArgs &A, unsigned grid, size_t lds, hipStream_t st);
+void launch_scan_refine(const ScanArgs &A, const int *blist, int nchunks, int64_t u_begin,
+                        int64_t u_end, unsigned grid, size_t lds, hipStream_t st);
 __global__ void k_fill_alive(uint64_t *__restrict__ alive, const SerDev *__restrict__ ser,
                              const int *__restrict__ bser, int64_t nwords);
 __global__ void k_fill_int(int *__restrict__ p, int64_t n, int v);

@@ -301,15 +301,39 @@ __device__ __forceinline__ void tile_steps(const ScanLds &L, RowState &R, int64_
     R.mx = mx;
 }
 
+// lane t: sum_k q[k] * xs[t+k] over the staged window.
+__device__ __forceinline__ float dot_window(const ScanLds &L, int m)
+{
+    const int t = threadIdx.x;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    int k = 0;
+    for (; k + 4 <= m; k += 4) {
+        const float4 qv = *reinterpret_cast<const float4 *>(L.q + k);
+        a0 = fmaf(qv.x, L.xs[t + k], a0);
+        a1 = fmaf(qv.y, L.xs[t + k + 1], a1);
+        a2 = fmaf(qv.z, L.xs[t + k + 2], a2);
+        a3 = fmaf(qv.w, L.xs[t + k + 3], a3);
+    }
+    for (; k < m; k++) a0 = fmaf(L.q[k], L.xs[t + k], a0);
+    return (a0 + a1) + (a2 + a3);
+}
+
 // Scan columns [c0, c1) for the block of rows [s, s+64).  c0 % 64 == 0.  Returns false when
 // every row of the block is known dead.  With V.wit set, rows pruned by this block record
 // the pruning tile and columns pruned by it record the block.
+//
+// Software pipeline: the next tile's window (PX floats per lane, 64 + m <= 64 * PX) and
+// column terms are loaded into registers while the current tile's 64 steps run, and the
+// block's alive word read back by this tile's atomic is consumed one tile later, so no
+// global-memory round trip sits on the tile's critical path.
+template <int PX>
 __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t s, int64_t c0,
                            int64_t c1)
 {
     const int t = threadIdx.x;
     const int m = V.m;
     const int64_t N = V.N;
+    const int win = WAVE + m;
     // Chain initialisation: lane u holds cov(s+u, c0-1), or cov(s+u+1, 0) when c0 == 0
     // (position 0 has zero recurrence terms, so the first step reproduces it).
     {
@@ -323,9 +347,33 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
     const float murow = V.colv[s].w;
     const float qsum = stage_centred(L, V.x, s, m, murow);
     const uint64_t word = (uint64_t)(s >> 6);
+    float px[PX];
+#pragma unroll
+    for (int p = 0; p < PX; p++) {
+        const int k = p * WAVE + t;
+        px[p] = (k < win) ? V.x[c0 + k] : 0.f;
+    }
+    float4 pc = V.colv[c0 + t];
+    uint64_t pend_lane0 = ~0ull;  // lane 0: the alive word returned by the previous tile
     for (int64_t c = c0; c < c1; c += WAVE) {
-        const float4 cv = V.colv[c + t];
-        const float seed = corr_lanes(L, V.x, c, m, qsum, cv.w);
+        const float ref = __shfl(px[0], 0, 64);  // x[c]
+#pragma unroll
+        for (int p = 0; p < PX; p++) {
+            const int k = p * WAVE + t;
+            if (k < win) L.xs[k] = px[p] - ref;
+        }
+        const float4 cv = pc;
+        const int64_t cn = c + WAVE;
+        if (cn < c1) {
+#pragma unroll
+            for (int p = 0; p < PX; p++) {
+                const int k = p * WAVE + t;
+                if (k < win) px[p] = V.x[cn + k];
+            }
+            pc = V.colv[cn + t];
+        }
+        __syncthreads();
+        const float seed = dot_window(L, m) - (cv.w - ref) * qsum;
         L.tile[t] = make_float4(cv.x, cv.y, cv.z, seed);
         __syncthreads();
         R.tiles++;
@@ -346,18 +394,41 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
         if (t == 0 && colhit)
             __hip_atomic_fetch_and(V.alive + (c >> 6), ~colhit, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
-        uint64_t cur;
+        // Rows pruned elsewhere, as read back one tile ago, and rows pruned here now.
+        const uint64_t pend = __shfl(pend_lane0, 0, 64);
+        const uint64_t newdead = rowhit & R.alive;
+        R.alive &= pend & ~rowhit;
         if (t == 0) {
-            cur = rowhit ? __hip_atomic_fetch_and(V.alive + word, ~rowhit, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT)
-                         : __hip_atomic_load(V.alive + word, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
+            pend_lane0 = newdead ? __hip_atomic_fetch_and(V.alive + word, ~newdead,
+                                                          __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT)
+                                 : __hip_atomic_load(V.alive + word, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
         }
-        cur = __shfl(cur, 0, 64);
-        R.alive &= cur & ~rowhit;
         if (R.alive == 0) return false;
     }
     return true;
+}
+
+// Columns at cyclic offsets [u0, u1) from the block start s: column (s + u) mod N.  u0 is a
+// multiple of 64, so the unwrapped part starts aligned; the wrapped part starts at the
+// aligned-down column, re-scanning at most 63 columns (harmless: same values, same
+// decisions).  Covering u in [0, N) scans every column once.
+template <int PX>
+__device__ bool scan_cyclic(const ScanLds &L, RowState &R, const View &V, int64_t s, int64_t u0,
+                            int64_t u1)
+{
+    const int64_t N = V.N;
+    u1 = u1 < N ? u1 : N;
+    if (u0 >= u1) return true;
+    const int64_t a = s + u0, b = s + u1;
+    bool live = true;
+    if (a < N) live = scan_range<PX>(L, R, V, s, a, b < N ? b : N);
+    if (live && b > N) {
+        const int64_t a2 = (a > N ? a - N : 0) & ~(int64_t)63;
+        live = scan_range<PX>(L, R, V, s, a2, b - N);
+    }
+    return live;
 }
 
 __device__ __forceinline__ void load_rows(RowState &R, const View &V, int64_t s, float thr_rho)
@@ -419,6 +490,7 @@ __device__ __forceinline__ int make_view(const ScanArgs &A, int64_t gb, View &V,
     return sid;
 }
 
+template <int PX>
 __global__ __launch_bounds__(64) void k_scan_select(ScanArgs A)
 {
     const int64_t gb = blockIdx.x;
@@ -444,21 +516,21 @@ __global__ __launch_bounds__(64) void k_scan_select(ScanArgs A)
             const int wt = __shfl(w, lane, 64);
             if (w == wt) w = -1;
             const int64_t e = (wt + WAVE < lim) ? wt + WAVE : lim;
-            live = scan_range(L, R, V, s, wt, e);
+            live = scan_range<PX>(L, R, V, s, wt, e);
         }
     }
-    if (live) {
-        const int64_t c1 = (s + A.WA < V.N) ? s + A.WA : V.N;
-        live = scan_range(L, R, V, s, s, c1);
-    }
+    if (live) live = scan_cyclic<PX>(L, R, V, s, 0, A.WA);
     if (live) flush_best(R, V, s);
     count_pairs(R, A.pairs);
 }
 
-// Work item = (surviving global block, column chunk q).  The chunk [q*CH, (q+1)*CH) minus
-// the window [s, s+WA) already scanned by k_scan_select.
+// Work item = (live global block, chunk q of the offset range [u_begin, u_end)): the columns
+// at cyclic offsets [u_begin + q*CH, u_begin + (q+1)*CH) from the block (see scan_cyclic).
+// Rounds of growing u_begin let nearby columns kill most blocks before the far ones are
+// launched.
+template <int PX>
 __global__ __launch_bounds__(64) void k_scan_refine(ScanArgs A, const int *__restrict__ blist,
-                                                    int nchunks)
+                                                    int nchunks, int64_t u_begin, int64_t u_end)
 {
     const int item = blockIdx.x;
     const int64_t gb = blist[item / nchunks];
@@ -467,29 +539,50 @@ __global__ __launch_bounds__(64) void k_scan_refine(ScanArgs A, const int *__res
     int64_t s;
     float thr_rho;
     if (make_view(A, gb, V, s, thr_rho) < 0) return;
-    const int64_t CH = A.CH;
-    const int64_t N = V.N;
-    const int64_t a0 = q * CH;
-    if (a0 >= N) return;
+    const int64_t u0 = u_begin + q * A.CH;
+    int64_t u1 = u0 + A.CH;
+    u1 = u1 < u_end ? u1 : u_end;
+    if (u0 >= u1 || u0 >= V.N) return;
     RowState R;
     R.alive = __hip_atomic_load(V.alive + (s >> 6), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (R.alive == 0) return;
-    const int64_t a1 = ((q + 1) * CH < N) ? (q + 1) * CH : N;
-    const int64_t w0 = s, w1 = (s + A.WA < N) ? s + A.WA : N;
     ScanLds L = carve_lds(V.m);
     load_rows(R, V, s, thr_rho);
-    bool live = true;
-    if (a0 < w0 && live) {
-        const int64_t e = a1 < w0 ? a1 : w0;
-        live = scan_range(L, R, V, s, a0, e);
-    }
-    if (a1 > w1 && live) {
-        int64_t st = a0 > w1 ? a0 : w1;
-        st &= ~(int64_t)63;  // align; columns re-scanned below w1 are harmless
-        live = scan_range(L, R, V, s, st, a1);
-    }
+    const bool live = scan_cyclic<PX>(L, R, V, s, u0, u1);
     if (live) flush_best(R, V, s);
     count_pairs(R, A.pairs);
+}
+
+// Launchers: the prefetch depth PX = ceil((64 + m) / 64) is a template parameter.
+void launch_scan_select(const ScanArgs &A, unsigned grid, size_t lds, hipStream_t st)
+{
+    const int px = (WAVE + A.m + WAVE - 1) / WAVE;
+    if (px <= 3)
+        hipLaunchKernelGGL((k_scan_select<3>), dim3(grid), dim3(64), lds, st, A);
+    else if (px <= 5)
+        hipLaunchKernelGGL((k_scan_select<5>), dim3(grid), dim3(64), lds, st, A);
+    else if (px <= 9)
+        hipLaunchKernelGGL((k_scan_select<9>), dim3(grid), dim3(64), lds, st, A);
+    else
+        hipLaunchKernelGGL((k_scan_select<33>), dim3(grid), dim3(64), lds, st, A);
+}
+
+void launch_scan_refine(const ScanArgs &A, const int *blist, int nchunks, int64_t u_begin,
+                        int64_t u_end, unsigned grid, size_t lds, hipStream_t st)
+{
+    const int px = (WAVE + A.m + WAVE - 1) / WAVE;
+    if (px <= 3)
+        hipLaunchKernelGGL((k_scan_refine<3>), dim3(grid), dim3(64), lds, st, A, blist, nchunks,
+                           u_begin, u_end);
+    else if (px <= 5)
+        hipLaunchKernelGGL((k_scan_refine<5>), dim3(grid), dim3(64), lds, st, A, blist, nchunks,
+                           u_begin, u_end);
+    else if (px <= 9)
+        hipLaunchKernelGGL((k_scan_refine<9>), dim3(grid), dim3(64), lds, st, A, blist, nchunks,
+                           u_begin, u_end);
+    else
+        hipLaunchKernelGGL((k_scan_refine<33>), dim3(grid), dim3(64), lds, st, A, blist, nchunks,
+                           u_begin, u_end);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -585,40 +678,54 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     const float *x = E.x + S.xoff;
     const double2 *st64 = E.st64 + S.roff;
     const int64_t N = S.N;
-    const int64_t c = (item - E.goff[grp]) * 256;
+    const int64_t cblk = (item - E.goff[grp]) * EXACT_COLS;
     for (int u = 0; u < nU; u++) {
         const int64_t i = E.cand[u0 + u];
         const double2 st = st64[i];
         if (t == 0) ainv[u] = st.y;
         for (int k = t; k < m; k += 256) A[u * m + k] = (double)x[i + k] - st.x;
     }
-    for (int k = t; k < 256 + m; k += 256) xs[k] = (double)x[c + k];
-    __syncthreads();
-    const int64_t j = c + t;
-    const double2 sj = (j < N) ? st64[j] : make_double2(0.0, 0.0);
-    double acc[MAXU];
+    unsigned long long kmin[MAXU];
 #pragma unroll
-    for (int u = 0; u < MAXU; u++) acc[u] = 0.0;
-    for (int k = 0; k < m; k++) {
-        const double b = xs[t + k] - sj.x;
+    for (int u = 0; u < MAXU; u++) kmin[u] = ~0ull;
+    for (int sub = 0; sub < EXACT_COLS / 256; sub++) {
+        const int64_t c = cblk + sub * 256;
+        if (c >= N) break;
+        __syncthreads();  // previous sub-tile's readers are done with xs
+        for (int k = t; k < 256 + m; k += 256) xs[k] = (double)x[c + k];
+        __syncthreads();
+        const int64_t j = c + t;
+        const double2 sj = (j < N) ? st64[j] : make_double2(0.0, 0.0);
+        double acc[MAXU];
 #pragma unroll
-        for (int u = 0; u < MAXU; u++)
-            if (u < nU) acc[u] = fma(A[u * m + k], b, acc[u]);
+        for (int u = 0; u < MAXU; u++) acc[u] = 0.0;
+        for (int k = 0; k < m; k++) {
+            const double b = xs[t + k] - sj.x;
+#pragma unroll
+            for (int u = 0; u < MAXU; u++)
+                if (u < nU) acc[u] = fma(A[u * m + k], b, acc[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < MAXU; u++) {
+            if (u >= nU) break;
+            const int64_t i = E.cand[u0 + u];
+            double ed = DBL_MAX;
+            const bool valid = (j < N) && ((j - i >= m - 1) || (i - j >= m + 1));
+            if (valid) {
+                if (ainv[u] > 0.0 && sj.y > 0.0)
+                    ed = 2.0 * m * (1.0 - acc[u] * (ainv[u] * sj.y));
+                else
+                    ed = 2.0 * m;
+            }
+            const unsigned long long key = ord_key64(ed);
+            kmin[u] = key < kmin[u] ? key : kmin[u];
+        }
     }
     const int lane = t & 63, w = t >> 6;
 #pragma unroll
     for (int u = 0; u < MAXU; u++) {
         if (u >= nU) break;
-        const int64_t i = E.cand[u0 + u];
-        double ed = DBL_MAX;
-        const bool valid = (j < N) && ((j - i >= m - 1) || (i - j >= m + 1));
-        if (valid) {
-            if (ainv[u] > 0.0 && sj.y > 0.0)
-                ed = 2.0 * m * (1.0 - acc[u] * (ainv[u] * sj.y));
-            else
-                ed = 2.0 * m;
-        }
-        unsigned long long key = ord_key64(ed);
+        unsigned long long key = kmin[u];
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {
             const unsigned long long other = __shfl_xor(key, o, 64);
