@@ -51,7 +51,7 @@ struct TopK {
     std::vector<Cand> acc;
     double r2 = 0.0, delta = 0.0;
     int m = 0, k = 0;
-    int spec = 8;      // candidates refined per round (speculative beyond the undecided set)
+    int spec = 4;      // candidates refined per round (speculative beyond the undecided set)
     bool done = false;
     void reset(std::vector<Cand> &&c, double r2_, int m_, int k_);
     // Advances the greedy; returns true with `need` (indices into cands) when exact values

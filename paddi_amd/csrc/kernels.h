@@ -8,8 +8,10 @@ namespace paddi {
 constexpr int PB = 4096;       // prefix-sum block (points); also the largest supported m
 constexpr int MAXM = 2048;     // largest subsequence length the scan kernels accept
 constexpr int MAXU = 8;        // candidates per exact-refine group
-constexpr int EXACT_COLS = 2048; // columns per k_exact block (8 sub-tiles of 256)
-constexpr int64_t XPAD = 1024; // zero padding after each series (floats)
+constexpr int EXACT_SUB = 1024;  // columns per k_exact sub-tile (4 per thread)
+constexpr int EXACT_COLS = 4096; // columns per k_exact block (4 sub-tiles)
+constexpr int64_t XPAD = 2048; // zero padding after each series (floats); k_exact reads up
+                               // to n + EXACT_SUB - 1, the scans up to n + 63
 
 // One series of a batch, as the kernels see it (rebuilt per length: N and thr change).
 struct SerDev {
@@ -79,12 +81,12 @@ inline size_t scan_lds_bytes(int m)
 // LDS bytes of k_exact for groups of up to nU candidates of length m.
 inline size_t exact_lds_bytes(int m, int nU)
 {
-    return ((size_t)nU * m + 256 + m + 8) * 8;
+    return ((size_t)nU * m + EXACT_SUB + m + 8) * 8;
 }
-// Candidates per exact group that fit 64 KiB of LDS at length m.
+// Candidates per exact group that fit 64 KiB of LDS at length m (>= 1 for m <= MAXM).
 inline int exact_max_u(int m)
 {
-    int u = (int)((65536 / 8 - 264 - m) / m);
+    int u = (int)((65536 / 8 - EXACT_SUB - 8 - m) / m);
     return u < 1 ? 1 : (u > MAXU ? MAXU : u);
 }
 

@@ -660,8 +660,8 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
 {
     extern __shared__ double dsh[];
     const int m = E.m;
-    double *xs = dsh;                   // [256 + m]   raw x as double
-    double *A = dsh + (256 + m + 8);    // [nU][m]     centred candidate windows
+    double *xs = dsh;                         // [EXACT_SUB + m]  raw x as double
+    double *A = dsh + (EXACT_SUB + m + 8);    // [nU][m]          centred candidate windows
     __shared__ double ainv[MAXU];
     __shared__ unsigned long long wmin[MAXU][4];
     const int t = threadIdx.x;
@@ -688,37 +688,60 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     unsigned long long kmin[MAXU];
 #pragma unroll
     for (int u = 0; u < MAXU; u++) kmin[u] = ~0ull;
-    for (int sub = 0; sub < EXACT_COLS / 256; sub++) {
-        const int64_t c = cblk + sub * 256;
+    // Sub-tiles of EXACT_SUB columns; thread t owns columns c + t + 256 q, q < 4 (conflict-free
+    // LDS reads), and accumulates all nU candidates for each: 4 + 4 nU VALU per k against
+    // 4 + nU LDS reads.
+    for (int sub = 0; sub < EXACT_COLS / EXACT_SUB; sub++) {
+        const int64_t c = cblk + sub * EXACT_SUB;
         if (c >= N) break;
         __syncthreads();  // previous sub-tile's readers are done with xs
-        for (int k = t; k < 256 + m; k += 256) xs[k] = (double)x[c + k];
+        for (int k = t; k < EXACT_SUB + m; k += 256) xs[k] = (double)x[c + k];
         __syncthreads();
-        const int64_t j = c + t;
-        const double2 sj = (j < N) ? st64[j] : make_double2(0.0, 0.0);
-        double acc[MAXU];
+        double muj[4], invj[4];
 #pragma unroll
-        for (int u = 0; u < MAXU; u++) acc[u] = 0.0;
+        for (int q = 0; q < 4; q++) {
+            const int64_t j = c + t + 256 * q;
+            const double2 sj = (j < N) ? st64[j] : make_double2(0.0, 0.0);
+            muj[q] = sj.x;
+            invj[q] = sj.y;
+        }
+        double acc[MAXU][4];
+#pragma unroll
+        for (int u = 0; u < MAXU; u++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[u][q] = 0.0;
+        const double *xp = xs + t;
         for (int k = 0; k < m; k++) {
-            const double b = xs[t + k] - sj.x;
+            double b[4];
 #pragma unroll
-            for (int u = 0; u < MAXU; u++)
-                if (u < nU) acc[u] = fma(A[u * m + k], b, acc[u]);
+            for (int q = 0; q < 4; q++) b[q] = xp[k + 256 * q] - muj[q];
+#pragma unroll
+            for (int u = 0; u < MAXU; u++) {
+                if (u < nU) {
+                    const double a = A[u * m + k];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) acc[u][q] = fma(a, b[q], acc[u][q]);
+                }
+            }
         }
 #pragma unroll
         for (int u = 0; u < MAXU; u++) {
             if (u >= nU) break;
             const int64_t i = E.cand[u0 + u];
-            double ed = DBL_MAX;
-            const bool valid = (j < N) && ((j - i >= m - 1) || (i - j >= m + 1));
-            if (valid) {
-                if (ainv[u] > 0.0 && sj.y > 0.0)
-                    ed = 2.0 * m * (1.0 - acc[u] * (ainv[u] * sj.y));
-                else
-                    ed = 2.0 * m;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int64_t j = c + t + 256 * q;
+                double ed = DBL_MAX;
+                const bool valid = (j < N) && ((j - i >= m - 1) || (i - j >= m + 1));
+                if (valid) {
+                    if (ainv[u] > 0.0 && invj[q] > 0.0)
+                        ed = 2.0 * m * (1.0 - acc[u][q] * (ainv[u] * invj[q]));
+                    else
+                        ed = 2.0 * m;
+                }
+                const unsigned long long key = ord_key64(ed);
+                kmin[u] = key < kmin[u] ? key : kmin[u];
             }
-            const unsigned long long key = ord_key64(ed);
-            kmin[u] = key < kmin[u] ? key : kmin[u];
         }
     }
     const int lane = t & 63, w = t >> 6;
