@@ -77,6 +77,19 @@ typedef struct {
 typedef struct paddi_ctx paddi_ctx;
 typedef struct paddi_series paddi_series;
 
+/* Schedule tunables of a context (chosen by paddi_amd/autotune.py; no counterpart in the
+ * reference, whose tile shapes are compile-time macros, src/parameters.hpp:4-24). */
+typedef struct {
+    int64_t select_window;   /* columns each 64-row block scans in k_scan_select */
+    int64_t refine_chunk;    /* longest k_scan_refine work item (columns) */
+    int64_t min_chunk;       /* shortest k_scan_refine work item (columns) */
+    int64_t refine_items;    /* target work items per refine round */
+    int32_t witness_tiles;   /* witness ranges scanned first (0 = off) */
+    int32_t speculate;       /* candidates refined per exact round */
+    int32_t use_graph;       /* capture the per-length device pipeline in a hipGraph */
+    int32_t reserved;
+} paddi_tunables;
+
 const char *paddi_status_str(paddi_status s);
 const char *paddi_version(void);
 /* 1 if this build supports PADDI_MODE_* `mode` (the learned mode needs its model).
@@ -87,6 +100,8 @@ int32_t paddi_has_mode(int32_t mode);
  * src/main.cu:54-125 (the library owns all device buffers). */
 paddi_status paddi_create(int32_t device, paddi_ctx **out);
 void paddi_destroy(paddi_ctx *ctx);
+paddi_status paddi_get_tunables(paddi_ctx *ctx, paddi_tunables *out);
+paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t);
 
 /* Sliding mean and standard deviation of the n-m+1 windows of T.
  * Replaces compute_statistics / compute_statistics_cpu

@@ -353,6 +353,8 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
     A.WA = select_window_;
     A.CH = refine_chunk_;
     A.m = m;
+    A.U0 = round_up(m + 64, 64);
+    A.witness_max = witness_tiles_;
     const size_t lds = scan_lds_bytes(m);
 
     hipLaunchKernelGGL(k_fill_alive, dim3(grid1(nwords, 256)), dim3(256), 0, stream_, A.alive,
@@ -377,11 +379,13 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
     // rounds the live blocks are listed again, so far columns are launched only for the
     // few blocks the near ones did not kill.  Chunk size: about refine_waves_ work items
     // per round (16 waves per SIMD), within [min_chunk_, refine_chunk_] columns.
-    const int64_t bounds[4] = {A.WA, A.WA + 2 * refine_chunk_, A.WA + 8 * refine_chunk_, nmax};
+    const int64_t U0 = A.U0;
+    const int64_t bounds[4] = {U0 + A.WA, U0 + A.WA + 2 * refine_chunk_,
+                               U0 + A.WA + 8 * refine_chunk_, nmax + U0};
     HIPC(hipEventRecord(ev_[2], stream_));
     int rounds = 0;
     for (int rd = 0; rd < 3 && nblk > 0; rd++) {
-        const int64_t ub = bounds[rd], ue = std::min(bounds[rd + 1], nmax);
+        const int64_t ub = bounds[rd], ue = std::min(bounds[rd + 1], nmax + U0);
         if (ub >= ue) continue;
         if (rounds > 0) {
             hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, stream_, cnt_.as<int>(), 1, 0);
@@ -609,6 +613,7 @@ void Engine::plain(Batch &B, int sid, int minL, int maxL, int step, int k, const
             r2v[(size_t)sid] = r2;
             pass(B, m, r2v, false, surv, st);
             tk[(size_t)sid].reset(std::move(surv[(size_t)sid]), r2, m, k);
+            tk[(size_t)sid].spec = spec_;
             topk_all(B, m, tk, st);
             att++;
             const std::vector<Cand> &acc = tk[(size_t)sid].acc;
@@ -683,8 +688,12 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
             if (!any) break;
             pass(B, m, r2p, true, surv, st);
             for (int q = 0; q < ns; q++)
-                if (active[(size_t)q]) tk[(size_t)q].reset(std::move(surv[(size_t)q]), r2[(size_t)q], m, k);
-                else tk[(size_t)q].done = true;
+                if (active[(size_t)q]) {
+                    tk[(size_t)q].reset(std::move(surv[(size_t)q]), r2[(size_t)q], m, k);
+                    tk[(size_t)q].spec = spec_;
+                } else {
+                    tk[(size_t)q].done = true;
+                }
             topk_all(B, m, tk, st);
             for (int q = 0; q < ns; q++) {
                 if (!active[(size_t)q]) continue;
@@ -783,6 +792,33 @@ void paddi_destroy(paddi_ctx *ctx)
     if (!ctx) return;
     delete ctx->eng;
     delete ctx;
+}
+
+paddi_status paddi_get_tunables(paddi_ctx *ctx, paddi_tunables *out)
+{
+    if (!ctx || !out) return PADDI_ERR_INVALID;
+    const paddi::Engine &E = *ctx->eng;
+    *out = paddi_tunables{E.select_window_, E.refine_chunk_, E.min_chunk_, E.refine_waves_,
+                          E.witness_tiles_, E.spec_, E.use_graph_ ? 1 : 0, 0};
+    return PADDI_OK;
+}
+
+paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
+{
+    if (!ctx || !t) return PADDI_ERR_INVALID;
+    if (t->select_window < 64 || t->refine_chunk < 64 || t->min_chunk < 64 ||
+        t->min_chunk > t->refine_chunk || t->refine_items < 1 || t->witness_tiles < 0 ||
+        t->witness_tiles > 16 || t->speculate < 1 || t->speculate > 64)
+        return PADDI_ERR_INVALID;
+    paddi::Engine &E = *ctx->eng;
+    E.select_window_ = (t->select_window + 63) / 64 * 64;
+    E.refine_chunk_ = (t->refine_chunk + 63) / 64 * 64;
+    E.min_chunk_ = (t->min_chunk + 63) / 64 * 64;
+    E.refine_waves_ = t->refine_items;
+    E.witness_tiles_ = t->witness_tiles;
+    E.spec_ = t->speculate;
+    E.use_graph_ = t->use_graph != 0;
+    return PADDI_OK;
 }
 
 paddi_status paddi_series_upload(paddi_ctx *ctx, const float *T, int64_t n, paddi_series **out)

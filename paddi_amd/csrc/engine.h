@@ -98,6 +98,9 @@ public:
     int64_t refine_chunk_ = 65536;   // longest k_scan_refine work item (columns)
     int64_t min_chunk_ = 2048;       // shortest k_scan_refine work item (columns)
     int64_t refine_waves_ = 16384;   // target number of k_scan_refine work items
+    int witness_tiles_ = 4;          // witness ranges scanned first by k_scan_select
+    int spec_ = 4;                   // candidates refined per exact round
+    bool use_graph_ = false;         // per-length device pipeline as a hipGraph
     bool debug_ = false;             // per-pass trace on stderr (PADDI_DEBUG=1)
 
 private:

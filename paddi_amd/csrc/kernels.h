@@ -36,7 +36,9 @@ struct ScanArgs {
     unsigned long long *pairs;
     int64_t WA;               // columns right of a block scanned by k_scan_select
     int64_t CH;               // columns per k_scan_refine work item
+    int64_t U0;               // first cyclic offset scanned: round_up(m + 64, 64)
     int m;
+    int witness_max;          // witness ranges scanned first
 };
 
 struct ExactArgs {
@@ -76,7 +78,7 @@ __global__ void k_exact(ExactArgs E);
 // LDS bytes of the scan kernels for length m.
 inline size_t scan_lds_bytes(int m)
 {
-    return 64 * 16 + (size_t)((m + 3) & ~3) * 4 + (size_t)(64 + m + 8) * 4;
+    return 64 * 16 + 2 * (size_t)((m + 3) & ~3) * 4 + (size_t)(64 + m + 8) * 4;
 }
 // LDS bytes of k_exact for groups of up to nU candidates of length m.
 inline size_t exact_lds_bytes(int m, int nU)

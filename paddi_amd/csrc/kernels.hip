@@ -194,13 +194,15 @@ __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
 
 struct ScanLds {
     float *xs;    // [64 + m + 4]  x - ref over the current window
-    float *q;     // [m]           uniform centred vector
+    float *q;     // [m]           the block's first row, centred (seeds; staged once)
+    float *qi;    // [m]           a column window, centred (chain initialisation)
     float4 *tile; // [64]          (df, dg, invn, seed) of the tile's columns
 };
 
-// lane t: sum_k q[k] * (x[base+t+k] - mu_{base+t}), using x - ref staged in LDS.
-__device__ __forceinline__ float corr_lanes(const ScanLds &L, const float *__restrict__ x,
-                                            int64_t base, int m, float qsum, float mu_lane)
+// lane t: sum_k v[k] * (x[base+t+k] - mu_{base+t}), using x - ref staged in LDS.
+__device__ __forceinline__ float corr_lanes(const ScanLds &L, const float *v,
+                                            const float *__restrict__ x, int64_t base, int m,
+                                            float vsum, float mu_lane)
 {
     const int t = threadIdx.x;
     const float ref = x[base];
@@ -209,27 +211,27 @@ __device__ __forceinline__ float corr_lanes(const ScanLds &L, const float *__res
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
     int k = 0;
     for (; k + 4 <= m; k += 4) {
-        const float4 qv = *reinterpret_cast<const float4 *>(L.q + k);
+        const float4 qv = *reinterpret_cast<const float4 *>(v + k);
         a0 = fmaf(qv.x, L.xs[t + k], a0);
         a1 = fmaf(qv.y, L.xs[t + k + 1], a1);
         a2 = fmaf(qv.z, L.xs[t + k + 2], a2);
         a3 = fmaf(qv.w, L.xs[t + k + 3], a3);
     }
-    for (; k < m; k++) a0 = fmaf(L.q[k], L.xs[t + k], a0);
+    for (; k < m; k++) a0 = fmaf(v[k], L.xs[t + k], a0);
     const float acc = (a0 + a1) + (a2 + a3);
     __syncthreads();
-    return acc - (mu_lane - ref) * qsum;
+    return acc - (mu_lane - ref) * vsum;
 }
 
-// Stage q[k] = x[p+k] - mu_p and return sum q (wave-reduced, uniform).
-__device__ __forceinline__ float stage_centred(const ScanLds &L, const float *__restrict__ x,
+// Stage dst[k] = x[p+k] - mu_p and return sum dst (wave-reduced, uniform).
+__device__ __forceinline__ float stage_centred(float *dst, const float *__restrict__ x,
                                                int64_t p, int m, float mu_p)
 {
     const int t = threadIdx.x;
     float s = 0.f;
     for (int k = t; k < m; k += WAVE) {
         const float v = x[p + k] - mu_p;
-        L.q[k] = v;
+        dst[k] = v;
         s += v;
     }
 #pragma unroll
@@ -245,6 +247,7 @@ struct RowState {
     float mx;      // max over valid columns of cov*invn_j
     uint64_t alive;// wave-uniform alive mask of the 64 rows
     uint32_t tiles;// 64x64 tiles evaluated (pair counter)
+    float qsum;    // sum of the staged, centred first row (L.q)
 };
 
 // One series' view of the batch.
@@ -340,12 +343,11 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
         const int64_t cI = c0 > 0 ? c0 - 1 : 0;
         const int64_t rb = c0 > 0 ? s : s + 1;
         const float muc = V.colv[cI].w;
-        const float qs = stage_centred(L, V.x, cI, m, muc);
-        R.cov = corr_lanes(L, V.x, rb, m, qs, V.colv[rb + t].w);
+        const float qs = stage_centred(L.qi, V.x, cI, m, muc);
+        R.cov = corr_lanes(L, L.qi, V.x, rb, m, qs, V.colv[rb + t].w);
     }
-    // Seeds use the centred first row of the block.
-    const float murow = V.colv[s].w;
-    const float qsum = stage_centred(L, V.x, s, m, murow);
+    // Seeds use the centred first row of the block, staged once per kernel in L.q.
+    const float qsum = R.qsum;
     const uint64_t word = (uint64_t)(s >> 6);
     float px[PX];
 #pragma unroll
@@ -410,24 +412,30 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
     return true;
 }
 
-// Columns at cyclic offsets [u0, u1) from the block start s: column (s + u) mod N.  u0 is a
-// multiple of 64, so the unwrapped part starts aligned; the wrapped part starts at the
-// aligned-down column, re-scanning at most 63 columns (harmless: same values, same
-// decisions).  Covering u in [0, N) scans every column once.
+// Columns at cyclic offsets [u0, u1) from the block start s: column (s + u) mod N, with the
+// offsets clamped to [U0, N + U0) -- every column exactly once.  Scanning starts at
+// U0 = round_up(m + 64, 64), past the exclusion band right of the block, so the early tiles
+// that kill most blocks take the unmasked path; the band comes last, wrapped.  u0 is a
+// multiple of 64: an unwrapped piece starts aligned, a wrapped one at the aligned-down
+// column (re-scanning at most 63 columns is harmless: same values, same decisions).
 template <int PX>
 __device__ bool scan_cyclic(const ScanLds &L, RowState &R, const View &V, int64_t s, int64_t u0,
-                            int64_t u1)
+                            int64_t u1, int64_t U0)
 {
     const int64_t N = V.N;
-    u1 = u1 < N ? u1 : N;
+    u0 = u0 > U0 ? u0 : U0;
+    u1 = u1 < N + U0 ? u1 : N + U0;
     if (u0 >= u1) return true;
-    const int64_t a = s + u0, b = s + u1;
-    bool live = true;
-    if (a < N) live = scan_range<PX>(L, R, V, s, a, b < N ? b : N);
-    if (live && b > N) {
-        const int64_t a2 = (a > N ? a - N : 0) & ~(int64_t)63;
-        live = scan_range<PX>(L, R, V, s, a2, b - N);
+    int64_t c = s + u0;
+    bool wrapped = false;
+    while (c >= N) {
+        c -= N;
+        wrapped = true;
     }
+    const int64_t e = c + (u1 - u0);
+    if (wrapped) c &= ~(int64_t)63;
+    bool live = scan_range<PX>(L, R, V, s, c, e < N ? e : N);
+    if (live && e > N) live = scan_range<PX>(L, R, V, s, 0, e - N);
     return live;
 }
 
@@ -446,6 +454,14 @@ __device__ __forceinline__ void load_rows(RowState &R, const View &V, int64_t s,
     R.tiles = 0;
 }
 
+// Row terms plus the block's centred first row in L.q (the seeds' left operand).
+__device__ __forceinline__ void load_block(const ScanLds &L, RowState &R, const View &V,
+                                           int64_t s, float thr_rho)
+{
+    load_rows(R, V, s, thr_rho);
+    R.qsum = stage_centred(L.q, V.x, s, V.m, V.colv[s].w);
+}
+
 __device__ __forceinline__ void flush_best(const RowState &R, const View &V, int64_t s)
 {
     const int t = threadIdx.x;
@@ -461,7 +477,8 @@ __device__ __forceinline__ ScanLds carve_lds(int m)
     ScanLds L;
     L.tile = lds4;
     L.q = reinterpret_cast<float *>(lds4 + WAVE);
-    L.xs = L.q + ((m + 3) & ~3);
+    L.qi = L.q + ((m + 3) & ~3);
+    L.xs = L.qi + ((m + 3) & ~3);
     return L;
 }
 
@@ -502,24 +519,31 @@ __global__ __launch_bounds__(64) void k_scan_select(ScanArgs A)
     R.alive = __hip_atomic_load(V.alive + (s >> 6), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (R.alive == 0) return;
     ScanLds L = carve_lds(V.m);
-    load_rows(R, V, s, thr_rho);
+    load_block(L, R, V, s, thr_rho);
     bool live = true;
     if (A.wit_read) {
         // Witness tiles: the tiles that pruned these rows at the previous length.  Scan up
-        // to four distinct ones first.
+        // to four distinct ones first; a run of adjacent witness tiles is one range (one
+        // chain initialisation).
         int w = A.wit_read[gb * WAVE + threadIdx.x];
         const int64_t lim = V.N;
-        for (int rep = 0; rep < 4 && live; rep++) {
+        for (int rep = 0; rep < A.witness_max && live; rep++) {
             const uint64_t has = __builtin_amdgcn_ballot_w64(w >= 0 && w < lim && !(w & 63));
             if (!has) break;
             const int lane = __builtin_ctzll(has);
             const int wt = __shfl(w, lane, 64);
+            int we = wt + WAVE;
             if (w == wt) w = -1;
-            const int64_t e = (wt + WAVE < lim) ? wt + WAVE : lim;
+            for (int ext = 0; ext < 3; ext++) {
+                if (!__builtin_amdgcn_ballot_w64(w == we)) break;
+                if (w == we) w = -1;
+                we += WAVE;
+            }
+            const int64_t e = (we < lim) ? we : lim;
             live = scan_range<PX>(L, R, V, s, wt, e);
         }
     }
-    if (live) live = scan_cyclic<PX>(L, R, V, s, 0, A.WA);
+    if (live) live = scan_cyclic<PX>(L, R, V, s, A.U0, A.U0 + A.WA, A.U0);
     if (live) flush_best(R, V, s);
     count_pairs(R, A.pairs);
 }
@@ -542,13 +566,13 @@ __global__ __launch_bounds__(64) void k_scan_refine(ScanArgs A, const int *__res
     const int64_t u0 = u_begin + q * A.CH;
     int64_t u1 = u0 + A.CH;
     u1 = u1 < u_end ? u1 : u_end;
-    if (u0 >= u1 || u0 >= V.N) return;
+    if (u0 >= u1 || u0 >= V.N + A.U0) return;
     RowState R;
     R.alive = __hip_atomic_load(V.alive + (s >> 6), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (R.alive == 0) return;
     ScanLds L = carve_lds(V.m);
-    load_rows(R, V, s, thr_rho);
-    const bool live = scan_cyclic<PX>(L, R, V, s, u0, u1);
+    load_block(L, R, V, s, thr_rho);
+    const bool live = scan_cyclic<PX>(L, R, V, s, u0, u1, A.U0);
     if (live) flush_best(R, V, s);
     count_pairs(R, A.pairs);
 }

@@ -58,7 +58,28 @@ class Stats(C.Structure):
                     launches=dict(zip(self.KERNELS, list(self.launches))))
 
 
-EXPORTS = ("paddi_status_str", "paddi_version", "paddi_create", "paddi_destroy",
+class Tunables(C.Structure):
+    _fields_ = [("select_window", C.c_int64), ("refine_chunk", C.c_int64),
+                ("min_chunk", C.c_int64), ("refine_items", C.c_int64),
+                ("witness_tiles", C.c_int32), ("speculate", C.c_int32),
+                ("use_graph", C.c_int32), ("reserved", C.c_int32)]
+    NAMES = ("select_window", "refine_chunk", "min_chunk", "refine_items", "witness_tiles",
+             "speculate", "use_graph")
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k in self.NAMES}
+
+
+TUNED_PATH = os.path.join(_HERE, "model", "tuned.json")
+
+
+def tuned_key(minL, arch="gfx950"):
+    """Autotuner table key: device and length bucket (m < 128, < 256, < 512, >= 512)."""
+    b = 64 if minL < 128 else 128 if minL < 256 else 256 if minL < 512 else 512
+    return f"{arch}/m{b}"
+
+
+EXPORTS = ("paddi_get_tunables", "paddi_set_tunables", "paddi_status_str", "paddi_version", "paddi_create", "paddi_destroy",
            "paddi_compute_statistics", "paddi_range_discords", "paddi_discords",
            "paddi_discords_batch", "paddi_series_upload", "paddi_series_free",
            "paddi_discords_series", "paddi_has_mode", "paddi_mlp_sizes", "paddi_features",
@@ -95,6 +116,8 @@ def load_library(path: str = LIB_PATH):
                                         C.POINTER(i64), C.POINTER(Stats)]
     L.paddi_has_mode.argtypes = [i32]
     L.paddi_has_mode.restype = i32
+    L.paddi_get_tunables.argtypes = [P, C.POINTER(Tunables)]
+    L.paddi_set_tunables.argtypes = [P, C.POINTER(Tunables)]
     L.paddi_mlp_sizes.argtypes = [P, P, P]
     L.paddi_features.argtypes = [P, P, i32, i32, P]
     L.paddi_mlp_forward.argtypes = [P, P, P, P, P, i32, P]
@@ -177,6 +200,34 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    # -- schedule tunables (autotuner) ---------------------------------------------
+
+    def get_tunables(self):
+        t = Tunables()
+        self._check(self._L.paddi_get_tunables(self._c, C.byref(t)), "paddi_get_tunables")
+        return t.as_dict()
+
+    def set_tunables(self, **kw):
+        t = Tunables()
+        self._check(self._L.paddi_get_tunables(self._c, C.byref(t)), "paddi_get_tunables")
+        for k, v in kw.items():
+            if k not in Tunables.NAMES:
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        self._check(self._L.paddi_set_tunables(self._c, C.byref(t)), "paddi_set_tunables")
+
+    def apply_tuned(self, minL, path=TUNED_PATH):
+        """Apply the autotuner's entry for this length range, if the table has one."""
+        import json
+        if not os.path.exists(path):
+            return None
+        with open(path) as f:
+            table = json.load(f)
+        ent = table.get(tuned_key(minL))
+        if ent:
+            self.set_tunables(**ent["tunables"])
+        return ent
 
     # -- per-segment contracts ---------------------------------------------------
 

@@ -1,0 +1,59 @@
+"""The engine's schedule tunables (the autotuner's search space) change speed, never results.
+
+Every configuration below must return the discord set the oracle returns on c1 (indices and
+lengths bit-exact, squared distances within 1e-4 relative), and the C ABI must reject
+out-of-range values with PADDI_ERR_INVALID and leave the previous values in place.
+"""
+import json
+import os
+
+import pytest
+
+from paddi_amd import synth
+from paddi_amd.engine import PaddiError
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+RTOL = 1e-4
+
+SCHEDULES = [
+    dict(witness_tiles=0),
+    dict(witness_tiles=8, select_window=2048),
+    dict(select_window=32768, refine_chunk=16384, min_chunk=512, refine_items=4096),
+    dict(refine_chunk=131072, min_chunk=8192, refine_items=65536, speculate=1),
+    dict(speculate=16, witness_tiles=1),
+]
+
+
+@pytest.fixture
+def restore(ctx):
+    saved = ctx.get_tunables()
+    yield
+    ctx.set_tunables(**saved)
+
+
+def test_tunables_roundtrip_and_validation(ctx, restore):
+    t = ctx.get_tunables()
+    assert t["select_window"] % 64 == 0 and t["min_chunk"] <= t["refine_chunk"]
+    ctx.set_tunables(select_window=4000, witness_tiles=2)
+    u = ctx.get_tunables()
+    assert u["select_window"] == 4032 and u["witness_tiles"] == 2   # rounded up to 64
+    for bad in (dict(select_window=0), dict(min_chunk=1 << 20, refine_chunk=1 << 16),
+                dict(witness_tiles=-1), dict(witness_tiles=17), dict(speculate=0)):
+        with pytest.raises(PaddiError):
+            ctx.set_tunables(**bad)
+        assert ctx.get_tunables() == u
+
+
+@pytest.mark.parametrize("mode", ["batched", "learned"])
+@pytest.mark.parametrize("sched", SCHEDULES, ids=lambda s: ",".join(f"{k}={v}" for k, v in s.items()))
+def test_schedule_does_not_change_discords(ctx, restore, sched, mode):
+    w = synth.workload_c1()
+    with open(os.path.join(GOLD, "palmad_c1.json")) as f:
+        ref = json.load(f)
+    exp = [(int(i), r["m"], float(d)) for r in ref for i, d in zip(r["idx"], r["d2"])]
+    ctx.set_tunables(**sched)
+    got = ctx.discords(w["series"][0], w["minL"], w["maxL"], w["step"], w["k"], mode=mode)
+    assert [(g.index, g.length) for g in got] == [(i, m) for i, m, _ in exp]
+    for g, (_, _, d) in zip(got, exp):
+        assert abs(g.nn_dist2 - d) <= RTOL * d
