@@ -1,7 +1,7 @@
 """Autotuner of the engine's schedule (select window, refine chunking, witness tiles,
 exact-refine speculation, hipGraph use).
 
-    python -m paddi_amd.autotune [--workload c3] [--lengths 16] [--mode learned]
+    python -m paddi_amd.autotune [--workload c3] [--lengths 0] [--mode learned]
 
 Coordinate descent: starting from the current values, each tunable in turn is set to each
 of its candidates; a candidate is timed as the median of `--reps` complete searches over
@@ -33,7 +33,6 @@ GRID = {
     "min_chunk": [512, 1024, 2048, 4096, 8192],
     "refine_chunk": [16384, 32768, 65536, 131072],
     "speculate": [1, 2, 4, 8],
-    "use_graph": [0, 1],
 }
 
 
@@ -44,9 +43,9 @@ def key_of(recs):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="c3", choices=sorted(synth.WORKLOADS))
-    ap.add_argument("--lengths", type=int, default=16)
+    ap.add_argument("--lengths", type=int, default=0, help="0 = every length")
     ap.add_argument("--mode", default="learned")
-    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--sweeps", type=int, default=2)
     ap.add_argument("--out", default=TUNED_PATH)
     a = ap.parse_args()
@@ -54,7 +53,8 @@ def main():
     w = synth.WORKLOADS[a.workload]()
     ctx = Context(0)
     hs = [ctx.upload(x) for x in w["series"]]
-    maxL = min(w["maxL"], w["minL"] + (a.lengths - 1) * w["step"])
+    maxL = w["maxL"] if a.lengths <= 0 else \
+        min(w["maxL"], w["minL"] + (a.lengths - 1) * w["step"])
     base = ctx.get_tunables()
 
     def run(tun):

@@ -190,6 +190,8 @@ Engine::Engine(int device) : device_(device)
     HIPC(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (auto &e : ev_) HIPC(hipEventCreate(&e));
     HIPC(hipHostMalloc(&pin_, 64, hipHostMallocDefault));
+    HIPC(hipHostMalloc(&pin_out_, sizeof(PassCtl) + (size_t)SCAP * 12, hipHostMallocDefault));
+    HIPC(hipEventCreateWithFlags(&ser_ev_, hipEventDisableTiming));
     const char *dbg = std::getenv("PADDI_DEBUG");
     debug_ = dbg && dbg[0] == '1';
 }
@@ -197,9 +199,42 @@ Engine::Engine(int device) : device_(device)
 Engine::~Engine()
 {
     (void)hipSetDevice(device_);
+    (void)hipStreamSynchronize(stream_);
+    drop_graphs();
     for (auto &e : ev_) (void)hipEventDestroy(e);
-    if (pin_) (void)hipHostFree(pin_);
+    (void)hipEventDestroy(ser_ev_);
+    for (void *p : {pin_, pin_par_, pin_out_, pin_ser_})
+        if (p) (void)hipHostFree(p);
     (void)hipStreamDestroy(stream_);
+}
+
+void Engine::drop_graphs()
+{
+    for (auto &g : graphs_)
+        if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+    graphs_.clear();
+}
+
+// Pinned staging grows with the batch; a graph that copies from the old buffer is dropped.
+void Engine::ensure_pinned(size_t par_bytes, size_t ser_bytes)
+{
+    if (par_bytes > pin_par_bytes_) {
+        HIPC(hipStreamSynchronize(stream_));
+        drop_graphs();
+        if (pin_par_) HIPC(hipHostFree(pin_par_));
+        pin_par_ = nullptr;
+        pin_par_bytes_ = 0;
+        HIPC(hipHostMalloc(&pin_par_, par_bytes, hipHostMallocDefault));
+        pin_par_bytes_ = par_bytes;
+    }
+    if (ser_bytes > pin_ser_bytes_) {
+        HIPC(hipEventSynchronize(ser_ev_));
+        if (pin_ser_) HIPC(hipHostFree(pin_ser_));
+        pin_ser_ = nullptr;
+        pin_ser_bytes_ = 0;
+        HIPC(hipHostMalloc(&pin_ser_, ser_bytes, hipHostMallocDefault));
+        pin_ser_bytes_ = ser_bytes;
+    }
 }
 
 Series *Engine::upload(const float *T, int64_t n)
@@ -271,12 +306,16 @@ void Engine::build_batch(Batch &B, const std::vector<const Series *> &ss, int mi
     HIPC(hipStreamSynchronize(stream_));  // host vectors above must outlive the copies
 }
 
+// Series table (N at the current length) to the device, ordered on stream_ before the
+// kernels that follow; the host array may change right after the call.
 void Engine::upload_ser(Batch &B)
 {
-    // synchronous with respect to the stream: the host array may change right after
-    HIPC(hipMemcpyAsync(B.ser.p, B.host.data(), B.host.size() * sizeof(SerDev),
-                        hipMemcpyHostToDevice, stream_));
-    HIPC(hipStreamSynchronize(stream_));
+    const size_t bytes = B.host.size() * sizeof(SerDev);
+    ensure_pinned(0, bytes);
+    HIPC(hipEventSynchronize(ser_ev_));  // the previous copy out of pin_ser_ has finished
+    std::memcpy(pin_ser_, B.host.data(), bytes);
+    HIPC(hipMemcpyAsync(B.ser.p, pin_ser_, bytes, hipMemcpyHostToDevice, stream_));
+    HIPC(hipEventRecord(ser_ev_, stream_));
 }
 
 void Engine::prefix(Batch &B)
@@ -299,18 +338,66 @@ void Engine::derive(Batch &B, int m, paddi_stats *st)
                        B.refs.as<double>(), B.rows, m, colv_.as<float4>(), st64_.as<double2>());
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ev_[5], stream_));
-    if (st) {
-        HIPC(hipEventSynchronize(ev_[5]));
-        float a = 0.f;
-        HIPC(hipEventElapsedTime(&a, ev_[4], ev_[5]));
-        st->kernel_ms[0] += a;
-        st->launches[0] += 1;
-    }
+    derive_st_ = st;  // timed at the next synchronisation (collect_derive)
+}
+
+void Engine::collect_derive()
+{
+    if (!derive_st_) return;
+    HIPC(hipEventSynchronize(ev_[5]));
+    float a = 0.f;
+    HIPC(hipEventElapsedTime(&a, ev_[4], ev_[5]));
+    derive_st_->kernel_ms[0] += a;
+    derive_st_->launches[0] += 1;
+    derive_st_ = nullptr;
 }
 
 void Engine::copy_st64(double2 *dst, int64_t N)
 {
-    HIPC(hipMemcpy(dst, st64_.p, (size_t)N * sizeof(double2), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpyAsync(dst, st64_.p, (size_t)N * sizeof(double2), hipMemcpyDeviceToHost,
+                        stream_));
+    HIPC(hipStreamSynchronize(stream_));
+}
+
+// The device work of one pass, in stream order: parameter upload, k_pass_init, k_scan_select,
+// then REFINE_ROUNDS k_scan_refine rounds each preceded by a listing of the live
+// blocks, k_survivors, and the download of the counters and the first scap survivors.  No
+// step depends on a value read back by the host, so the sequence is captured once per
+// (px, lds) class and replayed.  Eager launches are bracketed by events for the statistics;
+// a captured sequence has none (event nodes of a graph do not time the kernels between them).
+void Engine::enqueue_pass(const ScanPtrs &Q, int px, size_t lds, size_t parb, int64_t rows,
+                          int64_t nwords, int64_t scap, bool timed)
+{
+    const PassDev *P = par_.as<PassDev>();
+    PassCtl *ctl = ctl_.as<PassCtl>();
+    // k_scan_refine needs more workgroups than listed blocks (see there)
+    const unsigned grid =
+        (unsigned)std::max<int64_t>(refine_waves_ + refine_waves_ / 2, nwords + 64);
+    HIPC(hipMemcpyAsync(par_.p, pin_par_, parb, hipMemcpyHostToDevice, stream_));
+    hipLaunchKernelGGL(k_pass_init, dim3(grid1(std::max<int64_t>(rows, 64), 256)), dim3(256), 0,
+                       stream_, Q, P, rows, nwords, ctl);
+    if (timed) HIPC(hipEventRecord(ev_[0], stream_));
+    launch_scan_select(px, Q, P, (unsigned)nwords, lds, stream_);
+    if (timed) HIPC(hipEventRecord(ev_[1], stream_));
+    hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
+                       (const uint64_t *)alive_.p, nwords, blist_.as<int>(), &ctl->list[0]);
+    if (timed) HIPC(hipEventRecord(ev_[2], stream_));
+    for (int r = 0; r < REFINE_ROUNDS; r++) {
+        if (r > 0)
+            hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
+                               (const uint64_t *)alive_.p, nwords, blist_.as<int>(),
+                               &ctl->list[r]);
+        launch_scan_refine(px, Q, P, blist_.as<int>(), ctl, r, grid, lds, stream_);
+    }
+    if (timed) HIPC(hipEventRecord(ev_[3], stream_));
+    hipLaunchKernelGGL(k_survivors, dim3(grid1(rows, 256)), dim3(256), 0, stream_, Q, P, rows,
+                       ctl, sidx_.as<int64_t>(), sd2_.as<float>(), rows);
+    HIPC(hipMemcpyAsync(pin_out_, ctl_.p, sizeof(PassCtl), hipMemcpyDeviceToHost, stream_));
+    char *o = (char *)pin_out_ + sizeof(PassCtl);
+    HIPC(hipMemcpyAsync(o, sidx_.p, (size_t)scap * 8, hipMemcpyDeviceToHost, stream_));
+    HIPC(hipMemcpyAsync(o + (size_t)SCAP * 8, sd2_.p, (size_t)scap * 4, hipMemcpyDeviceToHost,
+                        stream_));
+    HIPC(hipGetLastError());
 }
 
 // One (m, r2_s) pass: survivors per series.  Series with r2 < 0 are skipped (no rows).
@@ -318,120 +405,125 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
                   std::vector<std::vector<Cand>> &out, paddi_stats *st)
 {
     const double delta = delta_ed2(m);
-    for (int q = 0; q < B.nser; q++) {
-        SerDev &S = B.host[(size_t)q];
-        S.N = r2[(size_t)q] < 0.0 ? 0 : B.n[(size_t)q] - m + 1;
-        S.thr = (float)(1.0 - (r2[(size_t)q] - delta) / (2.0 * m));
-    }
-    upload_ser(B);
     const int64_t rows = B.rows, nwords = B.nblocks;
+    const int px = scan_px(m);
+    // a graph is shared by every length of its class, so it carries the class's largest LDS
+    const int mcls = px < 33 ? 64 * px - 64 : (int)round_up(m, 256);
+    const size_t lds = use_graph_ ? scan_lds_bytes(std::min(mcls, MAXM)) : scan_lds_bytes(m);
+    const size_t parb = sizeof(PassDev) + (size_t)B.nser * sizeof(SerDev);
+    const int64_t scap = std::min<int64_t>(SCAP, rows);
     alive_.reserve((size_t)nwords * 8);
     best_.reserve((size_t)rows * 4);
     blist_.reserve((size_t)nwords * 4);
-    cnt_.reserve(32);
+    ctl_.reserve(sizeof(PassCtl));
     sidx_.reserve((size_t)rows * 8);
     sd2_.reserve((size_t)rows * 4);
-    int *wread = nullptr, *wwrite = nullptr;
-    if (witness) {
-        wit_[0].reserve((size_t)rows * 4);
-        wit_[1].reserve((size_t)rows * 4);
-        wread = wit_[witcur_ ^ 1].as<int>();
-        wwrite = wit_[witcur_].as<int>();
-        hipLaunchKernelGGL(k_fill_int, dim3(grid1(rows, 256)), dim3(256), 0, stream_, wwrite,
-                           rows, -1);
-    }
-    ScanArgs A{};
-    A.x = B.x.as<float>();
-    A.ser = B.ser.as<SerDev>();
-    A.bser = B.bser.as<int>();
-    A.colv = colv_.as<float4>();
-    A.alive = alive_.as<uint64_t>();
-    A.best = best_.as<int>();
-    A.wit = wwrite;
-    A.wit_read = wread_valid_ ? wread : nullptr;
-    A.pairs = (unsigned long long *)((char *)cnt_.p + 16);
-    A.WA = select_window_;
-    A.CH = refine_chunk_;
-    A.m = m;
-    A.U0 = round_up(m + 64, 64);
-    A.witness_max = witness_tiles_;
-    const size_t lds = scan_lds_bytes(m);
+    wit_[0].reserve((size_t)rows * 4);
+    wit_[1].reserve((size_t)rows * 4);
+    par_.reserve(parb);
+    ensure_pinned(parb, 0);
 
-    hipLaunchKernelGGL(k_fill_alive, dim3(grid1(nwords, 256)), dim3(256), 0, stream_, A.alive,
-                       A.ser, A.bser, nwords);
-    hipLaunchKernelGGL(k_fill_int, dim3(grid1(rows, 256)), dim3(256), 0, stream_, A.best, rows,
-                       (int)0x807fffff /* ord_key(-inf) */);
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, stream_, cnt_.as<int>(), 8, 0);
-    HIPC(hipEventRecord(ev_[0], stream_));
-    launch_scan_select(A, (unsigned)nwords, lds, stream_);
-    HIPC(hipGetLastError());
-    HIPC(hipEventRecord(ev_[1], stream_));
-    hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
-                       (const uint64_t *)A.alive, nwords, blist_.as<int>(), cnt_.as<int>());
-    HIPC(hipMemcpyAsync(pin_, cnt_.p, 32, hipMemcpyDeviceToHost, stream_));
-    HIPC(hipStreamSynchronize(stream_));
-    int nblk = ((int *)pin_)[0];
-    const int nblk_select = nblk;
-    const unsigned long long pairs_select = ((unsigned long long *)pin_)[2];
+    // parameters: the pass's series table (N = 0 switches a series off) and scan arguments
+    PassDev &P = *(PassDev *)pin_par_;
+    SerDev *S = (SerDev *)((char *)pin_par_ + sizeof(PassDev));
     int64_t nmax = 0;
-    for (int q = 0; q < B.nser; q++) nmax = std::max(nmax, B.host[(size_t)q].N);
-    // Refine rounds over cyclic offsets [WA, N) from each block, nearest first; between
-    // rounds the live blocks are listed again, so far columns are launched only for the
-    // few blocks the near ones did not kill.  Chunk size: about refine_waves_ work items
-    // per round (16 waves per SIMD), within [min_chunk_, refine_chunk_] columns.
-    const int64_t U0 = A.U0;
-    const int64_t bounds[4] = {U0 + A.WA, U0 + A.WA + 2 * refine_chunk_,
-                               U0 + A.WA + 8 * refine_chunk_, nmax + U0};
-    HIPC(hipEventRecord(ev_[2], stream_));
-    int rounds = 0;
-    for (int rd = 0; rd < 3 && nblk > 0; rd++) {
-        const int64_t ub = bounds[rd], ue = std::min(bounds[rd + 1], nmax + U0);
-        if (ub >= ue) continue;
-        if (rounds > 0) {
-            hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, stream_, cnt_.as<int>(), 1, 0);
-            hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
-                               (const uint64_t *)A.alive, nwords, blist_.as<int>(),
-                               cnt_.as<int>());
-            HIPC(hipMemcpyAsync(pin_, cnt_.p, 4, hipMemcpyDeviceToHost, stream_));
-            HIPC(hipStreamSynchronize(stream_));
-            nblk = ((int *)pin_)[0];
-            if (nblk == 0) break;
-        }
-        const int64_t len = ue - ub;
-        int64_t ch = round_up((len * (int64_t)nblk + refine_waves_ - 1) / refine_waves_, 64);
-        A.CH = std::min<int64_t>(refine_chunk_, std::max<int64_t>(min_chunk_, ch));
-        const int nchunks = (int)((len + A.CH - 1) / A.CH);
-        launch_scan_refine(A, (const int *)blist_.p, nchunks, ub, ue,
-                           (unsigned)((int64_t)nblk * nchunks), lds, stream_);
-        HIPC(hipGetLastError());
-        rounds++;
+    for (int q = 0; q < B.nser; q++) {
+        S[q] = B.host[(size_t)q];
+        S[q].N = r2[(size_t)q] < 0.0 ? 0 : B.n[(size_t)q] - m + 1;
+        S[q].thr = (float)(1.0 - (r2[(size_t)q] - delta) / (2.0 * m));
+        nmax = std::max(nmax, S[q].N);
     }
-    HIPC(hipEventRecord(ev_[3], stream_));
-    hipLaunchKernelGGL(k_survivors, dim3(grid1(rows, 256)), dim3(256), 0, stream_,
-                       (const uint64_t *)A.alive, (const int *)A.best, rows, m,
-                       cnt_.as<int>() + 1, sidx_.as<int64_t>(), sd2_.as<float>(), rows);
-    HIPC(hipGetLastError());
-    HIPC(hipMemcpyAsync(pin_, cnt_.p, 32, hipMemcpyDeviceToHost, stream_));
+    ScanPtrs Q{};
+    Q.x = B.x.as<float>();
+    Q.ser = (const SerDev *)((char *)par_.p + sizeof(PassDev));
+    Q.bser = B.bser.as<int>();
+    Q.colv = colv_.as<float4>();
+    Q.alive = alive_.as<uint64_t>();
+    Q.best = best_.as<int>();
+    Q.wit0 = wit_[0].as<int>();
+    Q.wit1 = wit_[1].as<int>();
+    Q.pairs = ctl_.as<PassCtl>()->pairs;
+    ScanScal &A = P.S;
+    A = ScanScal{};
+    A.WA = select_window_;
+    A.U0 = round_up(m + 64, 64);
+    A.m = m;
+    A.witness_max = witness_tiles_;
+    A.wsel = witness ? witcur_ : -1;
+    A.rsel = (witness && wread_valid_) ? (witcur_ ^ 1) : -1;
+    // Refine rounds over cyclic offsets [U0 + WA, N + U0) from each block, nearest first;
+    // between rounds the live blocks are listed again, so far columns are scanned only for
+    // the few blocks the near ones did not kill.
+    RefineArgs &R = P.R;
+    const int64_t end = nmax + A.U0;
+    const int64_t raw[4] = {A.U0 + A.WA, A.U0 + A.WA + 2 * refine_chunk_,
+                            A.U0 + A.WA + 8 * refine_chunk_, end};
+    for (int r = 0; r < 4; r++) R.bound[r] = std::min(raw[r], end);
+    R.min_chunk = min_chunk_;
+    R.max_chunk = refine_chunk_;
+    R.items = refine_waves_;
+
+    if (use_graph_) {
+        const std::vector<uintptr_t> sig = {
+            (uintptr_t)par_.p,   (uintptr_t)pin_par_, (uintptr_t)pin_out_, (uintptr_t)ctl_.p,
+            (uintptr_t)alive_.p, (uintptr_t)blist_.p, (uintptr_t)sidx_.p,  (uintptr_t)sd2_.p,
+            (uintptr_t)Q.x,      (uintptr_t)Q.bser,   (uintptr_t)Q.colv,   (uintptr_t)Q.best,
+            (uintptr_t)Q.wit0,   (uintptr_t)Q.wit1,   (uintptr_t)rows,     (uintptr_t)nwords,
+            (uintptr_t)parb,     (uintptr_t)scap,     (uintptr_t)refine_waves_};
+        GraphEnt &G = graphs_[{px, lds}];
+        if (!G.exec || G.sig != sig) {
+            if (G.exec) HIPC(hipGraphExecDestroy(G.exec));
+            G.exec = nullptr;
+            hipGraph_t g = nullptr;
+            HIPC(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
+            try {
+                enqueue_pass(Q, px, lds, parb, rows, nwords, scap, false);
+            } catch (...) {
+                (void)hipStreamEndCapture(stream_, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPC(hipStreamEndCapture(stream_, &g));
+            const hipError_t e = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPC(e);
+            G.sig = sig;
+        }
+        HIPC(hipGraphLaunch(G.exec, stream_));
+    } else {
+        enqueue_pass(Q, px, lds, parb, rows, nwords, scap, true);
+    }
     HIPC(hipStreamSynchronize(stream_));
-    const int ns = ((int *)pin_)[1];
+    collect_derive();
+    const PassCtl C = *(const PassCtl *)pin_out_;
+    const int ns = C.survivors;
     if (debug_) {
         float a = 0.f, b = 0.f;
-        HIPC(hipEventElapsedTime(&a, ev_[0], ev_[1]));
-        HIPC(hipEventElapsedTime(&b, ev_[2], ev_[3]));
-        const unsigned long long pairs_all = ((unsigned long long *)pin_)[2];
+        if (!use_graph_) {
+            HIPC(hipEventElapsedTime(&a, ev_[0], ev_[1]));
+            HIPC(hipEventElapsedTime(&b, ev_[2], ev_[3]));
+        }
         std::fprintf(stderr,
-                     "[paddi] m=%d select %.2f ms %.3g pairs | refine blocks %d->%d rounds %d "
-                     "%.2f ms %.3g pairs | survivors %d\n",
-                     m, a, (double)pairs_select, nblk_select, nblk, rounds, b,
-                     (double)(pairs_all - pairs_select), ns);
+                     "[paddi] m=%d select %.2f ms %.3g pairs | refine blocks %d/%d/%d %.2f ms "
+                     "%.3g pairs | survivors %d\n",
+                     m, a, (double)C.pairs[0], C.list[0], C.list[1], C.list[2], b,
+                     (double)C.pairs[1], ns);
     }
     out.assign((size_t)B.nser, {});
     if (ns > 0) {
+        const char *o = (const char *)pin_out_ + sizeof(PassCtl);
         std::vector<int64_t> ix((size_t)ns);
         std::vector<float> dd((size_t)ns);
-        HIPC(hipMemcpyAsync(ix.data(), sidx_.p, (size_t)ns * 8, hipMemcpyDeviceToHost, stream_));
-        HIPC(hipMemcpyAsync(dd.data(), sd2_.p, (size_t)ns * 4, hipMemcpyDeviceToHost, stream_));
-        HIPC(hipStreamSynchronize(stream_));
+        const int64_t head = std::min<int64_t>(ns, scap);
+        std::memcpy(ix.data(), o, (size_t)head * 8);
+        std::memcpy(dd.data(), o + (size_t)SCAP * 8, (size_t)head * 4);
+        if (ns > head) {
+            HIPC(hipMemcpyAsync(ix.data() + head, sidx_.as<int64_t>() + head,
+                                (size_t)(ns - head) * 8, hipMemcpyDeviceToHost, stream_));
+            HIPC(hipMemcpyAsync(dd.data() + head, sd2_.as<float>() + head,
+                                (size_t)(ns - head) * 4, hipMemcpyDeviceToHost, stream_));
+            HIPC(hipStreamSynchronize(stream_));
+        }
         for (int q = 0; q < ns; q++) {
             const int64_t g = ix[(size_t)q];
             const int sid = B.bser_host[(size_t)(g >> 6)];
@@ -439,18 +531,18 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
         }
     }
     if (st) {
-        float a = 0.f, b = 0.f;
-        HIPC(hipEventElapsedTime(&a, ev_[0], ev_[1]));
-        HIPC(hipEventElapsedTime(&b, ev_[2], ev_[3]));
-        st->kernel_ms[1] += a;
-        st->launches[1] += 1;
-        if (rounds > 0) {
+        if (!use_graph_) {  // a replayed graph is timed as a whole by the caller only
+            float a = 0.f, b = 0.f;
+            HIPC(hipEventElapsedTime(&a, ev_[0], ev_[1]));
+            HIPC(hipEventElapsedTime(&b, ev_[2], ev_[3]));
+            st->kernel_ms[1] += a;
+            st->launches[1] += 1;
             st->kernel_ms[2] += b;
-            st->launches[2] += 1;
+            st->launches[2] += REFINE_ROUNDS;
         }
         st->attempts += 1;
         st->survivors += ns;
-        st->pairs += (int64_t)((unsigned long long *)pin_)[2];
+        st->pairs += (int64_t)(C.pairs[0] + C.pairs[1]);
     }
 }
 
@@ -460,9 +552,7 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
 {
     out.assign(req.size(), 0.0);
     if (req.empty()) return;
-    // restore N of every series (a pass may have switched series off)
-    for (int q = 0; q < B.nser; q++) B.host[(size_t)q].N = B.n[(size_t)q] - m + 1;
-    upload_ser(B);
+    // the series table on the device is derive(m)'s: every series at N = n - m + 1
     const int umax = exact_max_u(m);
     std::vector<size_t> order(req.size());
     for (size_t q = 0; q < req.size(); q++) order[q] = q;
@@ -517,6 +607,7 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
     std::vector<unsigned long long> keys(cand.size());
     HIPC(hipMemcpyAsync(keys.data(), okey_.p, cand.size() * 8, hipMemcpyDeviceToHost, stream_));
     HIPC(hipStreamSynchronize(stream_));
+    collect_derive();
     for (size_t q = 0; q < order.size(); q++) {
         const unsigned long long k = keys[q];
         const unsigned long long bits = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
@@ -721,6 +812,7 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
 void Engine::run(const std::vector<const Series *> &ss, int minL, int maxL, int step, int k,
                  const paddi_opts &o, std::vector<paddi_discord> &out, paddi_stats *st)
 {
+    derive_st_ = nullptr;
     if (o.mode == PADDI_MODE_PLAIN) {
         // the reference's method looped over series, each alone
         for (size_t q = 0; q < ss.size(); q++) {
@@ -731,12 +823,14 @@ void Engine::run(const std::vector<const Series *> &ss, int minL, int maxL, int 
             for (paddi_discord &d : r) d.series = (int32_t)q;
             out.insert(out.end(), r.begin(), r.end());
         }
+        collect_derive();
         return;
     }
     if (o.mode == PADDI_MODE_LEARNED) set_model_path(o.model_path);
     build_batch(batch_, ss, minL);
     prefix(batch_);
     fast(batch_, minL, maxL, step, k, o, out, st);
+    collect_derive();
 }
 
 }  // namespace paddi

@@ -2,11 +2,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <memory>
 #include <stdint.h>
-#include <vector>
-
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/paddi.h"
 #include "kernels.h"
@@ -16,6 +17,8 @@ namespace paddi {
 
 const char *status_str(paddi_status s);
 double delta_ed2(int m);
+
+constexpr int64_t SCAP = 8192;   // survivors returned with a pass's counters (more: a 2nd copy)
 
 // Grow-only device buffer.
 struct DevBuf {
@@ -122,6 +125,20 @@ public:
     void set_model_path(const char *p);
 
 private:
+    // One pass's device work (parameter upload .. survivor download) on stream_, eagerly or
+    // as a replay of the captured graph of its (px, lds) class.
+    void enqueue_pass(const ScanPtrs &Q, int px, size_t lds, size_t parb, int64_t rows,
+                      int64_t nwords, int64_t scap, bool timed);
+    void ensure_pinned(size_t par_bytes, size_t ser_bytes);
+    void collect_derive();
+    paddi_stats *derive_st_ = nullptr;   // derive events not yet read
+    struct GraphEnt {
+        std::vector<uintptr_t> sig;   // everything baked into the graph
+        hipGraphExec_t exec = nullptr;
+    };
+    std::map<std::pair<int, size_t>, GraphEnt> graphs_;
+    void drop_graphs();
+
     Model model_;
     std::string model_path_;
     DevBuf feat_, mlpbuf_;
@@ -131,7 +148,13 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[8] = {};
     void *pin_ = nullptr;
-    DevBuf colv_, st64_, alive_, best_, wit_[2], blist_, cnt_, sidx_, sd2_;
+    // pinned staging: pass parameters (PassDev + series table), pass results (PassCtl +
+    // the first SCAP survivors), series table of derive/exact; ser_ev_ marks the end of
+    // the last copy out of pin_ser_
+    void *pin_par_ = nullptr, *pin_out_ = nullptr, *pin_ser_ = nullptr;
+    size_t pin_par_bytes_ = 0, pin_ser_bytes_ = 0;
+    hipEvent_t ser_ev_ = nullptr;
+    DevBuf colv_, st64_, alive_, best_, wit_[2], blist_, sidx_, sd2_, par_, ctl_;
     DevBuf cand_, okey_, gfirst_, gser_, goff_;
     int witcur_ = 0;
 };

@@ -33,13 +33,58 @@ struct ScanArgs {
     int *best;                // global rows: ord_key of the max correlation
     int *wit;                 // global rows: pruning tile recorded at this length (or null)
     const int *wit_read;      // global rows: tiles recorded at the previous length (or null)
-    unsigned long long *pairs;
+    unsigned long long *pairs;  // [0] pairs of k_scan_select, [1] of k_scan_refine
     int64_t WA;               // columns right of a block scanned by k_scan_select
-    int64_t CH;               // columns per k_scan_refine work item
+    int64_t CH;               // columns per k_scan_refine work item (set per round on device)
     int64_t U0;               // first cyclic offset scanned: round_up(m + 64, 64)
     int m;
     int witness_max;          // witness ranges scanned first
 };
+
+// Refine rounds of one pass: round r scans cyclic offsets [bound[r], bound[r+1]) of the
+// blocks still alive when it starts, in work items of CH columns, where CH puts about
+// `items` work items in the round and is clamped to [min_chunk, max_chunk] (multiple of 64).
+struct RefineArgs {
+    int64_t bound[4];
+    int64_t min_chunk, max_chunk, items;
+};
+
+// Buffers of a pass, passed by value as a kernel argument: they stay fixed while a batch
+// runs, so a captured graph can hold them, and the compiler knows they are global memory.
+struct ScanPtrs {
+    const float *x;
+    const SerDev *ser;        // the pass's series table (N = 0 switches a series off)
+    const int *bser;
+    const float4 *colv;
+    uint64_t *alive;
+    int *best;
+    int *wit0, *wit1;         // the two witness buffers (one written, one read per length)
+    unsigned long long *pairs;
+};
+
+// Scalars of one pass.
+struct ScanScal {
+    int64_t WA, U0;
+    int m;
+    int witness_max;
+    int wsel, rsel;           // witness buffer written / read (0, 1; -1 = none)
+};
+
+// Everything one pass reads that changes between passes: one host-to-device copy (or one
+// memcpy node of a captured graph) brings it in.  The series table follows it.
+struct PassDev {
+    ScanScal S;
+    RefineArgs R;
+};
+
+// Device counters of one pass, cleared by k_pass_init.
+struct PassCtl {
+    int list[4];                  // live blocks listed before refine round r
+    int survivors;
+    int pad[3];
+    unsigned long long pairs[2];
+};
+constexpr int REFINE_ROUNDS = 3;
 
 struct ExactArgs {
     const float *x;
@@ -61,18 +106,25 @@ __global__ void k_derive(const float *__restrict__ x, const SerDev *__restrict__
                          const int *__restrict__ bser, const double2 *__restrict__ P,
                          const double *__restrict__ refs, int64_t rows, int m,
                          float4 *__restrict__ colv, double2 *__restrict__ st64);
-// k_scan_select / k_scan_refine are templates on the prefetch depth; these pick it from m.
-void launch_scan_select(const ScanArgs &A, unsigned grid, size_t lds, hipStream_t st);
-void launch_scan_refine(const ScanArgs &A, const int *blist, int nchunks, int64_t u_begin,
-                        int64_t u_end, unsigned grid, size_t lds, hipStream_t st);
-__global__ void k_fill_alive(uint64_t *__restrict__ alive, const SerDev *__restrict__ ser,
-                             const int *__restrict__ bser, int64_t nwords);
+// Prefetch depth of the scan kernels at length m (their template parameter).
+inline int scan_px(int m)
+{
+    const int px = (64 + m + 63) / 64;
+    return px <= 3 ? 3 : px <= 5 ? 5 : px <= 9 ? 9 : 33;
+}
+// k_scan_select / k_scan_refine are templates on the prefetch depth px = scan_px(m).
+void launch_scan_select(int px, const ScanPtrs &Q, const PassDev *P, unsigned grid, size_t lds,
+                        hipStream_t st);
+void launch_scan_refine(int px, const ScanPtrs &Q, const PassDev *P, const int *blist,
+                        PassCtl *ctl, int round, unsigned grid, size_t lds, hipStream_t st);
+__global__ void k_pass_init(ScanPtrs Q, const PassDev *__restrict__ P, int64_t rows,
+                            int64_t nwords, PassCtl *__restrict__ ctl);
 __global__ void k_fill_int(int *__restrict__ p, int64_t n, int v);
 __global__ void k_list_blocks(const uint64_t *__restrict__ alive, int64_t nwords,
                               int *__restrict__ list, int *__restrict__ count);
-__global__ void k_survivors(const uint64_t *__restrict__ alive, const int *__restrict__ best,
-                            int64_t rows, int m, int *__restrict__ count,
-                            int64_t *__restrict__ idx, float *__restrict__ d2, int64_t cap);
+__global__ void k_survivors(ScanPtrs Q, const PassDev *__restrict__ P, int64_t rows,
+                            PassCtl *__restrict__ ctl, int64_t *__restrict__ idx,
+                            float *__restrict__ d2, int64_t cap);
 __global__ void k_exact(ExactArgs E);
 
 // LDS bytes of the scan kernels for length m.

@@ -507,9 +507,33 @@ __device__ __forceinline__ int make_view(const ScanArgs &A, int64_t gb, View &V,
     return sid;
 }
 
-template <int PX>
-__global__ __launch_bounds__(64) void k_scan_select(ScanArgs A)
+// The scan arguments of a pass: buffers from the kernel argument, scalars from the pass
+// parameters in device memory (uniform scalar loads).
+__device__ __forceinline__ ScanArgs load_args(const ScanPtrs &Q, const PassDev *__restrict__ P)
 {
+    const ScanScal S = P->S;
+    ScanArgs A;
+    A.x = Q.x;
+    A.ser = Q.ser;
+    A.bser = Q.bser;
+    A.colv = Q.colv;
+    A.alive = Q.alive;
+    A.best = Q.best;
+    A.wit = S.wsel == 0 ? Q.wit0 : (S.wsel == 1 ? Q.wit1 : nullptr);
+    A.wit_read = S.rsel == 0 ? Q.wit0 : (S.rsel == 1 ? Q.wit1 : nullptr);
+    A.pairs = Q.pairs;
+    A.WA = S.WA;
+    A.CH = 0;
+    A.U0 = S.U0;
+    A.m = S.m;
+    A.witness_max = S.witness_max;
+    return A;
+}
+
+template <int PX>
+__global__ __launch_bounds__(64) void k_scan_select(ScanPtrs Q, const PassDev *__restrict__ P)
+{
+    const ScanArgs A = load_args(Q, P);
     const int64_t gb = blockIdx.x;
     View V;
     int64_t s;
@@ -523,8 +547,8 @@ __global__ __launch_bounds__(64) void k_scan_select(ScanArgs A)
     bool live = true;
     if (A.wit_read) {
         // Witness tiles: the tiles that pruned these rows at the previous length.  Scan up
-        // to four distinct ones first; a run of adjacent witness tiles is one range (one
-        // chain initialisation).
+        // to witness_max distinct ones first; a run of adjacent witness tiles is one range
+        // (one chain initialisation).
         int w = A.wit_read[gb * WAVE + threadIdx.x];
         const int64_t lim = V.N;
         for (int rep = 0; rep < A.witness_max && live; rep++) {
@@ -548,24 +572,15 @@ __global__ __launch_bounds__(64) void k_scan_select(ScanArgs A)
     count_pairs(R, A.pairs);
 }
 
-// Work item = (live global block, chunk q of the offset range [u_begin, u_end)): the columns
-// at cyclic offsets [u_begin + q*CH, u_begin + (q+1)*CH) from the block (see scan_cyclic).
-// Rounds of growing u_begin let nearby columns kill most blocks before the far ones are
-// launched.
+// One refine work item: global block gb over cyclic offsets [u0, u1) (see scan_cyclic).
 template <int PX>
-__global__ __launch_bounds__(64) void k_scan_refine(ScanArgs A, const int *__restrict__ blist,
-                                                    int nchunks, int64_t u_begin, int64_t u_end)
+__device__ __forceinline__ void refine_item(const ScanArgs &A, int64_t gb, int64_t u0,
+                                            int64_t u1)
 {
-    const int item = blockIdx.x;
-    const int64_t gb = blist[item / nchunks];
-    const int64_t q = item % nchunks;
     View V;
     int64_t s;
     float thr_rho;
     if (make_view(A, gb, V, s, thr_rho) < 0) return;
-    const int64_t u0 = u_begin + q * A.CH;
-    int64_t u1 = u0 + A.CH;
-    u1 = u1 < u_end ? u1 : u_end;
     if (u0 >= u1 || u0 >= V.N + A.U0) return;
     RowState R;
     R.alive = __hip_atomic_load(V.alive + (s >> 6), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -574,61 +589,105 @@ __global__ __launch_bounds__(64) void k_scan_refine(ScanArgs A, const int *__res
     load_block(L, R, V, s, thr_rho);
     const bool live = scan_cyclic<PX>(L, R, V, s, u0, u1, A.U0);
     if (live) flush_best(R, V, s);
-    count_pairs(R, A.pairs);
+    count_pairs(R, A.pairs + 1);
 }
 
-// Launchers: the prefetch depth PX = ceil((64 + m) / 64) is a template parameter.
-void launch_scan_select(const ScanArgs &A, unsigned grid, size_t lds, hipStream_t st)
+// Refine round: work item = (listed block, chunk q of the round's offset range).  The round's
+// block count comes from k_list_blocks, so the host never reads it: rounds of growing offset
+// let nearby columns kill most blocks before the far ones are scanned, and the chunk length
+// keeps about R.items work items per round whatever that count is.  The grid is fixed, so a
+// captured graph can replay it: at least 1.5 R.items workgroups and more than the number of
+// blocks; workgroup b takes item b, and the workgroups beyond the round's items exit at once.
+template <int PX>
+__global__ __launch_bounds__(64) void k_scan_refine(ScanPtrs Q, const PassDev *__restrict__ P,
+                                                    const int *__restrict__ blist,
+                                                    PassCtl *__restrict__ ctl, int round)
 {
-    const int px = (WAVE + A.m + WAVE - 1) / WAVE;
-    if (px <= 3)
-        hipLaunchKernelGGL((k_scan_select<3>), dim3(grid), dim3(64), lds, st, A);
-    else if (px <= 5)
-        hipLaunchKernelGGL((k_scan_select<5>), dim3(grid), dim3(64), lds, st, A);
-    else if (px <= 9)
-        hipLaunchKernelGGL((k_scan_select<9>), dim3(grid), dim3(64), lds, st, A);
-    else
-        hipLaunchKernelGGL((k_scan_select<33>), dim3(grid), dim3(64), lds, st, A);
+    const RefineArgs &RA = P->R;
+    const int64_t ub = RA.bound[round], ue = RA.bound[round + 1];
+    const int64_t nblk = ctl->list[round];
+    if (nblk == 0 || ub >= ue) return;
+    const int64_t len = ue - ub;
+    int64_t ch = ((len * nblk + RA.items - 1) / RA.items + 63) & ~(int64_t)63;
+    ch = ch < RA.min_chunk ? RA.min_chunk : (ch > RA.max_chunk ? RA.max_chunk : ch);
+    // Chunks long enough that every item has a workgroup: with G = gridDim.x > nblk,
+    // ch >= len * nblk / (G - nblk) gives nblk * ceil(len / ch) <= G.
+    const int64_t G = gridDim.x;
+    const int64_t chfit = ((len * nblk + (G - nblk) - 1) / (G - nblk) + 63) & ~(int64_t)63;
+    ch = ch > chfit ? ch : chfit;
+    const int64_t nchunks = (len + ch - 1) / ch;
+    const int64_t item = blockIdx.x;
+    if (item >= nblk * nchunks) return;
+    ScanArgs A = load_args(Q, P);
+    A.CH = ch;
+    const int64_t gb = blist[item / nchunks];
+    const int64_t u0 = ub + (item % nchunks) * ch;
+    const int64_t u1 = (u0 + ch < ue) ? u0 + ch : ue;
+    refine_item<PX>(A, gb, u0, u1);
 }
 
-void launch_scan_refine(const ScanArgs &A, const int *blist, int nchunks, int64_t u_begin,
-                        int64_t u_end, unsigned grid, size_t lds, hipStream_t st)
+// Launchers: the prefetch depth PX is a template parameter (scan_px).
+void launch_scan_select(int px, const ScanPtrs &Q, const PassDev *P, unsigned grid, size_t lds,
+                        hipStream_t st)
 {
-    const int px = (WAVE + A.m + WAVE - 1) / WAVE;
-    if (px <= 3)
-        hipLaunchKernelGGL((k_scan_refine<3>), dim3(grid), dim3(64), lds, st, A, blist, nchunks,
-                           u_begin, u_end);
-    else if (px <= 5)
-        hipLaunchKernelGGL((k_scan_refine<5>), dim3(grid), dim3(64), lds, st, A, blist, nchunks,
-                           u_begin, u_end);
-    else if (px <= 9)
-        hipLaunchKernelGGL((k_scan_refine<9>), dim3(grid), dim3(64), lds, st, A, blist, nchunks,
-                           u_begin, u_end);
+    if (px == 3)
+        hipLaunchKernelGGL((k_scan_select<3>), dim3(grid), dim3(64), lds, st, Q, P);
+    else if (px == 5)
+        hipLaunchKernelGGL((k_scan_select<5>), dim3(grid), dim3(64), lds, st, Q, P);
+    else if (px == 9)
+        hipLaunchKernelGGL((k_scan_select<9>), dim3(grid), dim3(64), lds, st, Q, P);
     else
-        hipLaunchKernelGGL((k_scan_refine<33>), dim3(grid), dim3(64), lds, st, A, blist, nchunks,
-                           u_begin, u_end);
+        hipLaunchKernelGGL((k_scan_select<33>), dim3(grid), dim3(64), lds, st, Q, P);
+}
+
+void launch_scan_refine(int px, const ScanPtrs &Q, const PassDev *P, const int *blist,
+                        PassCtl *ctl, int round, unsigned grid, size_t lds, hipStream_t st)
+{
+    if (px == 3)
+        hipLaunchKernelGGL((k_scan_refine<3>), dim3(grid), dim3(64), lds, st, Q, P, blist, ctl,
+                           round);
+    else if (px == 5)
+        hipLaunchKernelGGL((k_scan_refine<5>), dim3(grid), dim3(64), lds, st, Q, P, blist, ctl,
+                           round);
+    else if (px == 9)
+        hipLaunchKernelGGL((k_scan_refine<9>), dim3(grid), dim3(64), lds, st, Q, P, blist, ctl,
+                           round);
+    else
+        hipLaunchKernelGGL((k_scan_refine<33>), dim3(grid), dim3(64), lds, st, Q, P, blist, ctl,
+                           round);
 }
 
 // ---------------------------------------------------------------------------------------
 
-// Alive bits of every global row: 1 for real rows (< N of their series), 0 for padding.
-__global__ __launch_bounds__(256) void k_fill_alive(uint64_t *__restrict__ alive,
-                                                    const SerDev *__restrict__ ser,
-                                                    const int *__restrict__ bser, int64_t nwords)
+// Start of a pass: alive bits of every global row (1 for real rows, < N of their series;
+// 0 for padding and for series switched off), row maxima to -inf, this length's witness
+// tiles to none, counters to zero.  Fill kernel rather than memset nodes, so that a
+// captured graph replays it faithfully.
+__global__ __launch_bounds__(256) void k_pass_init(ScanPtrs Q, const PassDev *__restrict__ P,
+                                                   int64_t rows, int64_t nwords,
+                                                   PassCtl *__restrict__ ctl)
 {
-    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= nwords) return;
-    const int sid = bser[w];
-    uint64_t v = 0;
-    if (sid >= 0) {
-        const SerDev S = ser[sid];
-        const int64_t r0 = w * 64 - S.roff;
-        if (r0 + 64 <= S.N)
-            v = ~0ull;
-        else if (r0 < S.N)
-            v = (~0ull) >> (64 - (S.N - r0));
+    const ScanArgs A = load_args(Q, P);
+    uint64_t *alive = A.alive;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < rows) {
+        A.best[g] = (int)0x807fffff;  // ord_key(-inf)
+        if (A.wit) A.wit[g] = -1;
     }
-    alive[w] = v;
+    if (g < nwords) {
+        const int sid = A.bser[g];
+        uint64_t v = 0;
+        if (sid >= 0) {
+            const SerDev S = A.ser[sid];
+            const int64_t r0 = g * 64 - S.roff;
+            if (r0 + 64 <= S.N)
+                v = ~0ull;
+            else if (r0 < S.N)
+                v = (~0ull) >> (64 - (S.N - r0));
+        }
+        alive[g] = v;
+    }
+    if (g < (int64_t)(sizeof(PassCtl) / sizeof(int))) reinterpret_cast<int *>(ctl)[g] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_fill_int(int *__restrict__ p, int64_t n, int v)
@@ -650,22 +709,22 @@ __global__ __launch_bounds__(256) void k_list_blocks(const uint64_t *__restrict_
 }
 
 // Surviving rows -> (global row, FP32 squared distance); FLT_MAX when no valid pair was seen.
-__global__ __launch_bounds__(256) void k_survivors(const uint64_t *__restrict__ alive,
-                                                   const int *__restrict__ best, int64_t rows,
-                                                   int m, int *__restrict__ count,
+__global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__restrict__ P,
+                                                   int64_t rows, PassCtl *__restrict__ ctl,
                                                    int64_t *__restrict__ idx,
                                                    float *__restrict__ d2, int64_t cap)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= rows) return;
-    if (!((alive[g >> 6] >> (g & 63)) & 1ull)) return;
-    const int k = best[g];
+    const ScanArgs A = load_args(Q, P);
+    if (!((A.alive[g >> 6] >> (g & 63)) & 1ull)) return;
+    const int k = A.best[g];
     float d;
     if (k == ord_key(-INFINITY))
         d = FLT_MAX;
     else
-        d = 2.0f * (float)m * (1.0f - ord_val(k));
-    const int p = atomicAdd(count, 1);
+        d = 2.0f * (float)A.m * (1.0f - ord_val(k));
+    const int p = atomicAdd(&ctl->survivors, 1);
     if (p < cap) {
         idx[p] = g;
         d2[p] = d;

@@ -9,6 +9,7 @@ import os
 
 import pytest
 
+from oracle import oracle as O
 from paddi_amd import synth
 from paddi_amd.engine import PaddiError
 
@@ -22,6 +23,8 @@ SCHEDULES = [
     dict(select_window=32768, refine_chunk=16384, min_chunk=512, refine_items=4096),
     dict(refine_chunk=131072, min_chunk=8192, refine_items=65536, speculate=1),
     dict(speculate=16, witness_tiles=1),
+    dict(use_graph=1),
+    dict(use_graph=1, witness_tiles=0, refine_items=4096),
 ]
 
 
@@ -57,3 +60,25 @@ def test_schedule_does_not_change_discords(ctx, restore, sched, mode):
     assert [(g.index, g.length) for g in got] == [(i, m) for i, m, _ in exp]
     for g, (_, _, d) in zip(got, exp):
         assert abs(g.nn_dist2 - d) <= RTOL * d
+
+
+def test_graph_replays_match_eager_and_oracle(ctx, restore):
+    """Captured pass graphs, replayed over lengths of two prefetch classes (m <= 128 and
+    m > 128) and across calls, give the eager launches' discords, which are the oracle's."""
+    series = [synth.rw_planted(20000, 31, 2, 48, 12.0), synth.ecg_like(30000, 32, 1, 1)]
+    minL, maxL, step, k = 120, 136, 4, 2
+    ctx.set_tunables(use_graph=0)
+    eager = ctx.discords_batch(series, minL, maxL, step, k, mode="batched")
+    ctx.set_tunables(use_graph=1)
+    runs = [ctx.discords_batch(series, minL, maxL, step, k, mode=mode)
+            for mode in ("batched", "batched", "learned", "plain")]
+    key = [(g.series, g.index, g.length, g.nn_dist2) for g in eager]
+    for r in runs:
+        assert [(g.series, g.index, g.length, g.nn_dist2) for g in r] == key
+    for q, x in enumerate(series):
+        ref = O.palmad(x, minL, maxL, step, k)
+        exp = [(int(i), r["m"], float(d)) for r in ref for i, d in zip(r["idx"], r["d2"])]
+        mine = [g for g in eager if g.series == q]
+        assert [(g.index, g.length) for g in mine] == [(i, m) for i, m, _ in exp]
+        for g, (_, _, d) in zip(mine, exp):
+            assert abs(g.nn_dist2 - d) <= RTOL * d
