@@ -558,25 +558,55 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
     for (size_t q = 0; q < req.size(); q++) order[q] = q;
     std::stable_sort(order.begin(), order.end(),
                      [&](size_t a, size_t b) { return req[a].first < req[b].first; });
-    std::vector<int64_t> cand;
-    std::vector<int> gfirst, gser;
-    std::vector<int64_t> goff;
-    int64_t blocks = 0;
+    // Groups: a series' candidates split evenly into groups of at most umax; groups of equal
+    // size form one launch (k_exact's work per column is proportional to its group size).
+    struct Grp {
+        int sid;
+        size_t first, n;  // into `order`
+    };
+    std::vector<std::vector<Grp>> bucket((size_t)MAXU + 1);
     for (size_t q = 0; q < order.size();) {
         const int sid = req[order[q]].first;
-        gfirst.push_back((int)cand.size());
-        gser.push_back(sid);
-        goff.push_back(blocks);
-        blocks += (B.n[(size_t)sid] - m + 1 + EXACT_COLS - 1) / EXACT_COLS;
-        int u = 0;
-        while (q < order.size() && req[order[q]].first == sid && u < umax) {
-            cand.push_back(req[order[q]].second);
-            q++;
-            u++;
+        size_t e = q;
+        while (e < order.size() && req[order[e]].first == sid) e++;
+        const size_t cnt = e - q, ng = (cnt + (size_t)umax - 1) / (size_t)umax;
+        const size_t base = cnt / ng, extra = cnt % ng;
+        for (size_t g = 0, f = q; g < ng; g++) {
+            const size_t n = base + (g < extra ? 1 : 0);
+            bucket[n].push_back(Grp{sid, f, n});
+            f += n;
         }
+        q = e;
     }
-    gfirst.push_back((int)cand.size());
-    const int ng = (int)gser.size();
+    // per launch: gfirst/gser/goff hold ngroups + 1 entries (the last closes gfirst)
+    std::vector<int64_t> cand;
+    std::vector<size_t> slot;  // cand index -> index into req
+    std::vector<int> gfirst, gser;
+    std::vector<int64_t> goff;
+    struct Launch {
+        int nu, ngroups;
+        size_t off;
+        int64_t blocks;
+    };
+    std::vector<Launch> launches;
+    for (int nu = 1; nu <= MAXU; nu++) {
+        if (bucket[(size_t)nu].empty()) continue;
+        Launch L{nu, (int)bucket[(size_t)nu].size(), gfirst.size(), 0};
+        for (const Grp &g : bucket[(size_t)nu]) {
+            gfirst.push_back((int)cand.size());
+            gser.push_back(g.sid);
+            goff.push_back(L.blocks);
+            L.blocks += (B.n[(size_t)g.sid] - m + 1 + EXACT_COLS - 1) / EXACT_COLS;
+            for (size_t u = 0; u < g.n; u++) {
+                cand.push_back(req[order[g.first + u]].second);
+                slot.push_back(order[g.first + u]);
+            }
+        }
+        gfirst.push_back((int)cand.size());
+        gser.push_back(0);
+        goff.push_back(L.blocks);
+        launches.push_back(L);
+    }
     cand_.reserve(cand.size() * 8);
     okey_.reserve(cand.size() * 8);
     gfirst_.reserve(gfirst.size() * 4);
@@ -588,38 +618,39 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
     HIPC(hipMemcpyAsync(gser_.p, gser.data(), gser.size() * 4, hipMemcpyHostToDevice, stream_));
     HIPC(hipMemcpyAsync(goff_.p, goff.data(), goff.size() * 8, hipMemcpyHostToDevice, stream_));
     HIPC(hipMemsetAsync(okey_.p, 0xff, cand.size() * 8, stream_));
-    ExactArgs E{};
-    E.x = B.x.as<float>();
-    E.ser = B.ser.as<SerDev>();
-    E.st64 = st64_.as<double2>();
-    E.cand = cand_.as<int64_t>();
-    E.gfirst = gfirst_.as<int>();
-    E.gser = gser_.as<int>();
-    E.goff = goff_.as<int64_t>();
-    E.outkey = okey_.as<unsigned long long>();
-    E.ngroups = ng;
-    E.m = m;
     HIPC(hipEventRecord(ev_[6], stream_));
-    hipLaunchKernelGGL(k_exact, dim3((unsigned)blocks), dim3(256), exact_lds_bytes(m, umax),
-                       stream_, E);
-    HIPC(hipGetLastError());
+    for (const Launch &L : launches) {
+        ExactArgs E{};
+        E.x = B.x.as<float>();
+        E.ser = B.ser.as<SerDev>();
+        E.st64 = st64_.as<double2>();
+        E.cand = cand_.as<int64_t>();
+        E.gfirst = gfirst_.as<int>() + L.off;
+        E.gser = gser_.as<int>() + L.off;
+        E.goff = goff_.as<int64_t>() + L.off;
+        E.outkey = okey_.as<unsigned long long>();
+        E.ngroups = L.ngroups;
+        E.m = m;
+        launch_exact(E, L.nu, (unsigned)L.blocks, exact_lds_bytes(m, L.nu), stream_);
+        HIPC(hipGetLastError());
+    }
     HIPC(hipEventRecord(ev_[7], stream_));
     std::vector<unsigned long long> keys(cand.size());
     HIPC(hipMemcpyAsync(keys.data(), okey_.p, cand.size() * 8, hipMemcpyDeviceToHost, stream_));
     HIPC(hipStreamSynchronize(stream_));
     collect_derive();
-    for (size_t q = 0; q < order.size(); q++) {
+    for (size_t q = 0; q < slot.size(); q++) {
         const unsigned long long k = keys[q];
         const unsigned long long bits = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
         double d;
         std::memcpy(&d, &bits, 8);
-        out[order[q]] = d;
+        out[slot[q]] = d;
     }
     if (st) {
         float a = 0.f;
         HIPC(hipEventElapsedTime(&a, ev_[6], ev_[7]));
         st->kernel_ms[3] += a;
-        st->launches[3] += 1;
+        st->launches[3] += (int64_t)launches.size();
         st->exact_refines += (int64_t)cand.size();
     }
 }

@@ -11,7 +11,7 @@ constexpr int MAXU = 8;        // candidates per exact-refine group
 constexpr int EXACT_SUB = 1024;  // columns per k_exact sub-tile (4 per thread)
 constexpr int EXACT_COLS = 4096; // columns per k_exact block (4 sub-tiles)
 constexpr int64_t XPAD = 2048; // zero padding after each series (floats); k_exact reads up
-                               // to n + EXACT_SUB - 1, the scans up to n + 63
+                               // to n + EXACT_SUB + 3, the scans up to n + 63
 
 // One series of a batch, as the kernels see it (rebuilt per length: N and thr change).
 struct SerDev {
@@ -125,22 +125,29 @@ __global__ void k_list_blocks(const uint64_t *__restrict__ alive, int64_t nwords
 __global__ void k_survivors(ScanPtrs Q, const PassDev *__restrict__ P, int64_t rows,
                             PassCtl *__restrict__ ctl, int64_t *__restrict__ idx,
                             float *__restrict__ d2, int64_t cap);
-__global__ void k_exact(ExactArgs E);
+// k_exact is a template on the largest group size nu of the launch (1..MAXU).
+void launch_exact(const ExactArgs &E, int nu, unsigned grid, size_t lds, hipStream_t st);
 
 // LDS bytes of the scan kernels for length m.
 inline size_t scan_lds_bytes(int m)
 {
     return 64 * 16 + 2 * (size_t)((m + 3) & ~3) * 4 + (size_t)(64 + m + 8) * 4;
 }
-// LDS bytes of k_exact for groups of up to nU candidates of length m.
+// Doubles of k_exact's staged series window: EXACT_SUB + m4 + 8 values (m4 = m rounded up
+// to 4), 4 values per 6 doubles.
+__host__ __device__ inline int exact_xs_doubles(int m)
+{
+    return 6 * ((EXACT_SUB + ((m + 3) & ~3) + 8) / 4);
+}
+// LDS bytes of k_exact for groups of nU candidates of length m.
 inline size_t exact_lds_bytes(int m, int nU)
 {
-    return ((size_t)nU * m + EXACT_SUB + m + 8) * 8;
+    return ((size_t)exact_xs_doubles(m) + (size_t)nU * ((m + 3) & ~3)) * 8;
 }
 // Candidates per exact group that fit 64 KiB of LDS at length m (>= 1 for m <= MAXM).
 inline int exact_max_u(int m)
 {
-    int u = (int)((65536 / 8 - EXACT_SUB - 8 - m) / m);
+    int u = (int)((65536 / 8 - exact_xs_doubles(m)) / ((m + 3) & ~3));
     return u < 1 ? 1 : (u > MAXU ? MAXU : u);
 }
 

@@ -739,12 +739,26 @@ __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__
 // symmetric in (i, j): mutual nearest neighbours tie exactly and the tie goes to the lower
 // index.  outkey[u] receives the atomic minimum of ord_key64(ED^2) of candidate u.
 
+// NU = the group size of the launch (a template parameter, so the inner loop is straight
+// code); a smaller group fills its spare rows with copies of its first candidate and ignores
+// their results.
+//
+// Thread t owns the 4 adjacent columns c + 4t + q of a sub-tile and slides a window of 8
+// doubles over x: per 4 steps of k it reads 4 new doubles (two 16-byte LDS reads, laid out
+// 4 doubles per 48 bytes so that the 64 lanes hit distinct banks) and the candidates' next 4
+// values (broadcast), for 4 (1 + NU) FP64 VALU.  Per column the sum still runs in ascending
+// k with one FMA per term; candidate rows are zero-padded to a multiple of 4, which adds
+// exact zeros.
+__device__ __forceinline__ int xs_at(int p) { return 6 * (p >> 2) + (p & 3); }
+
+template <int NU>
 __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
 {
     extern __shared__ double dsh[];
     const int m = E.m;
-    double *xs = dsh;                         // [EXACT_SUB + m]  raw x as double
-    double *A = dsh + (EXACT_SUB + m + 8);    // [nU][m]          centred candidate windows
+    const int m4 = (m + 3) & ~3;
+    double *xs = dsh;                               // padded layout, exact_xs_doubles(m)
+    double *A = dsh + exact_xs_doubles(m);          // [NU][m4] centred candidate windows
     __shared__ double ainv[MAXU];
     __shared__ unsigned long long wmin[MAXU][4];
     const int t = threadIdx.x;
@@ -762,58 +776,73 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     const double2 *st64 = E.st64 + S.roff;
     const int64_t N = S.N;
     const int64_t cblk = (item - E.goff[grp]) * EXACT_COLS;
-    for (int u = 0; u < nU; u++) {
-        const int64_t i = E.cand[u0 + u];
+    for (int u = 0; u < NU; u++) {
+        const int64_t i = E.cand[u0 + (u < nU ? u : 0)];
         const double2 st = st64[i];
         if (t == 0) ainv[u] = st.y;
-        for (int k = t; k < m; k += 256) A[u * m + k] = (double)x[i + k] - st.x;
+        for (int k = t; k < m4; k += 256) A[u * m4 + k] = k < m ? (double)x[i + k] - st.x : 0.0;
     }
-    unsigned long long kmin[MAXU];
+    unsigned long long kmin[NU];
 #pragma unroll
-    for (int u = 0; u < MAXU; u++) kmin[u] = ~0ull;
-    // Sub-tiles of EXACT_SUB columns; thread t owns columns c + t + 256 q, q < 4 (conflict-free
-    // LDS reads), and accumulates all nU candidates for each: 4 + 4 nU VALU per k against
-    // 4 + nU LDS reads.
+    for (int u = 0; u < NU; u++) kmin[u] = ~0ull;
     for (int sub = 0; sub < EXACT_COLS / EXACT_SUB; sub++) {
         const int64_t c = cblk + sub * EXACT_SUB;
         if (c >= N) break;
         __syncthreads();  // previous sub-tile's readers are done with xs
-        for (int k = t; k < EXACT_SUB + m; k += 256) xs[k] = (double)x[c + k];
+        for (int k = t; k < EXACT_SUB + m4 + 4; k += 256) xs[xs_at(k)] = (double)x[c + k];
         __syncthreads();
         double muj[4], invj[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int64_t j = c + t + 256 * q;
+            const int64_t j = c + 4 * t + q;
             const double2 sj = (j < N) ? st64[j] : make_double2(0.0, 0.0);
             muj[q] = sj.x;
             invj[q] = sj.y;
         }
-        double acc[MAXU][4];
+        double acc[NU][4];
 #pragma unroll
-        for (int u = 0; u < MAXU; u++)
+        for (int u = 0; u < NU; u++)
 #pragma unroll
             for (int q = 0; q < 4; q++) acc[u][q] = 0.0;
-        const double *xp = xs + t;
-        for (int k = 0; k < m; k++) {
-            double b[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) b[q] = xp[k + 256 * q] - muj[q];
-#pragma unroll
-            for (int u = 0; u < MAXU; u++) {
-                if (u < nU) {
-                    const double a = A[u * m + k];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) acc[u][q] = fma(a, b[q], acc[u][q]);
-                }
+        // window w[0..7] = x[c + 4t + k0 .. + 7]
+        double w[8];
+        {
+            const double2 *g = reinterpret_cast<const double2 *>(xs + 6 * t);
+            const double2 lo2 = g[0], hi2 = g[1];
+            w[0] = lo2.x; w[1] = lo2.y; w[2] = hi2.x; w[3] = hi2.y;
+        }
+        for (int k0 = 0; k0 < m4; k0 += 4) {
+            {
+                const double2 *g = reinterpret_cast<const double2 *>(xs + 6 * (t + (k0 >> 2) + 1));
+                const double2 lo2 = g[0], hi2 = g[1];
+                w[4] = lo2.x; w[5] = lo2.y; w[6] = hi2.x; w[7] = hi2.y;
             }
+            double a[NU][4];
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+                const double2 *ap = reinterpret_cast<const double2 *>(A + u * m4 + k0);
+                const double2 p0 = ap[0], p1 = ap[1];
+                a[u][0] = p0.x; a[u][1] = p0.y; a[u][2] = p1.x; a[u][3] = p1.y;
+            }
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                double b[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) b[q] = w[kk + q] - muj[q];
+#pragma unroll
+                for (int u = 0; u < NU; u++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) acc[u][q] = fma(a[u][kk], b[q], acc[u][q]);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) w[q] = w[q + 4];
         }
 #pragma unroll
-        for (int u = 0; u < MAXU; u++) {
-            if (u >= nU) break;
-            const int64_t i = E.cand[u0 + u];
+        for (int u = 0; u < NU; u++) {
+            const int64_t i = E.cand[u0 + (u < nU ? u : 0)];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int64_t j = c + t + 256 * q;
+                const int64_t j = c + 4 * t + q;
                 double ed = DBL_MAX;
                 const bool valid = (j < N) && ((j - i >= m - 1) || (i - j >= m + 1));
                 if (valid) {
@@ -829,8 +858,7 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     }
     const int lane = t & 63, w = t >> 6;
 #pragma unroll
-    for (int u = 0; u < MAXU; u++) {
-        if (u >= nU) break;
+    for (int u = 0; u < NU; u++) {
         unsigned long long key = kmin[u];
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {
@@ -844,6 +872,20 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
         unsigned long long k = wmin[t][0];
         for (int q = 1; q < 4; q++) k = wmin[t][q] < k ? wmin[t][q] : k;
         atomicMin(E.outkey + u0 + t, k);
+    }
+}
+
+void launch_exact(const ExactArgs &E, int nu, unsigned grid, size_t lds, hipStream_t st)
+{
+    switch (nu) {
+    case 1: hipLaunchKernelGGL((k_exact<1>), dim3(grid), dim3(256), lds, st, E); break;
+    case 2: hipLaunchKernelGGL((k_exact<2>), dim3(grid), dim3(256), lds, st, E); break;
+    case 3: hipLaunchKernelGGL((k_exact<3>), dim3(grid), dim3(256), lds, st, E); break;
+    case 4: hipLaunchKernelGGL((k_exact<4>), dim3(grid), dim3(256), lds, st, E); break;
+    case 5: hipLaunchKernelGGL((k_exact<5>), dim3(grid), dim3(256), lds, st, E); break;
+    case 6: hipLaunchKernelGGL((k_exact<6>), dim3(grid), dim3(256), lds, st, E); break;
+    case 7: hipLaunchKernelGGL((k_exact<7>), dim3(grid), dim3(256), lds, st, E); break;
+    default: hipLaunchKernelGGL((k_exact<8>), dim3(grid), dim3(256), lds, st, E); break;
     }
 }
 
