@@ -363,9 +363,9 @@ void Engine::copy_st64(double2 *dst, int64_t N)
 // then REFINE_ROUNDS k_scan_refine rounds each preceded by a listing of the live
 // blocks, k_survivors, and the download of the counters and the first scap survivors.  No
 // step depends on a value read back by the host, so the sequence is captured once per
-// (px, lds) class and replayed.  Eager launches are bracketed by events for the statistics;
+// LDS class and replayed.  Eager launches are bracketed by events for the statistics;
 // a captured sequence has none (event nodes of a graph do not time the kernels between them).
-void Engine::enqueue_pass(const ScanPtrs &Q, int px, size_t lds, size_t parb, int64_t rows,
+void Engine::enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t rows,
                           int64_t nwords, int64_t scap, bool timed)
 {
     const PassDev *P = par_.as<PassDev>();
@@ -377,7 +377,7 @@ void Engine::enqueue_pass(const ScanPtrs &Q, int px, size_t lds, size_t parb, in
     hipLaunchKernelGGL(k_pass_init, dim3(grid1(std::max<int64_t>(rows, 64), 256)), dim3(256), 0,
                        stream_, Q, P, rows, nwords, ctl);
     if (timed) HIPC(hipEventRecord(ev_[0], stream_));
-    launch_scan_select(px, Q, P, (unsigned)nwords, lds, stream_);
+    launch_scan_select(Q, P, (unsigned)nwords, lds, stream_);
     if (timed) HIPC(hipEventRecord(ev_[1], stream_));
     hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
                        (const uint64_t *)alive_.p, nwords, blist_.as<int>(), &ctl->list[0]);
@@ -387,7 +387,7 @@ void Engine::enqueue_pass(const ScanPtrs &Q, int px, size_t lds, size_t parb, in
             hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
                                (const uint64_t *)alive_.p, nwords, blist_.as<int>(),
                                &ctl->list[r]);
-        launch_scan_refine(px, Q, P, blist_.as<int>(), ctl, r, grid, lds, stream_);
+        launch_scan_refine(Q, P, blist_.as<int>(), ctl, r, grid, lds, stream_);
     }
     if (timed) HIPC(hipEventRecord(ev_[3], stream_));
     hipLaunchKernelGGL(k_survivors, dim3(grid1(rows, 256)), dim3(256), 0, stream_, Q, P, rows,
@@ -406,10 +406,9 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
 {
     const double delta = delta_ed2(m);
     const int64_t rows = B.rows, nwords = B.nblocks;
-    const int px = scan_px(m);
-    // a graph is shared by every length of its class, so it carries the class's largest LDS
-    const int mcls = px < 33 ? 64 * px - 64 : (int)round_up(m, 256);
-    const size_t lds = use_graph_ ? scan_lds_bytes(std::min(mcls, MAXM)) : scan_lds_bytes(m);
+    // a graph is shared by the lengths of a 64-wide class, so it carries the class's LDS
+    const size_t lds = use_graph_ ? scan_lds_bytes((int)std::min<int64_t>(round_up(m, 64), MAXM))
+                                  : scan_lds_bytes(m);
     const size_t parb = sizeof(PassDev) + (size_t)B.nser * sizeof(SerDev);
     const int64_t scap = std::min<int64_t>(SCAP, rows);
     alive_.reserve((size_t)nwords * 8);
@@ -470,14 +469,14 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
             (uintptr_t)Q.x,      (uintptr_t)Q.bser,   (uintptr_t)Q.colv,   (uintptr_t)Q.best,
             (uintptr_t)Q.wit0,   (uintptr_t)Q.wit1,   (uintptr_t)rows,     (uintptr_t)nwords,
             (uintptr_t)parb,     (uintptr_t)scap,     (uintptr_t)refine_waves_};
-        GraphEnt &G = graphs_[{px, lds}];
+        GraphEnt &G = graphs_[{0, lds}];
         if (!G.exec || G.sig != sig) {
             if (G.exec) HIPC(hipGraphExecDestroy(G.exec));
             G.exec = nullptr;
             hipGraph_t g = nullptr;
             HIPC(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
             try {
-                enqueue_pass(Q, px, lds, parb, rows, nwords, scap, false);
+                enqueue_pass(Q, lds, parb, rows, nwords, scap, false);
             } catch (...) {
                 (void)hipStreamEndCapture(stream_, &g);
                 if (g) (void)hipGraphDestroy(g);
@@ -491,7 +490,7 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
         }
         HIPC(hipGraphLaunch(G.exec, stream_));
     } else {
-        enqueue_pass(Q, px, lds, parb, rows, nwords, scap, true);
+        enqueue_pass(Q, lds, parb, rows, nwords, scap, true);
     }
     HIPC(hipStreamSynchronize(stream_));
     collect_derive();

@@ -126,8 +126,8 @@ public:
 
 private:
     // One pass's device work (parameter upload .. survivor download) on stream_, eagerly or
-    // as a replay of the captured graph of its (px, lds) class.
-    void enqueue_pass(const ScanPtrs &Q, int px, size_t lds, size_t parb, int64_t rows,
+    // as a replay of the captured graph of its LDS class.
+    void enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t rows,
                       int64_t nwords, int64_t scap, bool timed);
     void ensure_pinned(size_t par_bytes, size_t ser_bytes);
     void collect_derive();

@@ -106,17 +106,11 @@ __global__ void k_derive(const float *__restrict__ x, const SerDev *__restrict__
                          const int *__restrict__ bser, const double2 *__restrict__ P,
                          const double *__restrict__ refs, int64_t rows, int m,
                          float4 *__restrict__ colv, double2 *__restrict__ st64);
-// Prefetch depth of the scan kernels at length m (their template parameter).
-inline int scan_px(int m)
-{
-    const int px = (64 + m + 63) / 64;
-    return px <= 3 ? 3 : px <= 5 ? 5 : px <= 9 ? 9 : 33;
-}
-// k_scan_select / k_scan_refine are templates on the prefetch depth px = scan_px(m).
-void launch_scan_select(int px, const ScanPtrs &Q, const PassDev *P, unsigned grid, size_t lds,
+// The scan kernels: one 64-row block per workgroup (one wave); LDS from scan_lds_bytes(m).
+void launch_scan_select(const ScanPtrs &Q, const PassDev *P, unsigned grid, size_t lds,
                         hipStream_t st);
-void launch_scan_refine(int px, const ScanPtrs &Q, const PassDev *P, const int *blist,
-                        PassCtl *ctl, int round, unsigned grid, size_t lds, hipStream_t st);
+void launch_scan_refine(const ScanPtrs &Q, const PassDev *P, const int *blist, PassCtl *ctl,
+                        int round, unsigned grid, size_t lds, hipStream_t st);
 __global__ void k_pass_init(ScanPtrs Q, const PassDev *__restrict__ P, int64_t rows,
                             int64_t nwords, PassCtl *__restrict__ ctl);
 __global__ void k_fill_int(int *__restrict__ p, int64_t n, int v);
@@ -128,10 +122,19 @@ __global__ void k_survivors(ScanPtrs Q, const PassDev *__restrict__ P, int64_t r
 // k_exact is a template on the largest group size nu of the launch (1..MAXU).
 void launch_exact(const ExactArgs &E, int nu, unsigned grid, size_t lds, hipStream_t st);
 
-// LDS bytes of the scan kernels for length m.
+// Scan seeds: the MFMA reduction length (m + 15 rounded up to 4) and the series ring
+// capacity (a multiple of 16 holding a 256-column super-tile window of 256 + K4 columns).
+__host__ __device__ inline int scan_k4(int m) { return (m + 15 + 3) & ~3; }
+__host__ __device__ inline int scan_ring_w(int m) { return (256 + scan_k4(m) + 15) & ~15; }
+// LDS bytes of the scan kernels for length m: tiles (2 x 64 float4), raw seeds (256),
+// the zero-padded first row (K4 + 20), the ring (both copies, padded; it also serves as the
+// chain-initialisation scratch).
 inline size_t scan_lds_bytes(int m)
 {
-    return 64 * 16 + 2 * (size_t)((m + 3) & ~3) * 4 + (size_t)(64 + m + 8) * 4;
+    const int K4 = scan_k4(m), W = scan_ring_w(m);
+    const int ring = 2 * W + (2 * W >> 4) + 4;
+    const int scratch = ((64 + m + 4 + 3) & ~3) + ((m + 3) & ~3);
+    return (size_t)(2 * 64 * 4 + 256 + ((K4 + 20 + 3) & ~3) + (ring > scratch ? ring : scratch)) * 4;
 }
 // Doubles of k_exact's staged series window: EXACT_SUB + m4 + 8 values (m4 = m rounded up
 // to 4), 4 values per 6 doubles.

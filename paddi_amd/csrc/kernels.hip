@@ -192,11 +192,16 @@ __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
 // ---------------------------------------------------------------------------------------
 // Scan kernels (all indices below are local to one series).
 
+// LDS of a scan workgroup (sizes in floats; scan_ring_w / scan_k4 / scan_lds_bytes).
 struct ScanLds {
-    float *xs;    // [64 + m + 4]  x - ref over the current window
-    float *q;     // [m]           the block's first row, centred (seeds; staged once)
-    float *qi;    // [m]           a column window, centred (chain initialisation)
-    float4 *tile; // [64]          (df, dg, invn, seed) of the tile's columns
+    float4 *tile; // [2][64]   (df, dg, invn, seed) of a tile's columns, double-buffered
+    float *seed;  // [256]     seeds of the super-tile's columns before the mean correction
+    float *qz;    // [K4 + 20] the block's first row, centred, at qz[16 + k]; zeros around it
+    float *ring;  // raw x of the super-tile window: column col at (col mod W) and
+                  // (col mod W) + W, float f of that range at f + f / 16 (bank padding)
+    float *xs;    // [64 + m + 4] chain-initialisation scratch (aliases ring)
+    float *qi;    // [m]          chain-initialisation scratch (aliases ring)
+    int W, K4;
 };
 
 // lane t: sum_k v[k] * (x[base+t+k] - mu_{base+t}), using x - ref staged in LDS.
@@ -247,7 +252,7 @@ struct RowState {
     float mx;      // max over valid columns of cov*invn_j
     uint64_t alive;// wave-uniform alive mask of the 64 rows
     uint32_t tiles;// 64x64 tiles evaluated (pair counter)
-    float qsum;    // sum of the staged, centred first row (L.q)
+    float qsum;    // sum of the staged, centred first row (L.qz)
 };
 
 // One series' view of the batch.
@@ -264,7 +269,7 @@ struct View {
 // climit: first column not to evaluate (min of the scan end and N); rows at or beyond N
 // are padding.
 template <bool MASKED>
-__device__ __forceinline__ void tile_steps(const ScanLds &L, RowState &R, int64_t s, int64_t c,
+__device__ __forceinline__ void tile_steps(const float4 *tile, RowState &R, int64_t s, int64_t c,
                                            int64_t N, int64_t climit, int m, uint64_t &rowhit,
                                            uint64_t &colhit)
 {
@@ -276,7 +281,7 @@ __device__ __forceinline__ void tile_steps(const ScanLds &L, RowState &R, int64_
     const int colmax = (int)((climit - c) < 64 ? (climit - c) : 64);
 #pragma unroll
     for (int j = 0; j < WAVE; j++) {
-        const float4 cd = L.tile[j];
+        const float4 cd = tile[j];
         const int prev = __builtin_amdgcn_update_dpp(__float_as_int(cd.w), __float_as_int(cov),
                                                      DPP_WAVE_SHR1, 0xF, 0xF, false);
         cov = fmaf(rdf, cd.y, __int_as_float(prev));
@@ -304,39 +309,66 @@ __device__ __forceinline__ void tile_steps(const ScanLds &L, RowState &R, int64_
     R.mx = mx;
 }
 
-// lane t: sum_k q[k] * xs[t+k] over the staged window.
-__device__ __forceinline__ float dot_window(const ScanLds &L, int m)
+__device__ __forceinline__ int ring_at(int f) { return f + (f >> 4); }
+
+// Seeds of the 256 columns c .. c+255 of a super-tile on the matrix cores, as one
+// 16 x 16 x K4 product (v_mfma_f32_16x16x4f32): row a of A is the window of column c + 16a,
+// A[a][k'] = x[c+16a+k'] - x[c+16a], and B[k'][j] = q[k'-j] (zero outside 0..m-1), so
+// D[a][j] = sum_k q[k] (x[c+16a+j+k] - x[c+16a]).  Lane l supplies A[l%16][l/16 + 4i] and
+// B[l/16 + 4i][l%16] and receives D[4(l/16)+r][l%16] in register r.  Column n = 16a + j then
+// has seed D[a][j] - (mu_{c+n} - x[c+16a]) * sum(q), applied per tile.  r0 = c mod W.
+__device__ __forceinline__ void super_seeds(const ScanLds &L, int r0)
 {
-    const int t = threadIdx.x;
-    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-    int k = 0;
-    for (; k + 4 <= m; k += 4) {
-        const float4 qv = *reinterpret_cast<const float4 *>(L.q + k);
-        a0 = fmaf(qv.x, L.xs[t + k], a0);
-        a1 = fmaf(qv.y, L.xs[t + k + 1], a1);
-        a2 = fmaf(qv.z, L.xs[t + k + 2], a2);
-        a3 = fmaf(qv.w, L.xs[t + k + 3], a3);
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int l = threadIdx.x, i = l & 15, kk = l >> 4;
+    const float *ap = L.ring + ring_at(r0 + 16 * i) + kk;
+    const float *bp = L.qz + 16 + kk - i;
+    const float refi = L.ring[ring_at(r0 + 16 * i)];
+    f4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    int k0 = 0;
+    for (; k0 + 16 <= L.K4; k0 += 16, ap += 17, bp += 16) {
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[0] - refi, bp[0], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[4] - refi, bp[4], acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[8] - refi, bp[8], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[12] - refi, bp[12], acc1, 0, 0, 0);
     }
-    for (; k < m; k++) a0 = fmaf(L.q[k], L.xs[t + k], a0);
-    return (a0 + a1) + (a2 + a3);
+    for (int e = 0; k0 < L.K4; k0 += 4, e += 4)
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[e] - refi, bp[e], acc0, 0, 0, 0);
+    float *out = L.seed + 64 * kk + i;
+#pragma unroll
+    for (int r = 0; r < 4; r++) out[16 * r] = acc0[r] + acc1[r];
+}
+
+// Ring columns [c, c + cnt) from x (both copies), lanes striding by 64.
+__device__ __forceinline__ void ring_fill(const ScanLds &L, const float *__restrict__ x,
+                                          int64_t c, int cnt)
+{
+    const int r0 = (int)(c % L.W);
+    for (int p = threadIdx.x; p < cnt; p += WAVE) {
+        int pos = r0 + p;
+        pos = pos >= L.W ? pos - L.W : pos;
+        const float v = x[c + p];
+        L.ring[ring_at(pos)] = v;
+        L.ring[ring_at(pos + L.W)] = v;
+    }
 }
 
 // Scan columns [c0, c1) for the block of rows [s, s+64).  c0 % 64 == 0.  Returns false when
 // every row of the block is known dead.  With V.wit set, rows pruned by this block record
 // the pruning tile and columns pruned by it record the block.
 //
-// Software pipeline: the next tile's window (PX floats per lane, 64 + m <= 64 * PX) and
-// column terms are loaded into registers while the current tile's 64 steps run, and the
-// block's alive word read back by this tile's atomic is consumed one tile later, so no
-// global-memory round trip sits on the tile's critical path.
-template <int PX>
+// Columns go by super-tiles of 256 (seeds on the matrix cores, then four tiles of 64 steps on
+// the vector units).  The series window of the next super-tile (its 256 new columns, four
+// floats per lane) and the next tile's column terms are loaded while the current ones are
+// processed, and the block's alive word read back by a tile's atomic is consumed one tile
+// later, so no global-memory round trip sits on the critical path.
 __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t s, int64_t c0,
                            int64_t c1)
 {
     const int t = threadIdx.x;
     const int m = V.m;
     const int64_t N = V.N;
-    const int win = WAVE + m;
+    const int W = L.W, K4 = L.K4;
     // Chain initialisation: lane u holds cov(s+u, c0-1), or cov(s+u+1, 0) when c0 == 0
     // (position 0 has zero recurrence terms, so the first step reproduces it).
     {
@@ -346,68 +378,80 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
         const float qs = stage_centred(L.qi, V.x, cI, m, muc);
         R.cov = corr_lanes(L, L.qi, V.x, rb, m, qs, V.colv[rb + t].w);
     }
-    // Seeds use the centred first row of the block, staged once per kernel in L.q.
     const float qsum = R.qsum;
     const uint64_t word = (uint64_t)(s >> 6);
-    float px[PX];
+    // the first window, then the next super-tile's new columns into registers
+    ring_fill(L, V.x, c0, 256 + K4);
+    float nx[4];
+    if (c0 + 256 < c1) {
 #pragma unroll
-    for (int p = 0; p < PX; p++) {
-        const int k = p * WAVE + t;
-        px[p] = (k < win) ? V.x[c0 + k] : 0.f;
+        for (int p = 0; p < 4; p++) nx[p] = V.x[c0 + 256 + K4 + p * WAVE + t];
     }
     float4 pc = V.colv[c0 + t];
     uint64_t pend_lane0 = ~0ull;  // lane 0: the alive word returned by the previous tile
-    for (int64_t c = c0; c < c1; c += WAVE) {
-        const float ref = __shfl(px[0], 0, 64);  // x[c]
-#pragma unroll
-        for (int p = 0; p < PX; p++) {
-            const int k = p * WAVE + t;
-            if (k < win) L.xs[k] = px[p] - ref;
-        }
-        const float4 cv = pc;
-        const int64_t cn = c + WAVE;
-        if (cn < c1) {
-#pragma unroll
-            for (int p = 0; p < PX; p++) {
-                const int k = p * WAVE + t;
-                if (k < win) px[p] = V.x[cn + k];
+    int par = 0;                  // tile buffer in use
+    for (int64_t c = c0; c < c1; c += 256) {
+        const int r0 = (int)(c % W);
+        __syncthreads();  // ring writes of the previous super-tile are visible
+        super_seeds(L, r0);
+        __syncthreads();
+        for (int T = 0; T < 4; T++) {
+            const int64_t ct = c + 64 * T;
+            if (ct >= c1) break;
+            const float4 cv = pc;
+            if (ct + WAVE < c1) pc = V.colv[ct + WAVE + t];
+            const float refa = L.ring[ring_at(r0 + 64 * T + 16 * (t >> 4))];
+            const float seed = L.seed[64 * T + t] - (cv.w - refa) * qsum;
+            float4 *tile = L.tile + WAVE * par;
+            par ^= 1;
+            tile[t] = make_float4(cv.x, cv.y, cv.z, seed);
+            __syncthreads();
+            R.tiles++;
+            const bool masked = (ct - s - 63 <= m && ct - s + 63 >= -m) || (ct + WAVE > N) ||
+                                (s + WAVE > N) || (ct + WAVE > c1);
+            uint64_t rowhit = 0, colhit = 0;
+            if (masked) {
+                const int64_t lim = c1 < N ? c1 : N;
+                tile_steps<true>(tile, R, s, ct, N, lim, m, rowhit, colhit);
+            } else {
+                tile_steps<false>(tile, R, s, ct, N, N, m, rowhit, colhit);
             }
-            pc = V.colv[cn + t];
+            if (V.wit) {
+                if (((rowhit & R.alive) >> t) & 1ull) V.wit[s + t] = (int)ct;
+                if ((colhit >> t) & 1ull) V.wit[ct + t] = (int)s;
+            }
+            if (t == 0 && colhit)
+                __hip_atomic_fetch_and(V.alive + (ct >> 6), ~colhit, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            // Rows pruned elsewhere, as read back one tile ago, and rows pruned here now.
+            const uint64_t pend = __shfl(pend_lane0, 0, 64);
+            const uint64_t newdead = rowhit & R.alive;
+            R.alive &= pend & ~rowhit;
+            if (t == 0) {
+                pend_lane0 = newdead ? __hip_atomic_fetch_and(V.alive + word, ~newdead,
+                                                              __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT)
+                                     : __hip_atomic_load(V.alive + word, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (R.alive == 0) return false;
         }
-        __syncthreads();
-        const float seed = dot_window(L, m) - (cv.w - ref) * qsum;
-        L.tile[t] = make_float4(cv.x, cv.y, cv.z, seed);
-        __syncthreads();
-        R.tiles++;
-        const bool masked = (c - s - 63 <= m && c - s + 63 >= -m) || (c + WAVE > N) ||
-                            (s + WAVE > N) || (c + WAVE > c1);
-        uint64_t rowhit = 0, colhit = 0;
-        if (masked) {
-            const int64_t lim = c1 < N ? c1 : N;
-            tile_steps<true>(L, R, s, c, N, lim, m, rowhit, colhit);
-        } else {
-            tile_steps<false>(L, R, s, c, N, N, m, rowhit, colhit);
+        // The next window: its new columns [c + 256 + K4, c + 512 + K4) replace columns
+        // before c + 256 (W >= 256 + K4), then the one after is requested.
+        if (c + 256 < c1) {
+            int rn = (int)((c + 256 + K4) % W);
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                int pos = rn + p * WAVE + t;
+                pos = pos >= W ? pos - W : pos;
+                L.ring[ring_at(pos)] = nx[p];
+                L.ring[ring_at(pos + W)] = nx[p];
+            }
+            if (c + 512 < c1) {
+#pragma unroll
+                for (int p = 0; p < 4; p++) nx[p] = V.x[c + 512 + K4 + p * WAVE + t];
+            }
         }
-        __syncthreads();
-        if (V.wit) {
-            if (((rowhit & R.alive) >> t) & 1ull) V.wit[s + t] = (int)c;
-            if ((colhit >> t) & 1ull) V.wit[c + t] = (int)s;
-        }
-        if (t == 0 && colhit)
-            __hip_atomic_fetch_and(V.alive + (c >> 6), ~colhit, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        // Rows pruned elsewhere, as read back one tile ago, and rows pruned here now.
-        const uint64_t pend = __shfl(pend_lane0, 0, 64);
-        const uint64_t newdead = rowhit & R.alive;
-        R.alive &= pend & ~rowhit;
-        if (t == 0) {
-            pend_lane0 = newdead ? __hip_atomic_fetch_and(V.alive + word, ~newdead,
-                                                          __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_AGENT)
-                                 : __hip_atomic_load(V.alive + word, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (R.alive == 0) return false;
     }
     return true;
 }
@@ -418,7 +462,6 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
 // that kill most blocks take the unmasked path; the band comes last, wrapped.  u0 is a
 // multiple of 64: an unwrapped piece starts aligned, a wrapped one at the aligned-down
 // column (re-scanning at most 63 columns is harmless: same values, same decisions).
-template <int PX>
 __device__ bool scan_cyclic(const ScanLds &L, RowState &R, const View &V, int64_t s, int64_t u0,
                             int64_t u1, int64_t U0)
 {
@@ -434,8 +477,8 @@ __device__ bool scan_cyclic(const ScanLds &L, RowState &R, const View &V, int64_
     }
     const int64_t e = c + (u1 - u0);
     if (wrapped) c &= ~(int64_t)63;
-    bool live = scan_range<PX>(L, R, V, s, c, e < N ? e : N);
-    if (live && e > N) live = scan_range<PX>(L, R, V, s, 0, e - N);
+    bool live = scan_range(L, R, V, s, c, e < N ? e : N);
+    if (live && e > N) live = scan_range(L, R, V, s, 0, e - N);
     return live;
 }
 
@@ -454,12 +497,15 @@ __device__ __forceinline__ void load_rows(RowState &R, const View &V, int64_t s,
     R.tiles = 0;
 }
 
-// Row terms plus the block's centred first row in L.q (the seeds' left operand).
+// Row terms plus the block's centred first row in L.qz (the seeds' right operand).
 __device__ __forceinline__ void load_block(const ScanLds &L, RowState &R, const View &V,
                                            int64_t s, float thr_rho)
 {
     load_rows(R, V, s, thr_rho);
-    R.qsum = stage_centred(L.q, V.x, s, V.m, V.colv[s].w);
+    const int t = threadIdx.x;
+    if (t < 16) L.qz[t] = 0.f;
+    for (int k = 16 + V.m + t; k < L.K4 + 20; k += WAVE) L.qz[k] = 0.f;
+    R.qsum = stage_centred(L.qz + 16, V.x, s, V.m, V.colv[s].w);
 }
 
 __device__ __forceinline__ void flush_best(const RowState &R, const View &V, int64_t s)
@@ -475,10 +521,14 @@ __device__ __forceinline__ ScanLds carve_lds(int m)
 {
     extern __shared__ float4 lds4[];
     ScanLds L;
+    L.K4 = scan_k4(m);
+    L.W = scan_ring_w(m);
     L.tile = lds4;
-    L.q = reinterpret_cast<float *>(lds4 + WAVE);
-    L.qi = L.q + ((m + 3) & ~3);
-    L.xs = L.qi + ((m + 3) & ~3);
+    L.seed = reinterpret_cast<float *>(lds4 + 2 * WAVE);
+    L.qz = L.seed + 256;
+    L.ring = L.qz + ((L.K4 + 20 + 3) & ~3);
+    L.xs = L.ring;
+    L.qi = L.xs + ((64 + m + 4 + 3) & ~3);
     return L;
 }
 
@@ -530,7 +580,6 @@ __device__ __forceinline__ ScanArgs load_args(const ScanPtrs &Q, const PassDev *
     return A;
 }
 
-template <int PX>
 __global__ __launch_bounds__(64) void k_scan_select(ScanPtrs Q, const PassDev *__restrict__ P)
 {
     const ScanArgs A = load_args(Q, P);
@@ -564,16 +613,15 @@ __global__ __launch_bounds__(64) void k_scan_select(ScanPtrs Q, const PassDev *_
                 we += WAVE;
             }
             const int64_t e = (we < lim) ? we : lim;
-            live = scan_range<PX>(L, R, V, s, wt, e);
+            live = scan_range(L, R, V, s, wt, e);
         }
     }
-    if (live) live = scan_cyclic<PX>(L, R, V, s, A.U0, A.U0 + A.WA, A.U0);
+    if (live) live = scan_cyclic(L, R, V, s, A.U0, A.U0 + A.WA, A.U0);
     if (live) flush_best(R, V, s);
     count_pairs(R, A.pairs);
 }
 
 // One refine work item: global block gb over cyclic offsets [u0, u1) (see scan_cyclic).
-template <int PX>
 __device__ __forceinline__ void refine_item(const ScanArgs &A, int64_t gb, int64_t u0,
                                             int64_t u1)
 {
@@ -587,7 +635,7 @@ __device__ __forceinline__ void refine_item(const ScanArgs &A, int64_t gb, int64
     if (R.alive == 0) return;
     ScanLds L = carve_lds(V.m);
     load_block(L, R, V, s, thr_rho);
-    const bool live = scan_cyclic<PX>(L, R, V, s, u0, u1, A.U0);
+    const bool live = scan_cyclic(L, R, V, s, u0, u1, A.U0);
     if (live) flush_best(R, V, s);
     count_pairs(R, A.pairs + 1);
 }
@@ -598,7 +646,6 @@ __device__ __forceinline__ void refine_item(const ScanArgs &A, int64_t gb, int64
 // keeps about R.items work items per round whatever that count is.  The grid is fixed, so a
 // captured graph can replay it: at least 1.5 R.items workgroups and more than the number of
 // blocks; workgroup b takes item b, and the workgroups beyond the round's items exit at once.
-template <int PX>
 __global__ __launch_bounds__(64) void k_scan_refine(ScanPtrs Q, const PassDev *__restrict__ P,
                                                     const int *__restrict__ blist,
                                                     PassCtl *__restrict__ ctl, int round)
@@ -623,38 +670,19 @@ __global__ __launch_bounds__(64) void k_scan_refine(ScanPtrs Q, const PassDev *_
     const int64_t gb = blist[item / nchunks];
     const int64_t u0 = ub + (item % nchunks) * ch;
     const int64_t u1 = (u0 + ch < ue) ? u0 + ch : ue;
-    refine_item<PX>(A, gb, u0, u1);
+    refine_item(A, gb, u0, u1);
 }
 
-// Launchers: the prefetch depth PX is a template parameter (scan_px).
-void launch_scan_select(int px, const ScanPtrs &Q, const PassDev *P, unsigned grid, size_t lds,
+void launch_scan_select(const ScanPtrs &Q, const PassDev *P, unsigned grid, size_t lds,
                         hipStream_t st)
 {
-    if (px == 3)
-        hipLaunchKernelGGL((k_scan_select<3>), dim3(grid), dim3(64), lds, st, Q, P);
-    else if (px == 5)
-        hipLaunchKernelGGL((k_scan_select<5>), dim3(grid), dim3(64), lds, st, Q, P);
-    else if (px == 9)
-        hipLaunchKernelGGL((k_scan_select<9>), dim3(grid), dim3(64), lds, st, Q, P);
-    else
-        hipLaunchKernelGGL((k_scan_select<33>), dim3(grid), dim3(64), lds, st, Q, P);
+    hipLaunchKernelGGL(k_scan_select, dim3(grid), dim3(64), lds, st, Q, P);
 }
 
-void launch_scan_refine(int px, const ScanPtrs &Q, const PassDev *P, const int *blist,
-                        PassCtl *ctl, int round, unsigned grid, size_t lds, hipStream_t st)
+void launch_scan_refine(const ScanPtrs &Q, const PassDev *P, const int *blist, PassCtl *ctl,
+                        int round, unsigned grid, size_t lds, hipStream_t st)
 {
-    if (px == 3)
-        hipLaunchKernelGGL((k_scan_refine<3>), dim3(grid), dim3(64), lds, st, Q, P, blist, ctl,
-                           round);
-    else if (px == 5)
-        hipLaunchKernelGGL((k_scan_refine<5>), dim3(grid), dim3(64), lds, st, Q, P, blist, ctl,
-                           round);
-    else if (px == 9)
-        hipLaunchKernelGGL((k_scan_refine<9>), dim3(grid), dim3(64), lds, st, Q, P, blist, ctl,
-                           round);
-    else
-        hipLaunchKernelGGL((k_scan_refine<33>), dim3(grid), dim3(64), lds, st, Q, P, blist, ctl,
-                           round);
+    hipLaunchKernelGGL(k_scan_refine, dim3(grid), dim3(64), lds, st, Q, P, blist, ctl, round);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -67,6 +67,25 @@ def test_range_discords_match_oracle(ctx, case):
             assert rel.max() <= RTOL, (name, q, rel.max())
 
 
+def delta_ed2(m):
+    """The engine's pruning margin on squared distances (engine.cpp, delta_ed2)."""
+    return 2.0 * m * (4.0e-5 + 2.0e-7 * m)
+
+
+@pytest.mark.parametrize("m", [16, 100, 241, 242, 300])
+def test_single_precision_error_within_margin(ctx, m):
+    """The single-precision distances (matrix-core seeds + recurrence) stay within a quarter
+    of the margin that makes pruning safe, on a series far from zero (level 3000), at lengths
+    on both sides of the seed kernel's window-size steps (m + 15 <= 256 < m + 15)."""
+    x = (synth.random_walk(5000, 77)[0] + 3000.0).astype(np.float32)
+    flag, nn = ctx.range_discords(x, m, 0.0)
+    _, nn_o = O.range_discords(x, m, 0.0)
+    sel = (flag == 1) & (nn_o < 1e300)
+    assert sel.sum() > 1000
+    err = np.abs(nn[sel].astype(np.float64) - nn_o[sel]).max()
+    assert err <= delta_ed2(m) / 4, (m, err, delta_ed2(m))
+
+
 @pytest.mark.parametrize("kind,n,minL,maxL,step,k", [
     ("rw", 4000, 40, 44, 1, 2),
     ("rw", 9000, 64, 70, 3, 3),
