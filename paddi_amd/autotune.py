@@ -3,7 +3,7 @@ exact-refine speculation, hipGraph use).
 
     python -m paddi_amd.autotune [--workload c3] [--lengths 0] [--mode learned]
 
-Coordinate descent: starting from the current values, each tunable in turn is set to each
+Coordinate descent: starting from the stored entry (or the built-in defaults), each tunable in turn is set to each
 of its candidates; a candidate is timed as the median of `--reps` complete searches over
 the sample (the workload's first `--lengths` lengths, series resident in HBM) and kept if it
 is faster.  Every configuration must return the same discord set as the starting one (the
@@ -27,12 +27,13 @@ from paddi_amd import synth  # noqa: E402
 from paddi_amd.engine import TUNED_PATH, Context, tuned_key  # noqa: E402
 
 GRID = {
-    "witness_tiles": [0, 1, 2, 4, 8],
-    "select_window": [2048, 4096, 8192, 16384, 32768],
+    "witness_tiles": [0, 1, 2, 4, 8, 12, 16],
+    "select_window": [512, 1024, 2048, 4096, 8192, 16384],
     "refine_items": [4096, 8192, 16384, 32768, 65536],
     "min_chunk": [512, 1024, 2048, 4096, 8192],
     "refine_chunk": [16384, 32768, 65536, 131072],
-    "speculate": [1, 2, 4, 8],
+    "speculate": [1, 2, 3, 4, 8],
+    "use_graph": [0, 1],
 }
 
 
@@ -48,6 +49,7 @@ def main():
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--sweeps", type=int, default=2)
     ap.add_argument("--out", default=TUNED_PATH)
+    ap.add_argument("--fresh", action="store_true", help="start from the built-in defaults")
     a = ap.parse_args()
 
     w = synth.WORKLOADS[a.workload]()
@@ -55,6 +57,8 @@ def main():
     hs = [ctx.upload(x) for x in w["series"]]
     maxL = w["maxL"] if a.lengths <= 0 else \
         min(w["maxL"], w["minL"] + (a.lengths - 1) * w["step"])
+    if not a.fresh:
+        ctx.apply_tuned(w["minL"], path=a.out)  # continue from the stored entry, if any
     base = ctx.get_tunables()
 
     def run(tun):

@@ -267,8 +267,10 @@ struct View {
 };
 
 // climit: first column not to evaluate (min of the scan end and N); rows at or beyond N
-// are padding.
-template <bool MASKED>
+// are padding.  COLS: also collect the columns pruned by this block (colhit), which lets
+// k_scan_select retire other blocks' rows early; k_scan_refine skips that bookkeeping (the
+// far columns it scans belong to blocks that are almost all dead by then).
+template <bool MASKED, bool COLS>
 __device__ __forceinline__ void tile_steps(const float4 *tile, RowState &R, int64_t s, int64_t c,
                                            int64_t N, int64_t climit, int m, uint64_t &rowhit,
                                            uint64_t &colhit)
@@ -292,17 +294,19 @@ __device__ __forceinline__ void tile_steps(const float4 *tile, RowState &R, int6
             mx = fmaxf(mx, cj);
             const uint64_t h = __builtin_amdgcn_ballot_w64(hit);
             rowhit |= h;
-            colhit |= (h != 0) ? (1ull << j) : 0ull;
+            if (COLS) colhit |= (h != 0) ? (1ull << j) : 0ull;
         } else {
             const int d = dbase + j;
             const bool colin = j < colmax;
             const bool rv = colin && ((d >= m - 1) || (d <= -(m + 1)));
-            const bool cv = colin && rowin && ((d >= m + 1) || (d <= -(m - 1)));
             mx = rv ? fmaxf(mx, cj) : mx;
             const uint64_t hr = __builtin_amdgcn_ballot_w64(hit && rv);
-            const uint64_t hc = __builtin_amdgcn_ballot_w64(hit && cv);
             rowhit |= hr;
-            colhit |= (hc != 0) ? (1ull << j) : 0ull;
+            if (COLS) {
+                const bool cv = colin && rowin && ((d >= m + 1) || (d <= -(m - 1)));
+                const uint64_t hc = __builtin_amdgcn_ballot_w64(hit && cv);
+                colhit |= (hc != 0) ? (1ull << j) : 0ull;
+            }
         }
     }
     R.cov = cov;
@@ -362,6 +366,7 @@ __device__ __forceinline__ void ring_fill(const ScanLds &L, const float *__restr
 // floats per lane) and the next tile's column terms are loaded while the current ones are
 // processed, and the block's alive word read back by a tile's atomic is consumed one tile
 // later, so no global-memory round trip sits on the critical path.
+template <bool COLS>
 __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t s, int64_t c0,
                            int64_t c1)
 {
@@ -412,15 +417,15 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
             uint64_t rowhit = 0, colhit = 0;
             if (masked) {
                 const int64_t lim = c1 < N ? c1 : N;
-                tile_steps<true>(tile, R, s, ct, N, lim, m, rowhit, colhit);
+                tile_steps<true, COLS>(tile, R, s, ct, N, lim, m, rowhit, colhit);
             } else {
-                tile_steps<false>(tile, R, s, ct, N, N, m, rowhit, colhit);
+                tile_steps<false, COLS>(tile, R, s, ct, N, N, m, rowhit, colhit);
             }
             if (V.wit) {
                 if (((rowhit & R.alive) >> t) & 1ull) V.wit[s + t] = (int)ct;
-                if ((colhit >> t) & 1ull) V.wit[ct + t] = (int)s;
+                if (COLS && ((colhit >> t) & 1ull)) V.wit[ct + t] = (int)s;
             }
-            if (t == 0 && colhit)
+            if (COLS && t == 0 && colhit)
                 __hip_atomic_fetch_and(V.alive + (ct >> 6), ~colhit, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
             // Rows pruned elsewhere, as read back one tile ago, and rows pruned here now.
@@ -462,6 +467,7 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
 // that kill most blocks take the unmasked path; the band comes last, wrapped.  u0 is a
 // multiple of 64: an unwrapped piece starts aligned, a wrapped one at the aligned-down
 // column (re-scanning at most 63 columns is harmless: same values, same decisions).
+template <bool COLS>
 __device__ bool scan_cyclic(const ScanLds &L, RowState &R, const View &V, int64_t s, int64_t u0,
                             int64_t u1, int64_t U0)
 {
@@ -477,8 +483,8 @@ __device__ bool scan_cyclic(const ScanLds &L, RowState &R, const View &V, int64_
     }
     const int64_t e = c + (u1 - u0);
     if (wrapped) c &= ~(int64_t)63;
-    bool live = scan_range(L, R, V, s, c, e < N ? e : N);
-    if (live && e > N) live = scan_range(L, R, V, s, 0, e - N);
+    bool live = scan_range<COLS>(L, R, V, s, c, e < N ? e : N);
+    if (live && e > N) live = scan_range<COLS>(L, R, V, s, 0, e - N);
     return live;
 }
 
@@ -613,10 +619,10 @@ __global__ __launch_bounds__(64) void k_scan_select(ScanPtrs Q, const PassDev *_
                 we += WAVE;
             }
             const int64_t e = (we < lim) ? we : lim;
-            live = scan_range(L, R, V, s, wt, e);
+            live = scan_range<true>(L, R, V, s, wt, e);
         }
     }
-    if (live) live = scan_cyclic(L, R, V, s, A.U0, A.U0 + A.WA, A.U0);
+    if (live) live = scan_cyclic<true>(L, R, V, s, A.U0, A.U0 + A.WA, A.U0);
     if (live) flush_best(R, V, s);
     count_pairs(R, A.pairs);
 }
@@ -635,7 +641,7 @@ __device__ __forceinline__ void refine_item(const ScanArgs &A, int64_t gb, int64
     if (R.alive == 0) return;
     ScanLds L = carve_lds(V.m);
     load_block(L, R, V, s, thr_rho);
-    const bool live = scan_cyclic(L, R, V, s, u0, u1, A.U0);
+    const bool live = scan_cyclic<false>(L, R, V, s, u0, u1, A.U0);
     if (live) flush_best(R, V, s);
     count_pairs(R, A.pairs + 1);
 }
