@@ -190,7 +190,8 @@ Engine::Engine(int device) : device_(device)
     HIPC(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (auto &e : ev_) HIPC(hipEventCreate(&e));
     HIPC(hipHostMalloc(&pin_, 64, hipHostMallocDefault));
-    HIPC(hipHostMalloc(&pin_out_, sizeof(PassCtl) + (size_t)SCAP * 12, hipHostMallocDefault));
+    HIPC(hipHostMalloc(&pin_out_, PASS_OUT_HEAD + (size_t)SCAP * sizeof(Surv),
+                       hipHostMallocDefault));
     HIPC(hipEventCreateWithFlags(&ser_ev_, hipEventDisableTiming));
     const char *dbg = std::getenv("PADDI_DEBUG");
     debug_ = dbg && dbg[0] == '1';
@@ -203,7 +204,7 @@ Engine::~Engine()
     drop_graphs();
     for (auto &e : ev_) (void)hipEventDestroy(e);
     (void)hipEventDestroy(ser_ev_);
-    for (void *p : {pin_, pin_par_, pin_out_, pin_ser_})
+    for (void *p : {pin_, pin_par_, pin_out_, pin_ser_, pin_x_})
         if (p) (void)hipHostFree(p);
     (void)hipStreamDestroy(stream_);
 }
@@ -369,7 +370,8 @@ void Engine::enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t ro
                           int64_t nwords, int64_t scap, bool timed)
 {
     const PassDev *P = par_.as<PassDev>();
-    PassCtl *ctl = ctl_.as<PassCtl>();
+    PassCtl *ctl = pout_.as<PassCtl>();
+    Surv *surv = (Surv *)((char *)pout_.p + PASS_OUT_HEAD);
     // k_scan_refine needs more workgroups than listed blocks (see there)
     const unsigned grid =
         (unsigned)std::max<int64_t>(refine_waves_ + refine_waves_ / 2, nwords + 64);
@@ -391,12 +393,9 @@ void Engine::enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t ro
     }
     if (timed) HIPC(hipEventRecord(ev_[3], stream_));
     hipLaunchKernelGGL(k_survivors, dim3(grid1(rows, 256)), dim3(256), 0, stream_, Q, P, rows,
-                       ctl, sidx_.as<int64_t>(), sd2_.as<float>(), rows);
-    HIPC(hipMemcpyAsync(pin_out_, ctl_.p, sizeof(PassCtl), hipMemcpyDeviceToHost, stream_));
-    char *o = (char *)pin_out_ + sizeof(PassCtl);
-    HIPC(hipMemcpyAsync(o, sidx_.p, (size_t)scap * 8, hipMemcpyDeviceToHost, stream_));
-    HIPC(hipMemcpyAsync(o + (size_t)SCAP * 8, sd2_.p, (size_t)scap * 4, hipMemcpyDeviceToHost,
-                        stream_));
+                       ctl, surv, rows);
+    HIPC(hipMemcpyAsync(pin_out_, pout_.p, PASS_OUT_HEAD + (size_t)scap * sizeof(Surv),
+                        hipMemcpyDeviceToHost, stream_));
     HIPC(hipGetLastError());
 }
 
@@ -414,9 +413,7 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
     alive_.reserve((size_t)nwords * 8);
     best_.reserve((size_t)rows * 4);
     blist_.reserve((size_t)nwords * 4);
-    ctl_.reserve(sizeof(PassCtl));
-    sidx_.reserve((size_t)rows * 8);
-    sd2_.reserve((size_t)rows * 4);
+    pout_.reserve(PASS_OUT_HEAD + (size_t)rows * sizeof(Surv));
     wit_[0].reserve((size_t)rows * 4);
     wit_[1].reserve((size_t)rows * 4);
     par_.reserve(parb);
@@ -441,7 +438,7 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
     Q.best = best_.as<int>();
     Q.wit0 = wit_[0].as<int>();
     Q.wit1 = wit_[1].as<int>();
-    Q.pairs = ctl_.as<PassCtl>()->pairs;
+    Q.pairs = pout_.as<PassCtl>()->pairs;
     ScanScal &A = P.S;
     A = ScanScal{};
     A.WA = select_window_;
@@ -464,8 +461,8 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
 
     if (use_graph_) {
         const std::vector<uintptr_t> sig = {
-            (uintptr_t)par_.p,   (uintptr_t)pin_par_, (uintptr_t)pin_out_, (uintptr_t)ctl_.p,
-            (uintptr_t)alive_.p, (uintptr_t)blist_.p, (uintptr_t)sidx_.p,  (uintptr_t)sd2_.p,
+            (uintptr_t)par_.p,   (uintptr_t)pin_par_, (uintptr_t)pin_out_, (uintptr_t)pout_.p,
+            (uintptr_t)alive_.p, (uintptr_t)blist_.p,
             (uintptr_t)Q.x,      (uintptr_t)Q.bser,   (uintptr_t)Q.colv,   (uintptr_t)Q.best,
             (uintptr_t)Q.wit0,   (uintptr_t)Q.wit1,   (uintptr_t)rows,     (uintptr_t)nwords,
             (uintptr_t)parb,     (uintptr_t)scap,     (uintptr_t)refine_waves_};
@@ -510,23 +507,20 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
     }
     out.assign((size_t)B.nser, {});
     if (ns > 0) {
-        const char *o = (const char *)pin_out_ + sizeof(PassCtl);
-        std::vector<int64_t> ix((size_t)ns);
-        std::vector<float> dd((size_t)ns);
+        std::vector<Surv> sv((size_t)ns);
         const int64_t head = std::min<int64_t>(ns, scap);
-        std::memcpy(ix.data(), o, (size_t)head * 8);
-        std::memcpy(dd.data(), o + (size_t)SCAP * 8, (size_t)head * 4);
+        std::memcpy(sv.data(), (const char *)pin_out_ + PASS_OUT_HEAD, (size_t)head * sizeof(Surv));
         if (ns > head) {
-            HIPC(hipMemcpyAsync(ix.data() + head, sidx_.as<int64_t>() + head,
-                                (size_t)(ns - head) * 8, hipMemcpyDeviceToHost, stream_));
-            HIPC(hipMemcpyAsync(dd.data() + head, sd2_.as<float>() + head,
-                                (size_t)(ns - head) * 4, hipMemcpyDeviceToHost, stream_));
+            HIPC(hipMemcpyAsync(sv.data() + head,
+                                (const char *)pout_.p + PASS_OUT_HEAD + (size_t)head * sizeof(Surv),
+                                (size_t)(ns - head) * sizeof(Surv), hipMemcpyDeviceToHost,
+                                stream_));
             HIPC(hipStreamSynchronize(stream_));
         }
         for (int q = 0; q < ns; q++) {
-            const int64_t g = ix[(size_t)q];
+            const int64_t g = sv[(size_t)q].g;
             const int sid = B.bser_host[(size_t)(g >> 6)];
-            out[(size_t)sid].push_back(Cand{g - B.roff[(size_t)sid], dd[(size_t)q], 0.0, false});
+            out[(size_t)sid].push_back(Cand{g - B.roff[(size_t)sid], sv[(size_t)q].d2, 0.0, false});
         }
     }
     if (st) {
@@ -606,37 +600,47 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
         goff.push_back(L.blocks);
         launches.push_back(L);
     }
-    cand_.reserve(cand.size() * 8);
-    okey_.reserve(cand.size() * 8);
-    gfirst_.reserve(gfirst.size() * 4);
-    gser_.reserve(gser.size() * 4);
-    goff_.reserve(goff.size() * 8);
-    HIPC(hipMemcpyAsync(cand_.p, cand.data(), cand.size() * 8, hipMemcpyHostToDevice, stream_));
-    HIPC(hipMemcpyAsync(gfirst_.p, gfirst.data(), gfirst.size() * 4, hipMemcpyHostToDevice,
-                        stream_));
-    HIPC(hipMemcpyAsync(gser_.p, gser.data(), gser.size() * 4, hipMemcpyHostToDevice, stream_));
-    HIPC(hipMemcpyAsync(goff_.p, goff.data(), goff.size() * 8, hipMemcpyHostToDevice, stream_));
-    HIPC(hipMemsetAsync(okey_.p, 0xff, cand.size() * 8, stream_));
+    // one block: outkey (set to the maximum key) | cand | goff | gfirst | gser, staged in
+    // pinned memory and copied in one piece
+    const size_t nc = cand.size(), ngx = gfirst.size();
+    const size_t o_cand = nc * 8, o_goff = o_cand + nc * 8, o_gfirst = o_goff + ngx * 8,
+                 o_gser = o_gfirst + ngx * 4, xbytes = o_gser + ngx * 4;
+    if (xbytes > pin_x_bytes_) {
+        if (pin_x_) HIPC(hipHostFree(pin_x_));
+        pin_x_ = nullptr;
+        pin_x_bytes_ = 0;
+        HIPC(hipHostMalloc(&pin_x_, xbytes * 2, hipHostMallocDefault));
+        pin_x_bytes_ = xbytes * 2;
+    }
+    char *hx = (char *)pin_x_;
+    std::memset(hx, 0xff, nc * 8);
+    std::memcpy(hx + o_cand, cand.data(), nc * 8);
+    std::memcpy(hx + o_goff, goff.data(), ngx * 8);
+    std::memcpy(hx + o_gfirst, gfirst.data(), ngx * 4);
+    std::memcpy(hx + o_gser, gser.data(), ngx * 4);
+    xarg_.reserve(xbytes);
+    char *dx = (char *)xarg_.p;
+    HIPC(hipMemcpyAsync(dx, hx, xbytes, hipMemcpyHostToDevice, stream_));
     HIPC(hipEventRecord(ev_[6], stream_));
     for (const Launch &L : launches) {
         ExactArgs E{};
         E.x = B.x.as<float>();
         E.ser = B.ser.as<SerDev>();
         E.st64 = st64_.as<double2>();
-        E.cand = cand_.as<int64_t>();
-        E.gfirst = gfirst_.as<int>() + L.off;
-        E.gser = gser_.as<int>() + L.off;
-        E.goff = goff_.as<int64_t>() + L.off;
-        E.outkey = okey_.as<unsigned long long>();
+        E.cand = (const int64_t *)(dx + o_cand);
+        E.gfirst = (const int *)(dx + o_gfirst) + L.off;
+        E.gser = (const int *)(dx + o_gser) + L.off;
+        E.goff = (const int64_t *)(dx + o_goff) + L.off;
+        E.outkey = (unsigned long long *)dx;
         E.ngroups = L.ngroups;
         E.m = m;
         launch_exact(E, L.nu, (unsigned)L.blocks, exact_lds_bytes(m, L.nu), stream_);
         HIPC(hipGetLastError());
     }
     HIPC(hipEventRecord(ev_[7], stream_));
-    std::vector<unsigned long long> keys(cand.size());
-    HIPC(hipMemcpyAsync(keys.data(), okey_.p, cand.size() * 8, hipMemcpyDeviceToHost, stream_));
+    HIPC(hipMemcpyAsync(hx, dx, nc * 8, hipMemcpyDeviceToHost, stream_));
     HIPC(hipStreamSynchronize(stream_));
+    const unsigned long long *keys = (const unsigned long long *)hx;
     collect_derive();
     for (size_t q = 0; q < slot.size(); q++) {
         const unsigned long long k = keys[q];

@@ -154,8 +154,10 @@ private:
     void *pin_par_ = nullptr, *pin_out_ = nullptr, *pin_ser_ = nullptr;
     size_t pin_par_bytes_ = 0, pin_ser_bytes_ = 0;
     hipEvent_t ser_ev_ = nullptr;
-    DevBuf colv_, st64_, alive_, best_, wit_[2], blist_, sidx_, sd2_, par_, ctl_;
-    DevBuf cand_, okey_, gfirst_, gser_, goff_;
+    DevBuf colv_, st64_, alive_, best_, wit_[2], blist_, par_, pout_;
+    DevBuf xarg_;                  // exact-refine arguments and results (one block)
+    void *pin_x_ = nullptr;        // its pinned staging
+    size_t pin_x_bytes_ = 0;
     int witcur_ = 0;
 };
 

@@ -116,9 +116,17 @@ __global__ void k_pass_init(ScanPtrs Q, const PassDev *__restrict__ P, int64_t r
 __global__ void k_fill_int(int *__restrict__ p, int64_t n, int v);
 __global__ void k_list_blocks(const uint64_t *__restrict__ alive, int64_t nwords,
                               int *__restrict__ list, int *__restrict__ count);
+// A surviving row: global row and single-precision squared NN distance.
+struct Surv {
+    int64_t g;
+    float d2;
+    int32_t pad;
+};
+// Pass results as one device block: the counters, then the survivors (one copy to the host).
+constexpr size_t PASS_OUT_HEAD = 64;
+static_assert(sizeof(PassCtl) <= PASS_OUT_HEAD, "PassCtl fits the head of the pass results");
 __global__ void k_survivors(ScanPtrs Q, const PassDev *__restrict__ P, int64_t rows,
-                            PassCtl *__restrict__ ctl, int64_t *__restrict__ idx,
-                            float *__restrict__ d2, int64_t cap);
+                            PassCtl *__restrict__ ctl, Surv *__restrict__ out, int64_t cap);
 // k_exact is a template on the largest group size nu of the launch (1..MAXU).
 void launch_exact(const ExactArgs &E, int nu, unsigned grid, size_t lds, hipStream_t st);
 

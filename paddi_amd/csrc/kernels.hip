@@ -745,8 +745,7 @@ __global__ __launch_bounds__(256) void k_list_blocks(const uint64_t *__restrict_
 // Surviving rows -> (global row, FP32 squared distance); FLT_MAX when no valid pair was seen.
 __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__restrict__ P,
                                                    int64_t rows, PassCtl *__restrict__ ctl,
-                                                   int64_t *__restrict__ idx,
-                                                   float *__restrict__ d2, int64_t cap)
+                                                   Surv *__restrict__ out, int64_t cap)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= rows) return;
@@ -759,10 +758,7 @@ __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__
     else
         d = 2.0f * (float)A.m * (1.0f - ord_val(k));
     const int p = atomicAdd(&ctl->survivors, 1);
-    if (p < cap) {
-        idx[p] = g;
-        d2[p] = d;
-    }
+    if (p < cap) out[p] = Surv{g, d, 0};
 }
 
 // ---------------------------------------------------------------------------------------
