@@ -45,7 +45,8 @@ def parse():
     p.add_argument("--primary", default="auto", help="plain|batched|learned|auto")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true", help="skip the plain form (profiling)")
-    p.add_argument("--cpu-points", type=int, default=65536)
+    p.add_argument("--cpu-points", type=int, default=131072)
+    p.add_argument("--no-tuned", action="store_true", help="ignore the autotuner's table")
     return p.parse_args()
 
 
@@ -108,6 +109,9 @@ def main():
     L = load_library()
     ctx = Context(0)
     handles = [ctx.upload(x) for x in series]
+    if not a.no_tuned:
+        ctx.apply_tuned(w["minL"])
+    tunables = ctx.get_tunables()
 
     primary = a.primary
     if primary == "auto":
@@ -142,8 +146,20 @@ def main():
     nl = {k: sum(s["launches"][k] for s in stats) for k in names}
     dom = max(names, key=lambda k: tot[k])
     avg_ms = tot[dom] / max(nl[dom], 1)
-    # algorithmic bytes per launch: 16 B per (point x length); one launch covers one length
-    bytes_per_launch = 16.0 * points
+    kname = {"select": "paddi::k_scan_select", "refine": "paddi::k_scan_refine",
+             "derive": "paddi::k_derive", "exact": "paddi::k_exact"}[dom]
+    # algorithmic bytes: 16 B per (point x length) for each pass over a length, shared by the
+    # kernel's launches of that pass (k_scan_refine runs as three round launches per pass)
+    passes = sum(s["attempts"] for s in stats)
+    bytes_per_launch = 16.0 * points * passes / max(nl[dom], 1)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{a.workload}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        for k, v in tj.get("kernels", {}).items():
+            if k.startswith(kname):
+                traffic = v["bytes_per_dispatch"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     pairs = sum(s["pairs"] for s in stats) / len(stats)
     scan_ms = (tot["select"] + tot["refine"]) / len(stats)
@@ -173,11 +189,11 @@ def main():
         "survivors": stats[-1]["survivors"],
         "exact_refines": stats[-1]["exact_refines"],
         "kernel_ms_per_step": {k: tot[k] / len(stats) for k in names},
-        "roofline": {"bound": "hbm", "kernel": f"k_scan_{dom}" if dom in ("select", "refine")
-                     else f"k_{dom}", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "avg_launch_ms": avg_ms, "launches": nl[dom],
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "avg_launch_ms": avg_ms, "launches": nl[dom],
                      "bytes_per_launch": bytes_per_launch},
+        "tunables": tunables,
         "valu": {"pairs_per_step": pairs, "ops_per_pair": OPS_PER_PAIR,
                  "frac_of_peak": valu},
     }

@@ -1,0 +1,110 @@
+"""Helpers for the rocprofv3 evidence of the discord engine.
+
+    python3 -m paddi_amd.profiles copy                # 1 GiB device-to-device copy (calibration)
+    python3 -m paddi_amd.profiles stats CSV OUT.md CMD  # kernel-trace stats -> markdown table
+    python3 -m paddi_amd.profiles traffic OUT.json KERNEL FETCH.csv WRITE.csv COPYF.csv COPYW.csv CMD
+
+`traffic` turns two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs (KiB per dispatch)
+into HBM bytes per dispatch of KERNEL, each counter scaled by the factor that makes the
+copy kernel's reading equal to the bytes it is known to move (1 GiB read, 1 GiB written).
+bench.py reads the JSON for its roofline `traffic`.
+"""
+from __future__ import annotations
+
+import csv
+import json
+import os
+import sys
+from collections import defaultdict
+
+COPY_BYTES = 1 << 30
+
+
+def copy():
+    import torch
+    a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print("copied", COPY_BYTES, "bytes", flush=True)
+
+
+def _counter_rows(path, counter):
+    """Per-dispatch values of `counter` by kernel (short name)."""
+    out = defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            name = r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+            out[name].append(float(r["Counter_Value"]))
+    return out
+
+
+def _mean(v):
+    return sum(v) / len(v) if v else 0.0
+
+
+def _copy_kernel(rows):
+    """The largest dispatch of the calibration run (the copy)."""
+    best = max(rows.items(), key=lambda kv: max(kv[1]))
+    return best[0], max(best[1])
+
+
+def traffic(out, kernel, fetch_csv, write_csv, copyf_csv, copyw_csv, cmd):
+    fetch = _counter_rows(fetch_csv, "FETCH_SIZE")
+    write = _counter_rows(write_csv, "WRITE_SIZE")
+    cname_f, cf = _copy_kernel(_counter_rows(copyf_csv, "FETCH_SIZE"))
+    cname_w, cw = _copy_kernel(_counter_rows(copyw_csv, "WRITE_SIZE"))
+    sf = COPY_BYTES / 1024.0 / cf if cf > 0 else None
+    sw = COPY_BYTES / 1024.0 / cw if cw > 0 else None
+    res = {"command": cmd, "copy_kernel": [cname_f, cname_w],
+           "copy_kib": {"FETCH_SIZE": cf, "WRITE_SIZE": cw},
+           "scale": {"FETCH_SIZE": sf, "WRITE_SIZE": sw}, "kernels": {}}
+    for k in sorted(set(fetch) | set(write)):
+        f_kib, w_kib = _mean(fetch.get(k, [])), _mean(write.get(k, []))
+        res["kernels"][k] = {
+            "dispatches": max(len(fetch.get(k, [])), len(write.get(k, []))),
+            "fetch_kib": f_kib, "write_kib": w_kib,
+            "bytes_per_dispatch": (f_kib * (sf or 0) + w_kib * (sw or 0)) * 1024.0}
+    res["kernel"] = kernel
+    with open(out, "w") as f:
+        json.dump(res, f, indent=1, sort_keys=True)
+    md = os.path.splitext(out)[0] + ".md"
+    with open(md, "w") as f:
+        f.write(f"# HBM traffic per dispatch (rocprofv3 PMC)\n\nCommand:\n\n```\n{cmd}\n```\n\n")
+        f.write(f"Calibration: a {COPY_BYTES >> 20} MiB device copy reads FETCH_SIZE = {cf:.0f} KiB "
+                f"and WRITE_SIZE = {cw:.0f} KiB; scale factors {sf:.3f} (fetch), {sw:.3f} (write).\n\n")
+        f.write("| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | scaled MB / dispatch |\n")
+        f.write("|---|---:|---:|---:|---:|\n")
+        for k, v in sorted(res["kernels"].items(), key=lambda kv: -kv[1]["bytes_per_dispatch"]):
+            f.write(f"| {k} | {v['dispatches']} | {v['fetch_kib']:.1f} | {v['write_kib']:.1f} | "
+                    f"{v['bytes_per_dispatch'] / 1e6:.3f} |\n")
+    print("wrote", out, md)
+
+
+def stats(path, out, cmd):
+    rows = list(csv.DictReader(open(path)))
+    with open(out, "w") as f:
+        f.write(f"# Kernel time (rocprofv3 --kernel-trace --stats)\n\nCommand:\n\n```\n{cmd}\n```\n\n")
+        f.write("| kernel | calls | total ms | average us | min us | max us | % |\n")
+        f.write("|---|---:|---:|---:|---:|---:|---:|\n")
+        for r in rows:
+            name = r["Name"].split("(")[0].replace("void ", "")
+            f.write(f"| {name} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
+                    f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
+                    f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |\n")
+    print("wrote", out)
+
+
+if __name__ == "__main__":
+    what = sys.argv[1]
+    if what == "copy":
+        copy()
+    elif what == "stats":
+        stats(sys.argv[2], sys.argv[3], sys.argv[4])
+    elif what == "traffic":
+        traffic(*sys.argv[2:9])
+    else:
+        raise SystemExit(__doc__)
