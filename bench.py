@@ -32,6 +32,7 @@ WORKLOAD_NAMES = {
     "c2": "ecg1m_m64-128_top3",
     "c3": "rw8m_m128-256_top3",
     "c4": "ar2mix2m_m256-512s4_top3",
+    "c5": "uneven1024_m64-96_top1",
 }
 
 
@@ -47,6 +48,8 @@ def parse():
     p.add_argument("--no-plain", action="store_true", help="skip the plain form (profiling)")
     p.add_argument("--cpu-points", type=int, default=131072)
     p.add_argument("--no-tuned", action="store_true", help="ignore the autotuner's table")
+    p.add_argument("--no-gain", action="store_true",
+                   help="skip the batched form without the learned proposal")
     return p.parse_args()
 
 
@@ -135,6 +138,15 @@ def main():
     if primary == "plain":
         recs, times, stats = plain_recs, plain_t, plain_st
     identical = same_set(recs, plain_recs)
+    gain = None
+    if primary == "learned" and not a.no_gain:
+        # what the trained network gains: the same batched form with the probe descent
+        b_recs, b_t, b_st = run_form(ctx, handles, w, maxL, "batched", 1, 0)
+        gain = {"batched_ms_per_step": 1e3 * b_t[0], "batched_attempts": b_st[0]["attempts"],
+                "learned_attempts": stats[-1]["attempts"],
+                "identical_discord_set": same_set(recs, b_recs)}
+        print(f"[bench] batched (no learned proposal): {b_t[0]:.3f} s", file=sys.stderr,
+              flush=True)
 
     t_step = sum(times) / len(times)
     value = units / t_step
@@ -184,6 +196,7 @@ def main():
         "plain": {"value": plain_value, "ms_per_step": 1e3 * sum(plain_t) / len(plain_t),
                   "attempts": plain_st[-1]["attempts"]},
         "speedup_vs_plain": value / plain_value,
+        "learned_gain": gain,
         "identical_discord_set": identical,
         "attempts": stats[-1]["attempts"],
         "survivors": stats[-1]["survivors"],

@@ -87,7 +87,8 @@ typedef struct {
     int32_t witness_tiles;   /* witness ranges scanned first (0 = off) */
     int32_t speculate;       /* candidates refined per exact round */
     int32_t use_graph;       /* capture the per-length device pipeline in a hipGraph */
-    int32_t reserved;
+    int32_t prev_shift;      /* the next length's r: previous discords evaluated at i and at
+                                i-1 .. i-prev_shift (0..4) */
 } paddi_tunables;
 
 const char *paddi_status_str(paddi_status s);

@@ -34,6 +34,7 @@ GRID = {
     "refine_chunk": [16384, 32768, 65536, 131072],
     "speculate": [1, 2, 3, 4, 8],
     "use_graph": [0, 1],
+    "prev_shift": [0, 1, 2],
 }
 
 

@@ -770,20 +770,45 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
         derive(B, m, st);
         std::vector<double> r2((size_t)ns, 0.0);
         std::vector<char> from_prev((size_t)ns, 0);
-        // r from the previous length's discords, evaluated exactly at this length
+        // r from the previous length's discords, evaluated exactly at this length.  A
+        // discord's window re-centres on its anomaly as m grows, so its start stays or moves
+        // back: each one is evaluated at i, i-1 .. i-prev_shift_ and the best kept (any row's
+        // exact NN distance is a lower bound of d_1); the minimum over the k discords is the
+        // estimate of d_k (a pass that finds fewer than k retries lower).
         {
             std::vector<std::pair<int, int64_t>> req;
+            std::vector<int> grp;  // request -> (series, discord) group
+            int ngrp = 0;
+            std::vector<int> gser;
             for (int q = 0; q < ns; q++)
                 if ((int)prev[(size_t)q].size() >= k)
-                    for (int64_t i : prev[(size_t)q])
-                        if (i <= B.n[(size_t)q] - m) req.emplace_back(q, i);
+                    for (int64_t i : prev[(size_t)q]) {
+                        const int64_t Nq = B.n[(size_t)q] - m + 1;
+                        bool any = false;
+                        for (int64_t d = -prev_shift_; d <= 0; d++) {
+                            const int64_t j = i + d;
+                            if (j < 0 || j >= Nq) continue;
+                            req.emplace_back(q, j);
+                            grp.push_back(ngrp);
+                            any = true;
+                        }
+                        if (any) {
+                            gser.push_back(q);
+                            ngrp++;
+                        }
+                    }
             std::vector<double> v;
             exact(B, m, req, v, st);
+            std::vector<double> best((size_t)ngrp, -INFINITY);
+            for (size_t u = 0; u < req.size(); u++)
+                if (v[u] < 3.0e38) best[(size_t)grp[u]] = std::max(best[(size_t)grp[u]], v[u]);
             std::vector<double> mn((size_t)ns, INFINITY);
             std::vector<int> cnt((size_t)ns, 0);
-            for (size_t u = 0; u < req.size(); u++) {
-                mn[(size_t)req[u].first] = std::min(mn[(size_t)req[u].first], v[u]);
-                cnt[(size_t)req[u].first]++;
+            for (int g = 0; g < ngrp; g++) {
+                const int q = gser[(size_t)g];
+                mn[(size_t)q] = std::min(mn[(size_t)q], best[(size_t)g] > -INFINITY
+                                                            ? best[(size_t)g] : INFINITY);
+                cnt[(size_t)q]++;
             }
             for (int q = 0; q < ns; q++)
                 if (cnt[(size_t)q] >= k && std::isfinite(mn[(size_t)q]) && mn[(size_t)q] > 0.0) {
@@ -927,7 +952,7 @@ paddi_status paddi_get_tunables(paddi_ctx *ctx, paddi_tunables *out)
     if (!ctx || !out) return PADDI_ERR_INVALID;
     const paddi::Engine &E = *ctx->eng;
     *out = paddi_tunables{E.select_window_, E.refine_chunk_, E.min_chunk_, E.refine_waves_,
-                          E.witness_tiles_, E.spec_, E.use_graph_ ? 1 : 0, 0};
+                          E.witness_tiles_, E.spec_, E.use_graph_ ? 1 : 0, E.prev_shift_};
     return PADDI_OK;
 }
 
@@ -936,7 +961,8 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
     if (!ctx || !t) return PADDI_ERR_INVALID;
     if (t->select_window < 64 || t->refine_chunk < 64 || t->min_chunk < 64 ||
         t->min_chunk > t->refine_chunk || t->refine_items < 1 || t->witness_tiles < 0 ||
-        t->witness_tiles > 16 || t->speculate < 1 || t->speculate > 64)
+        t->witness_tiles > 16 || t->speculate < 1 || t->speculate > 64 || t->prev_shift < 0 ||
+        t->prev_shift > 4)
         return PADDI_ERR_INVALID;
     paddi::Engine &E = *ctx->eng;
     E.select_window_ = (t->select_window + 63) / 64 * 64;
@@ -946,6 +972,7 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
     E.witness_tiles_ = t->witness_tiles;
     E.spec_ = t->speculate;
     E.use_graph_ = t->use_graph != 0;
+    E.prev_shift_ = t->prev_shift;
     return PADDI_OK;
 }
 

@@ -104,6 +104,7 @@ public:
     int witness_tiles_ = 4;          // witness ranges scanned first by k_scan_select
     int spec_ = 4;                   // candidates refined per exact round
     bool use_graph_ = false;         // per-length device pipeline as a hipGraph
+    int prev_shift_ = 1;             // next r from previous discords at i, i-1 .. i-prev_shift
     bool debug_ = false;             // per-pass trace on stderr (PADDI_DEBUG=1)
 
 private:

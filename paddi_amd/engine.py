@@ -62,9 +62,9 @@ class Tunables(C.Structure):
     _fields_ = [("select_window", C.c_int64), ("refine_chunk", C.c_int64),
                 ("min_chunk", C.c_int64), ("refine_items", C.c_int64),
                 ("witness_tiles", C.c_int32), ("speculate", C.c_int32),
-                ("use_graph", C.c_int32), ("reserved", C.c_int32)]
+                ("use_graph", C.c_int32), ("prev_shift", C.c_int32)]
     NAMES = ("select_window", "refine_chunk", "min_chunk", "refine_items", "witness_tiles",
-             "speculate", "use_graph")
+             "speculate", "use_graph", "prev_shift")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.NAMES}

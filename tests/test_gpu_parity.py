@@ -195,6 +195,30 @@ def test_learned_matches_oracle(ctx, kind, n, minL, maxL, step, k):
     assert_same_discords(got, expected(ref), f"learned {kind}")
 
 
+def test_learned_proposal_saves_passes(ctx):
+    """What the trained network gains: on 24 series it never saw (random walks, ECG-like and
+    AR(2) mixtures of 20k-60k points, seeds outside the training set), the learned first-length
+    proposal replaces the probe descent from r2 = 2m, so the call runs fewer passes, while the
+    discord set is identical to the batched form's (and the batched form is the oracle's,
+    tested above)."""
+    rng = np.random.default_rng(2024)
+    series = []
+    for q in range(24):
+        n = int(rng.integers(20000, 60000))
+        kind = q % 3
+        series.append(synth.rw_planted(n, 5000 + q, 2, 64, 16.0) if kind == 0 else
+                      synth.ecg_like(n, 6000 + q, 1, 1) if kind == 1 else
+                      synth.ar2_mix(n, 7000 + q))
+    minL, maxL, k = 64, 67, 2
+    b, sb = ctx.discords_batch(series, minL, maxL, 1, k, mode="batched", with_stats=True)
+    l, sl = ctx.discords_batch(series, minL, maxL, 1, k, mode="learned", with_stats=True)
+    assert [(g.series, g.index, g.length, g.nn_dist2) for g in l] == \
+        [(g.series, g.index, g.length, g.nn_dist2) for g in b]
+    print(f"passes: batched {sb['attempts']}, learned {sl['attempts']}; "
+          f"pairs {sb['pairs']:.3g} vs {sl['pairs']:.3g}")
+    assert sl["attempts"] < sb["attempts"]
+
+
 def test_learned_proposal_too_high_falls_back(ctx, tmp_path):
     """A model whose margin is hugely negative proposes r far above every discord distance:
     the descent must fall back and still return the oracle's set."""
