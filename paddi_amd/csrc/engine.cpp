@@ -961,7 +961,7 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
     if (!ctx || !t) return PADDI_ERR_INVALID;
     if (t->select_window < 64 || t->refine_chunk < 64 || t->min_chunk < 64 ||
         t->min_chunk > t->refine_chunk || t->refine_items < 1 || t->witness_tiles < 0 ||
-        t->witness_tiles > 16 || t->speculate < 1 || t->speculate > 64 || t->prev_shift < 0 ||
+        t->witness_tiles > 64 || t->speculate < 1 || t->speculate > 64 || t->prev_shift < 0 ||
         t->prev_shift > 4)
         return PADDI_ERR_INVALID;
     paddi::Engine &E = *ctx->eng;
