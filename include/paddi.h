@@ -89,6 +89,9 @@ typedef struct {
     int32_t use_graph;       /* capture the per-length device pipeline in a hipGraph */
     int32_t prev_shift;      /* the next length's r: previous discords evaluated at i and at
                                 i-1 .. i-prev_shift (0..4) */
+    int32_t bootstrap;       /* learned form, first length: r also bounded from below by the
+                                exact NN of the best-scored candidate rows (0/1) */
+    int32_t reserved;        /* 0 */
 } paddi_tunables;
 
 const char *paddi_status_str(paddi_status s);
@@ -120,12 +123,32 @@ paddi_status paddi_compute_statistics(paddi_ctx *ctx, const float *T, int64_t n,
 paddi_status paddi_range_discords(paddi_ctx *ctx, const float *T, int64_t n, int32_t m, float r2,
                                   int32_t *is_discord, float *nn_dist2);
 
+/* One length of a search, for the reference's times table (src/IOdata.hpp:140-177): the
+ * phases of one GPU (statistics, scans, exact refinement) and the candidate counts. */
+typedef struct {
+    int32_t m;
+    int32_t attempts;        /* passes at this length (r-descent steps) */
+    int64_t survivors;       /* rows left by the scans over those passes (candidates) */
+    int32_t discords;        /* discords accepted at this length */
+    int32_t reserved;
+    double ms_derive;        /* k_derive: window statistics */
+    double ms_scan;          /* k_scan_select + k_scan_refine */
+    double ms_exact;         /* k_exact: double-precision refinement */
+    double ms_total;         /* host wall time of the length */
+} paddi_length_stats;
+
 /* Top-k discords for every length m = minL, minL+step, .., <= maxL.
  * Replaces the PALMAD length loop of src/main.cu:136-233 (top-1 there; for
  * k > 1 discords are ranked by (nn_dist2 desc, index asc) and accepted when
  * |i - a| >= m for every accepted a).  out[] receives up to top_k records per
  * length, lengths ascending, each length's records in rank order; *count is
  * the number written.  opts and stats may be NULL. */
+/* paddi_discords with a record per length (per_length[(maxL - minL) / step + 1]); the
+ * CLI's times table.  No counterpart beyond the reference's timers (src/main.cu:136-233). */
+paddi_status paddi_discords_timed(paddi_ctx *ctx, const float *T, int64_t n, int32_t minL,
+                                  int32_t maxL, int32_t step, int32_t top_k,
+                                  const paddi_opts *opts, paddi_discord *out, int64_t cap,
+                                  int64_t *count, paddi_length_stats *per_length);
 paddi_status paddi_discords(paddi_ctx *ctx, const float *T, int64_t n, int32_t minL, int32_t maxL,
                             int32_t step, int32_t top_k, const paddi_opts *opts,
                             paddi_discord *out, int64_t cap, int64_t *count, paddi_stats *stats);
@@ -157,7 +180,7 @@ paddi_status paddi_discords_series(paddi_ctx *ctx, paddi_series *const *series, 
  * (X - fmu) / fsd.  Loss: mean squared error. */
 paddi_status paddi_mlp_sizes(int32_t *nfeat, int32_t *hidden, int32_t *nparam);
 paddi_status paddi_features(paddi_ctx *ctx, paddi_series *const *series, int32_t nseries,
-                            int32_t m, float *out /* [nseries][nfeat] */);
+                            int32_t m, int32_t top_k, float *out /* [nseries][nfeat] */);
 paddi_status paddi_mlp_forward(paddi_ctx *ctx, const float *params, const float *fmu,
                                const float *fsd, const float *X, int32_t B, float *Y);
 paddi_status paddi_mlp_gradient(paddi_ctx *ctx, const float *params, const float *fmu,

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import numpy as np
 
-NFEAT, HID, NPAIR = 34, 64, 1024
+NFEAT, HID, NPAIR = 36, 64, 1024
 NPARAM = NFEAT * HID + HID + HID * HID + HID + HID + 1
 FEAT_SEED = 0x5eed5eed1234
 
@@ -79,7 +79,7 @@ def pair_indices(sid, N, m, seed=FEAT_SEED):
     return out
 
 
-def features(x, m, sid=0):
+def features(x, m, sid=0, k=1):
     """Features of one series at length m in double precision (bins may differ from the
     single-precision kernel only for pair distances within rounding of a bin edge)."""
     x64 = np.asarray(x, np.float64)
@@ -95,5 +95,7 @@ def features(x, m, sid=0):
     f = np.empty(NFEAT)
     f[0] = m / 512.0
     f[1] = np.log10(max(x64.std(), 1e-30)) / 4.0
-    f[2:] = hist
+    f[2:34] = hist
+    f[34] = np.log10(len(x)) / 6.0
+    f[35] = k / 4.0
     return f

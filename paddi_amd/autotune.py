@@ -35,6 +35,7 @@ GRID = {
     "speculate": [1, 2, 3, 4, 8],
     "use_graph": [0, 1],
     "prev_shift": [0, 1, 2],
+    "bootstrap": [0, 1],
 }
 
 

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpaddi_hip.so")
 
-HIP_SOURCES = ["kernels.hip", "batched.hip", "mlp.hip", "engine.cpp", "learned.cpp"]
+HIP_SOURCES = ["kernels.hip", "batched.hip", "mlp.hip", "engine.cpp", "learned.cpp"]  # the library
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall",
     # packed FP32 VALU ops are disabled on purpose (hardware notes)
@@ -45,6 +45,21 @@ def build_hip(verbose=False):
     return LIB
 
 
+CLI = os.path.join(HERE, "paddi_main")
+
+
+def build_cli():
+    """The command-line drop-in (csrc/paddi_main.cpp), linked against the library."""
+    tmp = CLI + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "paddi_main.cpp"),
+           "-L" + HERE, "-lpaddi_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"CLI build failed:\n{' '.join(cmd)}\n{r.stdout}{r.stderr}")
+    os.replace(tmp, CLI)
+    return CLI
+
+
 def build_oracle():
     d = os.path.join(ROOT, "oracle")
     r = subprocess.run(["make", "-s", "-C", d, "-B"], capture_output=True, text=True)
@@ -54,6 +69,7 @@ def build_oracle():
 
 def build_all(verbose=False):
     build_hip(verbose)
+    build_cli()
     build_oracle()
 
 

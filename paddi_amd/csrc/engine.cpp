@@ -399,17 +399,15 @@ void Engine::enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t ro
     HIPC(hipGetLastError());
 }
 
-// One (m, r2_s) pass: survivors per series.  Series with r2 < 0 are skipped (no rows).
-void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
-                  std::vector<std::vector<Cand>> &out, paddi_stats *st)
+// Buffers and the parameter block of a pass (pinned, copied by enqueue_pass): the pass's
+// series table (r2 < 0 switches a series off) and scan arguments.  WA = select window;
+// refine = false leaves every refine round empty.
+ScanPtrs Engine::fill_pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
+                           int64_t WA, bool refine, size_t &parb)
 {
     const double delta = delta_ed2(m);
     const int64_t rows = B.rows, nwords = B.nblocks;
-    // a graph is shared by the lengths of a 64-wide class, so it carries the class's LDS
-    const size_t lds = use_graph_ ? scan_lds_bytes((int)std::min<int64_t>(round_up(m, 64), MAXM))
-                                  : scan_lds_bytes(m);
-    const size_t parb = sizeof(PassDev) + (size_t)B.nser * sizeof(SerDev);
-    const int64_t scap = std::min<int64_t>(SCAP, rows);
+    parb = sizeof(PassDev) + (size_t)B.nser * sizeof(SerDev);
     alive_.reserve((size_t)nwords * 8);
     best_.reserve((size_t)rows * 4);
     blist_.reserve((size_t)nwords * 4);
@@ -419,7 +417,6 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
     par_.reserve(parb);
     ensure_pinned(parb, 0);
 
-    // parameters: the pass's series table (N = 0 switches a series off) and scan arguments
     PassDev &P = *(PassDev *)pin_par_;
     SerDev *S = (SerDev *)((char *)pin_par_ + sizeof(PassDev));
     int64_t nmax = 0;
@@ -441,7 +438,7 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
     Q.pairs = pout_.as<PassCtl>()->pairs;
     ScanScal &A = P.S;
     A = ScanScal{};
-    A.WA = select_window_;
+    A.WA = WA;
     A.U0 = round_up(m + 64, 64);
     A.m = m;
     A.witness_max = witness_tiles_;
@@ -451,13 +448,116 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
     // between rounds the live blocks are listed again, so far columns are scanned only for
     // the few blocks the near ones did not kill.
     RefineArgs &R = P.R;
-    const int64_t end = nmax + A.U0;
+    const int64_t end = refine ? nmax + A.U0 : A.U0 + A.WA;
     const int64_t raw[4] = {A.U0 + A.WA, A.U0 + A.WA + 2 * refine_chunk_,
                             A.U0 + A.WA + 8 * refine_chunk_, end};
     for (int r = 0; r < 4; r++) R.bound[r] = std::min(raw[r], end);
     R.min_chunk = min_chunk_;
     R.max_chunk = refine_chunk_;
     R.items = refine_waves_;
+    return Q;
+}
+
+// Candidates for a first-length range: a select-only pass at r2 = 0 (nothing is pruned)
+// over a window of BOUND_WINDOW columns gives every row of the series in `need` an upper
+// bound on its NN distance; each 64-row block offers its row with the largest bound, and the
+// `count` best blocks of each series are returned (local rows, best first).
+void Engine::bound_candidates(Batch &B, int m, const std::vector<char> &need, int count,
+                              std::vector<std::vector<int64_t>> &cand, paddi_stats *st)
+{
+    std::vector<double> r2((size_t)B.nser, -1.0);
+    for (int q = 0; q < B.nser; q++)
+        if (need[(size_t)q]) r2[(size_t)q] = 0.0;
+    size_t parb = 0;
+    const ScanPtrs Q = fill_pass(B, m, r2, false, BOUND_WINDOW, false, parb);
+    const int64_t rows = B.rows, nwords = B.nblocks;
+    const size_t lds = scan_lds_bytes(m);
+    bkey_.reserve((size_t)nwords * 8);
+    const PassDev *P = par_.as<PassDev>();
+    PassCtl *ctl = pout_.as<PassCtl>();
+    HIPC(hipMemcpyAsync(par_.p, pin_par_, parb, hipMemcpyHostToDevice, stream_));
+    hipLaunchKernelGGL(k_pass_init, dim3(grid1(std::max<int64_t>(rows, 64), 256)), dim3(256), 0,
+                       stream_, Q, P, rows, nwords, ctl);
+    HIPC(hipEventRecord(ev_[0], stream_));
+    launch_scan_select(Q, P, (unsigned)nwords, lds, stream_);
+    HIPC(hipEventRecord(ev_[1], stream_));
+    hipLaunchKernelGGL(k_block_best, dim3(grid1(nwords * 64, 256)), dim3(256), 0, stream_, Q, P,
+                       nwords, bkey_.as<unsigned long long>());
+    HIPC(hipGetLastError());
+    std::vector<unsigned long long> keys((size_t)nwords);
+    HIPC(hipMemcpyAsync(keys.data(), bkey_.p, (size_t)nwords * 8, hipMemcpyDeviceToHost, stream_));
+    HIPC(hipStreamSynchronize(stream_));
+    collect_derive();
+    if (st) {
+        float a = 0.f;
+        HIPC(hipEventElapsedTime(&a, ev_[0], ev_[1]));
+        st->kernel_ms[1] += a;
+        st->launches[1] += 1;
+    }
+    cand.assign((size_t)B.nser, {});
+    for (int q = 0; q < B.nser; q++) {
+        if (!need[(size_t)q]) continue;
+        const int64_t b0 = B.roff[(size_t)q] / 64;
+        const int64_t nb = (B.n[(size_t)q] - m + 1 + 63) / 64;
+        std::vector<unsigned long long> kq(keys.begin() + b0, keys.begin() + b0 + nb);
+        const size_t c = std::min<size_t>((size_t)count, kq.size());
+        std::partial_sort(kq.begin(), kq.begin() + (std::ptrdiff_t)c, kq.end());
+        for (size_t u = 0; u < c; u++)
+            if (kq[u] != ~0ull)
+                cand[(size_t)q].push_back((int64_t)(kq[u] & 0xffffffffull) - B.roff[(size_t)q]);
+    }
+}
+
+// Lower bound of d_k per series from its best-scored candidate rows: their exact NN
+// distances, greedily accepted like the top-k (descending, ties to the lower index, no two
+// within m); the k-th accepted value is a lower bound of d_1 when k = 1 (any row's NN
+// distance is) and the usual estimate otherwise (a failed pass steps below it).
+void Engine::first_bound(Batch &B, int m, int k, const std::vector<char> &need,
+                         std::vector<double> &lb, paddi_stats *st)
+{
+    lb.assign((size_t)B.nser, 0.0);
+    std::vector<std::vector<int64_t>> cand;
+    bound_candidates(B, m, need, BOUND_CANDIDATES, cand, st);
+    std::vector<std::pair<int, int64_t>> req;
+    for (int q = 0; q < B.nser; q++)
+        for (int64_t i : cand[(size_t)q]) req.emplace_back(q, i);
+    if (req.empty()) return;
+    std::vector<double> v;
+    exact(B, m, req, v, st);
+    std::vector<std::vector<std::pair<double, int64_t>>> per((size_t)B.nser);
+    for (size_t u = 0; u < req.size(); u++)
+        if (v[u] < 3.0e38) per[(size_t)req[u].first].emplace_back(v[u], req[u].second);
+    for (int q = 0; q < B.nser; q++) {
+        auto &c = per[(size_t)q];
+        std::sort(c.begin(), c.end(), [](const std::pair<double, int64_t> &a,
+                                         const std::pair<double, int64_t> &b) {
+            return a.first > b.first || (a.first == b.first && a.second < b.second);
+        });
+        std::vector<int64_t> acc;
+        double last = 0.0;
+        for (const auto &e : c) {
+            bool ok = true;
+            for (int64_t a : acc) ok &= std::llabs(e.second - a) >= m;
+            if (!ok) continue;
+            acc.push_back(e.second);
+            last = e.first;
+            if ((int)acc.size() == k) break;
+        }
+        if ((int)acc.size() == k && last > 0.0) lb[(size_t)q] = last * (1.0 - 1e-6);
+    }
+}
+
+// One (m, r2_s) pass: survivors per series.  Series with r2 < 0 are skipped (no rows).
+void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
+                  std::vector<std::vector<Cand>> &out, paddi_stats *st)
+{
+    const int64_t rows = B.rows, nwords = B.nblocks;
+    // a graph is shared by the lengths of a 64-wide class, so it carries the class's LDS
+    const size_t lds = use_graph_ ? scan_lds_bytes((int)std::min<int64_t>(round_up(m, 64), MAXM))
+                                  : scan_lds_bytes(m);
+    const int64_t scap = std::min<int64_t>(SCAP, rows);
+    size_t parb = 0;
+    const ScanPtrs Q = fill_pass(B, m, r2, witness, select_window_, true, parb);
 
     if (use_graph_) {
         const std::vector<uintptr_t> sig = {
@@ -700,6 +800,37 @@ static void emit(std::vector<paddi_discord> &out, const std::vector<Cand> &acc, 
                                     (float)r2});
 }
 
+static double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+void Engine::len_begin(paddi_stats *st)
+{
+    if (!per_len_ || !st) return;
+    collect_derive();
+    len_snap_ = *st;
+    len_t0_ = now_ms();
+}
+
+void Engine::len_end(int j, int m, int discords, paddi_stats *st)
+{
+    if (!per_len_ || !st) return;
+    collect_derive();
+    paddi_length_stats &r = per_len_[j];
+    r.m = m;
+    r.attempts += (int32_t)(st->attempts - len_snap_.attempts);
+    r.survivors += st->survivors - len_snap_.survivors;
+    r.discords += discords;
+    r.ms_derive += st->kernel_ms[0] - len_snap_.kernel_ms[0];
+    r.ms_scan += (st->kernel_ms[1] - len_snap_.kernel_ms[1]) +
+                 (st->kernel_ms[2] - len_snap_.kernel_ms[2]);
+    r.ms_exact += st->kernel_ms[3] - len_snap_.kernel_ms[3];
+    r.ms_total += now_ms() - len_t0_;
+}
+
 // The plain form for series sid: src/main.cu:136-233 (guards as in the oracle).
 void Engine::plain(Batch &B, int sid, int minL, int maxL, int step, int k, const paddi_opts &o,
                    std::vector<paddi_discord> &out, paddi_stats *st)
@@ -713,6 +844,7 @@ void Engine::plain(Batch &B, int sid, int minL, int maxL, int step, int k, const
     wread_valid_ = false;
     for (int j = 0; j < nlen; j++) {
         const int m = minL + j * step;
+        len_begin(st);
         derive(B, m, st);
         double r, S = 0.0;
         if (j == 0) {
@@ -745,6 +877,7 @@ void Engine::plain(Batch &B, int sid, int minL, int maxL, int step, int k, const
             if ((int)acc.size() >= k || r == 0.0) {
                 emit(out, acc, m, sid, r2);
                 dk[(size_t)j] = acc.empty() ? 0.0 : acc.back().v;
+                len_end(j, m, (int)acc.size(), st);
                 break;
             }
             r = next_r_after_fail(j, r, S);
@@ -767,6 +900,7 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
     wread_valid_ = false;
     for (int j = 0; j < nlen; j++) {
         const int m = minL + j * step;
+        len_begin(st);
         derive(B, m, st);
         std::vector<double> r2((size_t)ns, 0.0);
         std::vector<char> from_prev((size_t)ns, 0);
@@ -816,14 +950,28 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
                     from_prev[(size_t)q] = 1;
                 }
         }
-        // first length (or no usable previous result): learned proposal or probe descent
-        std::vector<double> prop((size_t)ns, 0.0);
+        // first length (or no usable previous result): learned proposal, raised to the
+        // candidates' lower bound when that is higher; else the probe descent from 2m
+        std::vector<double> prop((size_t)ns, 0.0), floor2((size_t)ns, 0.0);
+        std::vector<char> need((size_t)ns, 0);
         bool need_prop = false;
-        for (int q = 0; q < ns; q++) need_prop |= !from_prev[(size_t)q];
-        if (need_prop && learned) propose_r2(B, m, k, prop);
+        for (int q = 0; q < ns; q++) {
+            need[(size_t)q] = !from_prev[(size_t)q];
+            need_prop |= !from_prev[(size_t)q];
+        }
+        if (need_prop && learned) {
+            propose_r2(B, m, k, prop);
+            if (bootstrap_) first_bound(B, m, k, need, floor2, st);
+        }
         for (int q = 0; q < ns; q++)
-            if (!from_prev[(size_t)q])
-                r2[(size_t)q] = (learned && prop[(size_t)q] > 0.0) ? prop[(size_t)q] : 2.0 * m;
+            if (!from_prev[(size_t)q]) {
+                double r = 2.0 * m;
+                if (learned) {
+                    const double p = prop[(size_t)q], f = floor2[(size_t)q];
+                    r = p > 0.0 ? std::max(p, f) : (f > 0.0 ? f : r);
+                }
+                r2[(size_t)q] = r;
+            }
         std::vector<char> active((size_t)ns, 1);
         std::vector<int> att((size_t)ns, 0);
         for (;;) {
@@ -855,11 +1003,24 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
                     for (const Cand &c : acc) prev[(size_t)q].push_back(c.i);
                     active[(size_t)q] = 0;
                 } else {
-                    // below the k-th distance the answer is the same: step r down
-                    r2[(size_t)q] *= from_prev[(size_t)q] ? 0.9 : 0.85;
+                    // below the k-th distance the answer is the same: step r down, not
+                    // below the candidates' bound unless that bound itself has failed
+                    const double f = floor2.empty() ? 0.0 : floor2[(size_t)q];
+                    const double nx = r2[(size_t)q] * (from_prev[(size_t)q] ? 0.9 : 0.85);
+                    if (f > 0.0 && r2[(size_t)q] > f) {
+                        r2[(size_t)q] = std::max(nx, f);
+                    } else {
+                        if (f > 0.0) floor2[(size_t)q] = 0.0;
+                        r2[(size_t)q] = nx;
+                    }
                     if (r2[(size_t)q] < 1e-9) r2[(size_t)q] = 0.0;
                 }
             }
+        }
+        if (per_len_) {
+            int nd = 0;
+            for (int q = 0; q < ns; q++) nd += (int)tk[(size_t)q].acc.size();
+            len_end(j, m, nd, st);
         }
         // this length's witnesses become the next length's read side
         wread_valid_ = true;
@@ -952,7 +1113,8 @@ paddi_status paddi_get_tunables(paddi_ctx *ctx, paddi_tunables *out)
     if (!ctx || !out) return PADDI_ERR_INVALID;
     const paddi::Engine &E = *ctx->eng;
     *out = paddi_tunables{E.select_window_, E.refine_chunk_, E.min_chunk_, E.refine_waves_,
-                          E.witness_tiles_, E.spec_, E.use_graph_ ? 1 : 0, E.prev_shift_};
+                          E.witness_tiles_, E.spec_, E.use_graph_ ? 1 : 0, E.prev_shift_,
+                          E.bootstrap_ ? 1 : 0, 0};
     return PADDI_OK;
 }
 
@@ -962,7 +1124,7 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
     if (t->select_window < 64 || t->refine_chunk < 64 || t->min_chunk < 64 ||
         t->min_chunk > t->refine_chunk || t->refine_items < 1 || t->witness_tiles < 0 ||
         t->witness_tiles > 64 || t->speculate < 1 || t->speculate > 64 || t->prev_shift < 0 ||
-        t->prev_shift > 4)
+        t->prev_shift > 4 || t->bootstrap < 0 || t->bootstrap > 1)
         return PADDI_ERR_INVALID;
     paddi::Engine &E = *ctx->eng;
     E.select_window_ = (t->select_window + 63) / 64 * 64;
@@ -973,6 +1135,7 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
     E.spec_ = t->speculate;
     E.use_graph_ = t->use_graph != 0;
     E.prev_shift_ = t->prev_shift;
+    E.bootstrap_ = t->bootstrap != 0;
     return PADDI_OK;
 }
 
@@ -1110,6 +1273,23 @@ paddi_status paddi_discords_batch(paddi_ctx *ctx, const float *const *T, const i
         st = paddi_discords_series(ctx, ss.data(), nseries, minL, maxL, step, top_k, opts, out,
                                    cap, count, stats);
     for (paddi_series *p : ss) paddi_series_free(p);
+    return st;
+}
+
+paddi_status paddi_discords_timed(paddi_ctx *ctx, const float *T, int64_t n, int32_t minL,
+                                  int32_t maxL, int32_t step, int32_t top_k,
+                                  const paddi_opts *opts, paddi_discord *out, int64_t cap,
+                                  int64_t *count, paddi_length_stats *per_length)
+{
+    if (!ctx || !per_length || !valid_lengths(n, minL, maxL, step, top_k))
+        return PADDI_ERR_INVALID;
+    const int nlen = (maxL - minL) / step + 1;
+    std::memset(per_length, 0, (size_t)nlen * sizeof(paddi_length_stats));
+    paddi_stats stats;
+    ctx->eng->per_len_ = per_length;
+    const paddi_status st = paddi_discords(ctx, T, n, minL, maxL, step, top_k, opts, out, cap,
+                                           count, &stats);
+    ctx->eng->per_len_ = nullptr;
     return st;
 }
 

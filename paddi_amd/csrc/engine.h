@@ -19,6 +19,8 @@ const char *status_str(paddi_status s);
 double delta_ed2(int m);
 
 constexpr int64_t SCAP = 8192;   // survivors returned with a pass's counters (more: a 2nd copy)
+constexpr int64_t BOUND_WINDOW = 1024;  // columns behind the first-length candidates' bounds
+constexpr int BOUND_CANDIDATES = 16;    // candidates per series refined for the first r
 
 // Grow-only device buffer.
 struct DevBuf {
@@ -87,6 +89,12 @@ public:
     // One (m, r2_s) pass over every series of the batch; survivors per series.
     void pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
               std::vector<std::vector<Cand>> &out, paddi_stats *st);
+    // First-length lower bound of d_k per series in `need` (0 = none; see engine.cpp).
+    void first_bound(Batch &B, int m, int k, const std::vector<char> &need,
+                     std::vector<double> &lb, paddi_stats *st);
+    // First-length candidates of the series in `need` (see engine.cpp).
+    void bound_candidates(Batch &B, int m, const std::vector<char> &need, int count,
+                          std::vector<std::vector<int64_t>> &cand, paddi_stats *st);
     // Exact NN ED^2 of (series, local row) requests at length m (derive(m) must be current).
     void exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &req,
                std::vector<double> &out, paddi_stats *st);
@@ -105,9 +113,16 @@ public:
     int spec_ = 4;                   // candidates refined per exact round
     bool use_graph_ = false;         // per-length device pipeline as a hipGraph
     int prev_shift_ = 1;             // next r from previous discords at i, i-1 .. i-prev_shift
+    bool bootstrap_ = true;          // learned first length: candidate lower bound on r
     bool debug_ = false;             // per-pass trace on stderr (PADDI_DEBUG=1)
+    paddi_length_stats *per_len_ = nullptr;  // per-length records of the running search
 
 private:
+    // per-length bookkeeping (per_len_): snapshot of the call's statistics at a length's start
+    void len_begin(paddi_stats *st);
+    void len_end(int j, int m, int discords, paddi_stats *st);
+    paddi_stats len_snap_{};
+    double len_t0_ = 0.0;
     void plain(Batch &B, int sid, int minL, int maxL, int step, int k, const paddi_opts &o,
                std::vector<paddi_discord> &out, paddi_stats *st);
     void fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opts &o,
@@ -116,8 +131,9 @@ public:
     void upload_ser(Batch &B);
     // Learned proposal of the squared range per series at length m (0 = no proposal).
     void propose_r2(Batch &B, int m, int k, std::vector<double> &out);
-    // Features of every series at length m (derive(m) must be current); NFEAT per series.
-    void features(Batch &B, int m, std::vector<float> &X);
+    // Features of every series at length m for a top-k search (derive(m) must be current);
+    // NFEAT per series.
+    void features(Batch &B, int m, int k, std::vector<float> &X);
     void mlp_forward(const float *prm, const float *fmu, const float *fsd, const float *X,
                      int Bn, float *Y);
     void mlp_train(float *prm, const float *fmu, const float *fsd, const float *X,
@@ -131,6 +147,8 @@ private:
     void enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t rows,
                       int64_t nwords, int64_t scap, bool timed);
     void ensure_pinned(size_t par_bytes, size_t ser_bytes);
+    ScanPtrs fill_pass(Batch &B, int m, const std::vector<double> &r2, bool witness, int64_t WA,
+                       bool refine, size_t &parb);
     void collect_derive();
     paddi_stats *derive_st_ = nullptr;   // derive events not yet read
     struct GraphEnt {
@@ -157,6 +175,7 @@ private:
     hipEvent_t ser_ev_ = nullptr;
     DevBuf colv_, st64_, alive_, best_, wit_[2], blist_, par_, pout_;
     DevBuf xarg_;                  // exact-refine arguments and results (one block)
+    DevBuf bkey_;                  // per-block candidate keys of bound_candidates
     void *pin_x_ = nullptr;        // its pinned staging
     size_t pin_x_bytes_ = 0;
     int witcur_ = 0;

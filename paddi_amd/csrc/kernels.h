@@ -125,6 +125,11 @@ struct Surv {
 // Pass results as one device block: the counters, then the survivors (one copy to the host).
 constexpr size_t PASS_OUT_HEAD = 64;
 static_assert(sizeof(PassCtl) <= PASS_OUT_HEAD, "PassCtl fits the head of the pass results");
+// Per 64-row block, its row with the lowest maximum correlation found (the largest upper
+// bound on the NN distance): ((key ^ 0x80000000) << 32 | global row), ~0 for blocks without
+// rows; one wave per block.
+__global__ void k_block_best(ScanPtrs Q, const PassDev *__restrict__ P, int64_t nblocks,
+                             unsigned long long *__restrict__ out);
 __global__ void k_survivors(ScanPtrs Q, const PassDev *__restrict__ P, int64_t rows,
                             PassCtl *__restrict__ ctl, Surv *__restrict__ out, int64_t cap);
 // k_exact is a template on the largest group size nu of the launch (1..MAXU).

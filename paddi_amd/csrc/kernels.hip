@@ -742,6 +742,31 @@ __global__ __launch_bounds__(256) void k_list_blocks(const uint64_t *__restrict_
     }
 }
 
+__global__ __launch_bounds__(256) void k_block_best(ScanPtrs Q, const PassDev *__restrict__ P,
+                                                    int64_t nblocks,
+                                                    unsigned long long *__restrict__ out)
+{
+    const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (b >= nblocks) return;
+    const ScanArgs A = load_args(Q, P);
+    const int sid = A.bser[b];
+    unsigned long long key = ~0ull;
+    if (sid >= 0) {
+        const SerDev S = A.ser[sid];
+        const int64_t g = b * 64 + lane;
+        if (g - S.roff < S.N)
+            key = ((unsigned long long)((unsigned)A.best[g] ^ 0x80000000u) << 32) |
+                  (unsigned long long)(unsigned)g;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned long long v = __shfl_xor(key, o, 64);
+        key = v < key ? v : key;
+    }
+    if (lane == 0) out[b] = key;
+}
+
 // Surviving rows -> (global row, FP32 squared distance); FLT_MAX when no valid pair was seen.
 __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__restrict__ P,
                                                    int64_t rows, PassCtl *__restrict__ ctl,

@@ -19,7 +19,7 @@
 namespace paddi {
 
 namespace {
-const char MAGIC[8] = {'P', 'D', 'R', 'M', 'L', 'P', '0', '1'};
+const char MAGIC[8] = {'P', 'D', 'R', 'M', 'L', 'P', '0', '2'};
 
 #define HIPL(expr)                                                                  \
     do {                                                                            \
@@ -87,7 +87,7 @@ bool learned_available()
 }
 
 // Features of every series of the batch at length m (derive(m) must be current).
-void Engine::features(Batch &B, int m, std::vector<float> &X)
+void Engine::features(Batch &B, int m, int k, std::vector<float> &X)
 {
     const int ns = B.nser;
     for (int q = 0; q < ns; q++) B.host[(size_t)q].N = B.n[(size_t)q] - m + 1;
@@ -111,6 +111,8 @@ void Engine::features(Batch &B, int m, std::vector<float> &X)
         r[0] = m / 512.0f;
         r[1] = std::log10(std::max(h[(size_t)ns * 32 + q], 1e-30f)) / 4.0f;
         for (int b = 0; b < 32; b++) r[2 + b] = h[(size_t)q * 32 + b];
+        r[34] = (float)(std::log10((double)B.n[(size_t)q]) / 6.0);
+        r[35] = k / 4.0f;
     }
 }
 
@@ -180,14 +182,13 @@ void Engine::mlp_train(float *prm, const float *fmu, const float *fsd, const flo
 
 void Engine::propose_r2(Batch &B, int m, int k, std::vector<double> &out)
 {
-    (void)k;
     out.assign((size_t)B.nser, 0.0);
     if (!model_.loaded) {
         const char *p = model_path_.empty() ? default_model_path() : model_path_.c_str();
         if (!model_load(model_, p)) return;
     }
     std::vector<float> X, Y((size_t)B.nser);
-    features(B, m, X);
+    features(B, m, k, X);
     mlp_forward(model_.prm, model_.fmu, model_.fsd, X.data(), B.nser, Y.data());
     for (int q = 0; q < B.nser; q++)
         out[(size_t)q] = 2.0 * m * std::exp((double)Y[(size_t)q] - model_.margin);
@@ -230,9 +231,9 @@ paddi_status paddi_mlp_sizes(int32_t *nfeat, int32_t *hidden, int32_t *nparam)
 }
 
 paddi_status paddi_features(paddi_ctx *ctx, paddi_series *const *series, int32_t nseries,
-                            int32_t m, float *out)
+                            int32_t m, int32_t top_k, float *out)
 {
-    if (!ctx || !series || nseries <= 0 || !out || m < 4 || m > paddi::MAXM)
+    if (!ctx || !series || nseries <= 0 || !out || m < 4 || m > paddi::MAXM || top_k < 1)
         return PADDI_ERR_INVALID;
     std::vector<const paddi::Series *> ss;
     for (int32_t q = 0; q < nseries; q++) {
@@ -247,7 +248,7 @@ paddi_status paddi_features(paddi_ctx *ctx, paddi_series *const *series, int32_t
         E.prefix(B);
         E.derive(B, m, nullptr);
         std::vector<float> X;
-        E.features(B, m, X);
+        E.features(B, m, top_k, X);
         std::memcpy(out, X.data(), X.size() * 4);
         return PADDI_OK;
     });

@@ -7,7 +7,7 @@
 
 namespace paddi {
 
-constexpr int NFEAT = 34;                      // m, global std, 32-bin pair histogram
+constexpr int NFEAT = 36;  // m, global std, 32-bin pair histogram, series length, k
 constexpr int HID = 64;                        // hidden width of both layers
 constexpr int NPAIR = 1024;                    // seeded random pairs per (series, length)
 constexpr int NPARAM = NFEAT * HID + HID + HID * HID + HID + HID + 1;

@@ -62,9 +62,10 @@ class Tunables(C.Structure):
     _fields_ = [("select_window", C.c_int64), ("refine_chunk", C.c_int64),
                 ("min_chunk", C.c_int64), ("refine_items", C.c_int64),
                 ("witness_tiles", C.c_int32), ("speculate", C.c_int32),
-                ("use_graph", C.c_int32), ("prev_shift", C.c_int32)]
+                ("use_graph", C.c_int32), ("prev_shift", C.c_int32),
+                ("bootstrap", C.c_int32), ("reserved", C.c_int32)]
     NAMES = ("select_window", "refine_chunk", "min_chunk", "refine_items", "witness_tiles",
-             "speculate", "use_graph", "prev_shift")
+             "speculate", "use_graph", "prev_shift", "bootstrap")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.NAMES}
@@ -119,7 +120,7 @@ def load_library(path: str = LIB_PATH):
     L.paddi_get_tunables.argtypes = [P, C.POINTER(Tunables)]
     L.paddi_set_tunables.argtypes = [P, C.POINTER(Tunables)]
     L.paddi_mlp_sizes.argtypes = [P, P, P]
-    L.paddi_features.argtypes = [P, P, i32, i32, P]
+    L.paddi_features.argtypes = [P, P, i32, i32, i32, P]
     L.paddi_mlp_forward.argtypes = [P, P, P, P, P, i32, P]
     L.paddi_mlp_gradient.argtypes = [P, P, P, P, P, P, i32, P, P]
     L.paddi_mlp_train.argtypes = [P, P, P, P, P, P, i32, i32, C.c_float, C.c_float, P]
@@ -288,11 +289,11 @@ class Context:
 
     # -- learned range proposal ---------------------------------------------------
 
-    def features(self, handles, m):
+    def features(self, handles, m, top_k=1):
         nf, _, _ = mlp_sizes()
         hs = (C.c_void_p * len(handles))(*[h._h.value for h in handles])
         out = np.empty((len(handles), nf), np.float32)
-        self._check(self._L.paddi_features(self._c, hs, len(handles), m, _ptr(out)),
+        self._check(self._L.paddi_features(self._c, hs, len(handles), m, top_k, _ptr(out)),
                     "paddi_features")
         return out
 

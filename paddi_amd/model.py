@@ -1,9 +1,10 @@
 """File format of the learned range-proposal model (read by the HIP library at
 paddi_amd/model/r_mlp.bin unless paddi_opts.model_path names another file).
 
-Layout, little endian: magic b"PDRMLP01"; int32 nfeat, hidden, nparam, 0; float32
-fmu[nfeat], fsd[nfeat], params[nparam], margin.  The network predicts
-t = log(d_k / 2m); the library proposes r2 = 2m * exp(t - margin).
+Layout, little endian: magic b"PDRMLP02"; int32 nfeat, hidden, nparam, 0; float32
+fmu[nfeat], fsd[nfeat], params[nparam], margin.  Features: m/512, log10(sigma)/4, the
+32-bin pair-distance histogram, log10(n)/6, k/4.  The network predicts t = log(d_k / 2m) of
+the k-th discord; the library proposes r2 = 2m * exp(t - margin).
 """
 from __future__ import annotations
 
@@ -11,11 +12,11 @@ import os
 
 import numpy as np
 
-MAGIC = b"PDRMLP01"
+MAGIC = b"PDRMLP02"
 DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "model", "r_mlp.bin")
 
 
-def init_params(nfeat=34, hidden=64, seed=0):
+def init_params(nfeat=36, hidden=64, seed=0):
     """He-initialised flat parameter vector W1 b1 W2 b2 W3 b3 (biases zero)."""
     rng = np.random.default_rng(seed)
     W1 = rng.standard_normal((nfeat, hidden)) * np.sqrt(2.0 / nfeat)
@@ -25,7 +26,7 @@ def init_params(nfeat=34, hidden=64, seed=0):
                            [0.0]]).astype(np.float32)
 
 
-def save(path, fmu, fsd, params, margin, nfeat=34, hidden=64):
+def save(path, fmu, fsd, params, margin, nfeat=36, hidden=64):
     params = np.asarray(params, np.float32)
     os.makedirs(os.path.dirname(path), exist_ok=True)
     with open(path, "wb") as f:

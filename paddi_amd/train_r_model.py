@@ -1,13 +1,14 @@
 """Train the learned range proposal on the GPU and write paddi_amd/model/r_mlp.bin.
 
-    python -m paddi_amd.train_r_model [--series 256] [--n 262144] [--steps 4000]
+    python -m paddi_amd.train_r_model [--series 256] [--steps 4000]
 
 1. Traces: the engine's batched form (model-free) runs over the training series
-   (random walks, ECG-like, AR(2); synth.training_series) at lengths 64..512 step 32 and
-   yields d_3(series, m), the third discord's squared distance.
-2. Features of every (series, m) come from the device (paddi_features).
+   (random walks, ECG-like, AR(2); synth.training_series; lengths log-uniform in
+   [20,000, 524,288]) at m = 64..512 step 32, top-3, and yields d_1, d_2, d_3 (series, m).
+2. Features of every (series, m) come from the device (paddi_features); a sample is
+   (features with k, log(d_k / 2m)) for k = 1, 2, 3.
 3. The MLP is trained on the device with the library's own forward/backward/Adam kernels
-   (paddi_mlp_train) to predict t = log(d_3 / 2m); series 224..255 are held out.
+   (paddi_mlp_train); the last 32 series are held out.
 4. margin = 0.9-quantile of (prediction - t) over all samples, so that about nine proposals
    in ten fall below the k-th distance and succeed in one pass.
 A JSON report (losses, margin, coverage) is written next to the model.
@@ -33,7 +34,8 @@ from paddi_amd.engine import Context  # noqa: E402
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--series", type=int, default=256)
-    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--nmin", type=int, default=20000)
+    ap.add_argument("--nmax", type=int, default=524288)
     ap.add_argument("--minL", type=int, default=64)
     ap.add_argument("--maxL", type=int, default=512)
     ap.add_argument("--step", type=int, default=32)
@@ -46,11 +48,13 @@ def main():
     a = ap.parse_args()
 
     t0 = time.time()
-    series = [synth.training_series(i, a.n) for i in range(a.series)]
+    rng = np.random.default_rng(77)
+    lens = np.exp(rng.uniform(np.log(a.nmin), np.log(a.nmax), a.series)).astype(int)
+    series = [synth.training_series(i, int(lens[i])) for i in range(a.series)]
     ctx = Context(0)
     hs = [ctx.upload(x) for x in series]
-    print(f"[train] {a.series} series x {a.n} points generated in {time.time() - t0:.1f}s",
-          flush=True)
+    print(f"[train] {a.series} series, {int(lens.sum())} points generated in "
+          f"{time.time() - t0:.1f}s", flush=True)
 
     t1 = time.time()
     recs = ctx.discords_series(hs, a.minL, a.maxL, a.step, a.k, mode="batched")
@@ -63,14 +67,15 @@ def main():
     t2 = time.time()
     X, T, S = [], [], []
     for m in lengths:
-        F = ctx.features(hs, m)
+        F = ctx.features(hs, m, top_k=1)
         for q in range(a.series):
-            d = dk.get((q, m), [])
-            if len(d) < a.k:
-                continue
-            X.append(F[q])
-            T.append(np.log(d[a.k - 1] / (2.0 * m)))
-            S.append(q)
+            d = sorted(dk.get((q, m), []), reverse=True)
+            for kk in range(1, min(len(d), a.k) + 1):
+                f = F[q].copy()
+                f[35] = kk / 4.0
+                X.append(f)
+                T.append(np.log(d[kk - 1] / (2.0 * m)))
+                S.append(q)
     X = np.array(X, np.float32)
     T = np.array(T, np.float32)
     S = np.array(S)

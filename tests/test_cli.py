@@ -1,0 +1,75 @@
+"""The command-line drop-in (paddi_amd/paddi_main): argv of src/main.cu:28-33, the discord
+CSV of src/IOdata.hpp:111-137, the times table of src/IOdata.hpp:140-177 and the range
+profile of src/IOdata.hpp:90-108."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from paddi_amd import build, synth
+
+CLI = build.CLI
+
+
+def run(*args):
+    return subprocess.run([CLI, *map(str, args)], capture_output=True, text=True, timeout=120)
+
+
+def test_cli_is_built():
+    assert os.access(CLI, os.X_OK)
+
+
+@pytest.mark.parametrize("args,msg", [
+    ((), "six positional"),
+    (("s.txt", 1000, 64, 60, "d.csv", "t.csv"), "minL <= maxL"),
+    (("s.txt", 1000, 64, 70, "d.csv", "t.csv", "--mode", "fast"), "unknown mode"),
+    (("s.txt", 1000, 64, 70, "d.csv", "t.csv", "--bogus"), "unknown option"),
+    (("/nonexistent/s.txt", 1000, 64, 70, "d.csv", "t.csv"), "cannot read"),
+    (("s.txt", 1000, 64, 70, "d.csv", "t.csv", "--range-profile", "p.csv"), "profile-m"),
+])
+def test_cli_rejects_bad_arguments_before_any_device_call(tmp_path, args, msg):
+    (tmp_path / "s.txt").write_text("1 2 3\n")
+    r = subprocess.run([CLI, *map(str, args)], capture_output=True, text=True, timeout=60,
+                       cwd=tmp_path)
+    assert r.returncode == 2
+    assert msg in r.stderr
+
+
+def test_cli_short_series_is_rejected(tmp_path):
+    (tmp_path / "s.txt").write_text(" ".join(["1.5"] * 50))
+    r = subprocess.run([CLI, "s.txt", "1000", "64", "70", "d.csv", "t.csv"],
+                       capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert r.returncode == 2 and "shorter than maxL" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["learned", "plain"])
+def test_cli_matches_library_and_formats(ctx, tmp_path, mode):
+    x = synth.rw_planted(20000, 61, 2, 64, 14.0)
+    path = tmp_path / "series.txt"
+    np.savetxt(path, x.astype(np.float64), fmt="%.9g")
+    xs = np.loadtxt(path).astype(np.float32)      # what the binary reads
+    r = run(path, len(x) + 100, 64, 67, tmp_path / "d.csv", tmp_path / "t.csv", "--top-k", 2,
+            "--mode", mode, "--range-profile", tmp_path / "p.csv", "--profile-m", 64,
+            "--profile-r2", 60.0)
+    assert r.returncode == 0, r.stderr
+    got = ctx.discords(xs, 64, 67, 1, 2, mode=mode)
+    lines = (tmp_path / "d.csv").read_text().splitlines()
+    assert lines[0] == "m;top-1 discord idx;top-1 discord nnDist"
+    rows = [l.split(";") for l in lines[1:]]
+    assert [(int(a), int(b)) for a, b, _ in rows] == [(g.length, g.index) for g in got]
+    for (_, _, d), g in zip(rows, got):
+        assert float(d) == pytest.approx(g.nn_dist2, rel=1e-5)   # 6 significant digits
+    t = [l.split(";") for l in (tmp_path / "t.csv").read_text().splitlines()]
+    assert len(t[0]) == 14 and t[0][0] == "m" and t[0][-1].startswith("Total time")
+    assert [int(row[0]) for row in t[1:]] == [64, 65, 66, 67]
+    for row in t[1:]:
+        assert len(row) == 14
+        assert int(row[12]) == 2 and int(row[6]) >= 2 and float(row[13]) > 0
+    p = (tmp_path / "p.csv").read_text().strip().split(";")
+    flag, nn = ctx.range_discords(xs, 64, 60.0)
+    assert int(p[0]) == 64 and len(p) == 1 + len(flag)
+    vals = np.array(p[1:], np.float64)
+    assert np.array_equal(vals > -1e38, flag == 1)
+    np.testing.assert_allclose(vals[flag == 1], nn[flag == 1], rtol=1e-5)

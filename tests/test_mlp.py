@@ -98,11 +98,13 @@ def test_gpu_features_match_restatement(ctx):
     xs = [synth.ar2_mix(20000, 5), synth.rw_planted(30000, 6, 1, 64, 16.0)]
     hs = [ctx.upload(x) for x in xs]
     for m in (64, 128):
-        F = ctx.features(hs, m)
+        F = ctx.features(hs, m, top_k=2)
         for q, x in enumerate(xs):
-            f = R.features(x, m, sid=q)
+            f = R.features(x, m, sid=q, k=2)
             assert F[q, 0] == pytest.approx(f[0])
             assert F[q, 1] == pytest.approx(f[1], abs=1e-6)
-            assert F[q, 2:].sum() == pytest.approx(1.0, abs=1e-5)
+            assert F[q, 2:34].sum() == pytest.approx(1.0, abs=1e-5)
             # bins agree except for pairs within rounding of a bin edge
-            assert np.abs(F[q, 2:] - f[2:]).sum() <= 4.0 / R.NPAIR
+            assert np.abs(F[q, 2:34] - f[2:34]).sum() <= 4.0 / R.NPAIR
+            assert F[q, 34] == pytest.approx(f[34], abs=1e-6)  # log10(n) / 6
+            assert F[q, 35] == pytest.approx(f[35])             # k / 4
