@@ -3,6 +3,11 @@
     python3 -m paddi_amd.profiles copy                # 1 GiB device-to-device copy (calibration)
     python3 -m paddi_amd.profiles stats CSV OUT.md CMD  # kernel-trace stats -> markdown table
     python3 -m paddi_amd.profiles traffic OUT.json KERNEL FETCH.csv WRITE.csv COPYF.csv COPYW.csv CMD
+    python3 -m paddi_amd.profiles util OUT.md TRAFFIC.json BUSY.csv CMD
+
+`util` combines the traffic summary with a `rocprofv3 --pmc VALUBusy SALUBusy MfmaUtil` run
+(the derived metrics, per dispatch) into per-kernel HBM GB/s against the 8 TB/s peak and
+duration-weighted VALU / SALU / MFMA busy percentages.
 
 `traffic` turns two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs (KiB per dispatch)
 into HBM bytes per dispatch of KERNEL, each counter scaled by the factor that makes the
@@ -84,6 +89,49 @@ def traffic(out, kernel, fetch_csv, write_csv, copyf_csv, copyw_csv, cmd):
     print("wrote", out, md)
 
 
+HBM_PEAK_GBS = 8000.0
+
+
+def util(out, traffic_json, busy_csv, cmd):
+    with open(traffic_json) as f:
+        tj = json.load(f)
+    busy = defaultdict(lambda: defaultdict(float))
+    dur = defaultdict(lambda: defaultdict(float))
+    calls = defaultdict(int)
+    with open(busy_csv) as f:
+        for r in csv.DictReader(f):
+            name = r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+            d = (float(r["End_Timestamp"]) - float(r["Start_Timestamp"])) * 1e-9
+            c = r["Counter_Name"]
+            busy[name][c] += float(r["Counter_Value"]) * d
+            dur[name][c] += d
+            calls[name] += c == "VALUBusy"
+    with open(out, "w") as f:
+        f.write("# HBM bandwidth and ALU busy per kernel (rocprofv3)\n\nCommands:\n\n```\n"
+                f"{tj['command']}\n{cmd}\n```\n\n")
+        f.write("HBM GB/s = scaled FETCH_SIZE + WRITE_SIZE bytes per dispatch / mean dispatch "
+                "time of the busy run; busy = duration-weighted mean of the derived metrics "
+                "VALUBusy, SALUBusy, MfmaUtil (100 = every CU's unit busy for the whole "
+                "dispatch).\n\n")
+        f.write("| kernel | dispatches | mean ms | HBM MB / dispatch | HBM GB/s | % of 8 TB/s | "
+                "VALUBusy % | SALUBusy % | MfmaUtil % |\n|---|---:|---:|---:|---:|---:|---:|---:|---:|\n")
+        rows = []
+        for k, v in tj["kernels"].items():
+            if k not in dur:
+                continue
+            rows.append((k, v, dur[k].get("VALUBusy", 0.0)))
+        for k, v, t in sorted(rows, key=lambda r: -r[2]):
+            nd = calls[k]
+            mean_s = t / nd if nd else 0.0
+            gbs = v["bytes_per_dispatch"] / mean_s / 1e9 if mean_s > 0 else 0.0
+            b = {c: busy[k][c] / dur[k][c] if dur[k][c] > 0 else 0.0
+                 for c in ("VALUBusy", "SALUBusy", "MfmaUtil")}
+            f.write(f"| {k} | {nd} | {mean_s * 1e3:.3f} | {v['bytes_per_dispatch'] / 1e6:.1f} | "
+                    f"{gbs:.0f} | {100 * gbs / HBM_PEAK_GBS:.1f} | {b['VALUBusy']:.1f} | "
+                    f"{b['SALUBusy']:.1f} | {b['MfmaUtil']:.1f} |\n")
+    print("wrote", out)
+
+
 def stats(path, out, cmd):
     rows = list(csv.DictReader(open(path)))
     with open(out, "w") as f:
@@ -106,5 +154,7 @@ if __name__ == "__main__":
         stats(sys.argv[2], sys.argv[3], sys.argv[4])
     elif what == "traffic":
         traffic(*sys.argv[2:9])
+    elif what == "util":
+        util(*sys.argv[2:6])
     else:
         raise SystemExit(__doc__)
