@@ -343,6 +343,23 @@ __device__ __forceinline__ void super_seeds(const ScanLds &L, int r0)
     for (int r = 0; r < 4; r++) out[16 * r] = acc0[r] + acc1[r];
 }
 
+// The recurrence alone over a tile (no pruning, no maxima): brings every lane's chain to
+// the tile's last column.
+__device__ __forceinline__ void warm_steps(const float4 *tile, RowState &R)
+{
+    float cov = R.cov;
+    const float rdf = R.rdf, rdg = R.rdg;
+#pragma unroll
+    for (int j = 0; j < WAVE; j++) {
+        const float4 cd = tile[j];
+        const int prev = __builtin_amdgcn_update_dpp(__float_as_int(cd.w), __float_as_int(cov),
+                                                     DPP_WAVE_SHR1, 0xF, 0xF, false);
+        cov = fmaf(rdf, cd.y, __int_as_float(prev));
+        cov = fmaf(rdg, cd.x, cov);
+    }
+    R.cov = cov;
+}
+
 // Ring columns [c, c + cnt) from x (both copies), lanes striding by 64.
 __device__ __forceinline__ void ring_fill(const ScanLds &L, const float *__restrict__ x,
                                           int64_t c, int cnt)
@@ -374,28 +391,32 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
     const int m = V.m;
     const int64_t N = V.N;
     const int W = L.W, K4 = L.K4;
-    // Chain initialisation: lane u holds cov(s+u, c0-1), or cov(s+u+1, 0) when c0 == 0
-    // (position 0 has zero recurrence terms, so the first step reproduces it).
-    {
-        const int64_t cI = c0 > 0 ? c0 - 1 : 0;
-        const int64_t rb = c0 > 0 ? s : s + 1;
-        const float muc = V.colv[cI].w;
-        const float qs = stage_centred(L.qi, V.x, cI, m, muc);
-        R.cov = corr_lanes(L, L.qi, V.x, rb, m, qs, V.colv[rb + t].w);
+    // Chain start.  From c0 >= 64 a warm-up tile over columns [c0-64, c0) runs the
+    // recurrence alone: lane 0 takes the seeds, and after its 64 steps lane u holds
+    // cov(s+u, c0-1), derived from the seed of column c0-1-u (lanes u > j still hold
+    // placeholders at step j, never read by a lane that keeps its value).  From c0 == 0,
+    // lane u holds cov(s+u+1, 0) computed directly (position 0 has zero recurrence terms,
+    // so the first step reproduces it).
+    const bool warm = c0 >= WAVE;
+    const int64_t cs = warm ? c0 - WAVE : c0;
+    R.cov = 0.f;
+    if (!warm) {
+        const float qs = stage_centred(L.qi, V.x, 0, m, V.colv[0].w);
+        R.cov = corr_lanes(L, L.qi, V.x, s + 1, m, qs, V.colv[s + 1 + t].w);
     }
     const float qsum = R.qsum;
     const uint64_t word = (uint64_t)(s >> 6);
     // the first window, then the next super-tile's new columns into registers
-    ring_fill(L, V.x, c0, 256 + K4);
+    ring_fill(L, V.x, cs, 256 + K4);
     float nx[4];
-    if (c0 + 256 < c1) {
+    if (cs + 256 < c1) {
 #pragma unroll
-        for (int p = 0; p < 4; p++) nx[p] = V.x[c0 + 256 + K4 + p * WAVE + t];
+        for (int p = 0; p < 4; p++) nx[p] = V.x[cs + 256 + K4 + p * WAVE + t];
     }
-    float4 pc = V.colv[c0 + t];
+    float4 pc = V.colv[cs + t];
     uint64_t pend_lane0 = ~0ull;  // lane 0: the alive word returned by the previous tile
     int par = 0;                  // tile buffer in use
-    for (int64_t c = c0; c < c1; c += 256) {
+    for (int64_t c = cs; c < c1; c += 256) {
         const int r0 = (int)(c % W);
         __syncthreads();  // ring writes of the previous super-tile are visible
         super_seeds(L, r0);
@@ -411,6 +432,10 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
             par ^= 1;
             tile[t] = make_float4(cv.x, cv.y, cv.z, seed);
             __syncthreads();
+            if (ct < c0) {  // the warm-up tile
+                warm_steps(tile, R);
+                continue;
+            }
             R.tiles++;
             const bool masked = (ct - s - 63 <= m && ct - s + 63 >= -m) || (ct + WAVE > N) ||
                                 (s + WAVE > N) || (ct + WAVE > c1);
