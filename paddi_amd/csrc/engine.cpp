@@ -376,7 +376,7 @@ void Engine::enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t ro
     const unsigned grid =
         (unsigned)std::max<int64_t>(refine_waves_ + refine_waves_ / 2, nwords + 64);
     HIPC(hipMemcpyAsync(par_.p, pin_par_, parb, hipMemcpyHostToDevice, stream_));
-    hipLaunchKernelGGL(k_pass_init, dim3(grid1(std::max<int64_t>(rows, 64), 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_pass_init, dim3(grid1(std::max<int64_t>(rows, 1024), 256)), dim3(256), 0,
                        stream_, Q, P, rows, nwords, ctl);
     if (timed) HIPC(hipEventRecord(ev_[0], stream_));
     launch_scan_select(Q, P, (unsigned)nwords, lds, stream_);
@@ -435,7 +435,7 @@ ScanPtrs Engine::fill_pass(Batch &B, int m, const std::vector<double> &r2, bool 
     Q.best = best_.as<int>();
     Q.wit0 = wit_[0].as<int>();
     Q.wit1 = wit_[1].as<int>();
-    Q.pairs = pout_.as<PassCtl>()->pairs;
+    Q.pairs = &pout_.as<PassCtl>()->pairs[0][0];
     ScanScal &A = P.S;
     A = ScanScal{};
     A.WA = WA;
@@ -476,7 +476,7 @@ void Engine::bound_candidates(Batch &B, int m, const std::vector<char> &need, in
     const PassDev *P = par_.as<PassDev>();
     PassCtl *ctl = pout_.as<PassCtl>();
     HIPC(hipMemcpyAsync(par_.p, pin_par_, parb, hipMemcpyHostToDevice, stream_));
-    hipLaunchKernelGGL(k_pass_init, dim3(grid1(std::max<int64_t>(rows, 64), 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_pass_init, dim3(grid1(std::max<int64_t>(rows, 1024), 256)), dim3(256), 0,
                        stream_, Q, P, rows, nwords, ctl);
     HIPC(hipEventRecord(ev_[0], stream_));
     launch_scan_select(Q, P, (unsigned)nwords, lds, stream_);
@@ -592,6 +592,11 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
     HIPC(hipStreamSynchronize(stream_));
     collect_derive();
     const PassCtl C = *(const PassCtl *)pin_out_;
+    double pairs_sel = 0.0, pairs_ref = 0.0;
+    for (int q = 0; q < PAIR_SLOTS; q++) {
+        pairs_sel += (double)C.pairs[0][q];
+        pairs_ref += (double)C.pairs[1][q];
+    }
     const int ns = C.survivors;
     if (debug_) {
         float a = 0.f, b = 0.f;
@@ -602,8 +607,7 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
         std::fprintf(stderr,
                      "[paddi] m=%d select %.2f ms %.3g pairs | refine blocks %d/%d/%d %.2f ms "
                      "%.3g pairs | survivors %d\n",
-                     m, a, (double)C.pairs[0], C.list[0], C.list[1], C.list[2], b,
-                     (double)C.pairs[1], ns);
+                     m, a, pairs_sel, C.list[0], C.list[1], C.list[2], b, pairs_ref, ns);
     }
     out.assign((size_t)B.nser, {});
     if (ns > 0) {
@@ -635,7 +639,7 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
         }
         st->attempts += 1;
         st->survivors += ns;
-        st->pairs += (int64_t)(C.pairs[0] + C.pairs[1]);
+        st->pairs += (int64_t)(pairs_sel + pairs_ref);
     }
 }
 

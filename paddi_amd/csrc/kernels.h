@@ -33,7 +33,7 @@ struct ScanArgs {
     int *best;                // global rows: ord_key of the max correlation
     int *wit;                 // global rows: pruning tile recorded at this length (or null)
     const int *wit_read;      // global rows: tiles recorded at the previous length (or null)
-    unsigned long long *pairs;  // [0] pairs of k_scan_select, [1] of k_scan_refine
+    unsigned long long *pairs;  // [0 .. PAIR_SLOTS) k_scan_select, [PAIR_SLOTS ..) k_scan_refine
     int64_t WA;               // columns right of a block scanned by k_scan_select
     int64_t CH;               // columns per k_scan_refine work item (set per round on device)
     int64_t U0;               // first cyclic offset scanned: round_up(m + 64, 64)
@@ -77,12 +77,15 @@ struct PassDev {
     RefineArgs R;
 };
 
-// Device counters of one pass, cleared by k_pass_init.
+// Device counters of one pass, cleared by k_pass_init.  The pair counts (statistics) are
+// spread over PAIR_SLOTS addresses by workgroup, so the workgroups' final adds do not all
+// meet on one address.
+constexpr int PAIR_SLOTS = 64;
 struct PassCtl {
     int list[4];                  // live blocks listed before refine round r
     int survivors;
     int pad[3];
-    unsigned long long pairs[2];
+    unsigned long long pairs[2][PAIR_SLOTS];  // [0] k_scan_select, [1] k_scan_refine
 };
 constexpr int REFINE_ROUNDS = 3;
 
@@ -123,7 +126,7 @@ struct Surv {
     int32_t pad;
 };
 // Pass results as one device block: the counters, then the survivors (one copy to the host).
-constexpr size_t PASS_OUT_HEAD = 64;
+constexpr size_t PASS_OUT_HEAD = 1536;
 static_assert(sizeof(PassCtl) <= PASS_OUT_HEAD, "PassCtl fits the head of the pass results");
 // Per 64-row block, its row with the lowest maximum correlation found (the largest upper
 // bound on the NN distance): ((key ^ 0x80000000) << 32 | global row), ~0 for blocks without

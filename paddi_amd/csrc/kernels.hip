@@ -566,7 +566,8 @@ __device__ __forceinline__ ScanLds carve_lds(int m)
 __device__ __forceinline__ void count_pairs(const RowState &R,
                                             unsigned long long *__restrict__ pairs)
 {
-    if (threadIdx.x == 0 && R.tiles) atomicAdd(pairs, (unsigned long long)R.tiles * 4096ull);
+    if (threadIdx.x == 0 && R.tiles)
+        atomicAdd(pairs + (blockIdx.x & (PAIR_SLOTS - 1)), (unsigned long long)R.tiles * 4096ull);
 }
 
 // Rebase the batch onto global block gb's series.  Returns the series id (-1: padding).
@@ -668,7 +669,7 @@ __device__ __forceinline__ void refine_item(const ScanArgs &A, int64_t gb, int64
     load_block(L, R, V, s, thr_rho);
     const bool live = scan_cyclic<false>(L, R, V, s, u0, u1, A.U0);
     if (live) flush_best(R, V, s);
-    count_pairs(R, A.pairs + 1);
+    count_pairs(R, A.pairs + PAIR_SLOTS);
 }
 
 // Refine round: work item = (listed block, chunk q of the round's offset range).  The round's
