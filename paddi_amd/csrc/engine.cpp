@@ -704,10 +704,11 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
         goff.push_back(L.blocks);
         launches.push_back(L);
     }
-    // one block: outkey (set to the maximum key) | cand | goff | gfirst | gser, staged in
-    // pinned memory and copied in one piece
+    // one block: outkey (EXACT_SLOTS per candidate, set to the maximum key) | cand | goff |
+    // gfirst | gser, staged in pinned memory and copied in one piece
     const size_t nc = cand.size(), ngx = gfirst.size();
-    const size_t o_cand = nc * 8, o_goff = o_cand + nc * 8, o_gfirst = o_goff + ngx * 8,
+    const size_t nk = nc * EXACT_SLOTS;
+    const size_t o_cand = nk * 8, o_goff = o_cand + nc * 8, o_gfirst = o_goff + ngx * 8,
                  o_gser = o_gfirst + ngx * 4, xbytes = o_gser + ngx * 4;
     if (xbytes > pin_x_bytes_) {
         if (pin_x_) HIPC(hipHostFree(pin_x_));
@@ -717,7 +718,7 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
         pin_x_bytes_ = xbytes * 2;
     }
     char *hx = (char *)pin_x_;
-    std::memset(hx, 0xff, nc * 8);
+    std::memset(hx, 0xff, nk * 8);
     std::memcpy(hx + o_cand, cand.data(), nc * 8);
     std::memcpy(hx + o_goff, goff.data(), ngx * 8);
     std::memcpy(hx + o_gfirst, gfirst.data(), ngx * 4);
@@ -742,9 +743,12 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
         HIPC(hipGetLastError());
     }
     HIPC(hipEventRecord(ev_[7], stream_));
-    HIPC(hipMemcpyAsync(hx, dx, nc * 8, hipMemcpyDeviceToHost, stream_));
+    HIPC(hipMemcpyAsync(hx, dx, nk * 8, hipMemcpyDeviceToHost, stream_));
     HIPC(hipStreamSynchronize(stream_));
-    const unsigned long long *keys = (const unsigned long long *)hx;
+    std::vector<unsigned long long> keys(nc, ~0ull);
+    for (size_t q = 0; q < nc; q++)
+        for (int b = 0; b < EXACT_SLOTS; b++)
+            keys[q] = std::min(keys[q], ((const unsigned long long *)hx)[q * EXACT_SLOTS + b]);
     collect_derive();
     for (size_t q = 0; q < slot.size(); q++) {
         const unsigned long long k = keys[q];

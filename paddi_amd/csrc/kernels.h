@@ -10,6 +10,7 @@ constexpr int MAXM = 2048;     // largest subsequence length the scan kernels ac
 constexpr int MAXU = 8;        // candidates per exact-refine group
 constexpr int EXACT_SUB = 1024;  // columns per k_exact sub-tile (4 per thread)
 constexpr int EXACT_COLS = 4096; // columns per k_exact block (4 sub-tiles)
+constexpr int EXACT_SLOTS = 64;  // result slots per candidate (blocks spread their minima)
 constexpr int64_t XPAD = 2048; // zero padding after each series (floats); k_exact reads up
                                // to n + EXACT_SUB + 3, the scans up to n + 63
 
@@ -97,7 +98,7 @@ struct ExactArgs {
     const int *gfirst;        // group g holds candidates [gfirst[g], gfirst[g+1])
     const int *gser;          // series of group g
     const int64_t *goff;      // first block of group g (prefix sums of 256-column chunks)
-    unsigned long long *outkey;
+    unsigned long long *outkey;  // [candidate][EXACT_SLOTS]: minima by block slot
     int ngroups;
     int m;
 };

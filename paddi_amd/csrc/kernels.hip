@@ -818,7 +818,8 @@ __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__
 // binary search in goff (prefix sums of chunk counts).  The pair value is formed as
 // sum_k (x[i+k]-mu_i)(x[j+k]-mu_j) in ascending k, times (invn_i*invn_j), which is bitwise
 // symmetric in (i, j): mutual nearest neighbours tie exactly and the tie goes to the lower
-// index.  outkey[u] receives the atomic minimum of ord_key64(ED^2) of candidate u.
+// index.  outkey[u][b mod EXACT_SLOTS] receives the atomic minimum of ord_key64(ED^2) of
+// candidate u over blocks b (the host takes the minimum over the slots).
 
 // NU = the group size of the launch (a template parameter, so the inner loop is straight
 // code); a smaller group fills its spare rows with copies of its first candidate and ignores
@@ -952,7 +953,7 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     if (t < nU) {
         unsigned long long k = wmin[t][0];
         for (int q = 1; q < 4; q++) k = wmin[t][q] < k ? wmin[t][q] : k;
-        atomicMin(E.outkey + u0 + t, k);
+        atomicMin(E.outkey + (size_t)(u0 + t) * EXACT_SLOTS + (blockIdx.x & (EXACT_SLOTS - 1)), k);
     }
 }
 
