@@ -971,6 +971,7 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
             propose_r2(B, m, k, prop);
             if (bootstrap_) first_bound(B, m, k, need, floor2, st);
         }
+        const std::vector<double> floor0 = floor2;  // for the trace
         for (int q = 0; q < ns; q++)
             if (!from_prev[(size_t)q]) {
                 double r = 2.0 * m;
@@ -1006,6 +1007,13 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
                 att[(size_t)q]++;
                 const std::vector<Cand> &acc = tk[(size_t)q].acc;
                 if ((int)acc.size() >= k || r2[(size_t)q] <= 0.0) {
+                    if (debug_ && !from_prev[(size_t)q] && q < 8)
+                        std::fprintf(stderr,
+                                     "[paddi] first length m=%d series %d: proposal %.4g bound "
+                                     "%.4g final r2 %.4g d_k %.4g attempts %d\n",
+                                     m, q, prop.empty() ? 0.0 : prop[(size_t)q],
+                                     floor0[(size_t)q], r2[(size_t)q],
+                                     acc.empty() ? 0.0 : acc.back().v, att[(size_t)q]);
                     emit(res[(size_t)q], acc, m, q, r2[(size_t)q]);
                     prev[(size_t)q].clear();
                     for (const Cand &c : acc) prev[(size_t)q].push_back(c.i);
