@@ -153,18 +153,28 @@ bool TopK::advance(std::vector<size_t> &need)
                                   [&](size_t a, size_t b) { return hi(cands[a]) > hi(cands[b]); });
                 need.resize(64);
             }
-            // Speculate: refine the next-ranked candidates in the same launch (they are
-            // usually the next ones decided), up to `spec` per round.
+            // Speculate: refine in the same launch the candidates likely to be accepted next,
+            // up to `spec` per round: by descending upper bound, skipping any within m of T or
+            // of one already taken (those would be dropped as trivial matches).
             if ((int)need.size() < spec) {
                 std::vector<char> in(cands.size(), 0);
                 for (size_t q : need) in[q] = 1;
                 std::vector<size_t> extra;
                 for (size_t q = 0; q < cands.size(); q++)
                     if (!gone[q] && !in[q] && !cands[q].exact) extra.push_back(q);
-                const size_t want = std::min(extra.size(), (size_t)spec - need.size());
-                std::partial_sort(extra.begin(), extra.begin() + want, extra.end(),
-                                  [&](size_t a, size_t b) { return hi(cands[a]) > hi(cands[b]); });
-                need.insert(need.end(), extra.begin(), extra.begin() + want);
+                std::sort(extra.begin(), extra.end(), [&](size_t a, size_t b) {
+                    return hi(cands[a]) > hi(cands[b]) ||
+                           (hi(cands[a]) == hi(cands[b]) && cands[a].i < cands[b].i);
+                });
+                std::vector<int64_t> anchor{cands[(size_t)T].i};
+                for (size_t q : extra) {
+                    if ((int)need.size() >= spec) break;
+                    bool apart = true;
+                    for (int64_t a : anchor) apart &= std::llabs(cands[q].i - a) >= m;
+                    if (!apart) continue;
+                    need.push_back(q);
+                    anchor.push_back(cands[q].i);
+                }
             }
             return true;
         }
