@@ -28,7 +28,7 @@ from paddi_amd.engine import TUNED_PATH, Context, tuned_key  # noqa: E402
 
 GRID = {
     "witness_tiles": [0, 1, 2, 4, 8, 12, 16, 24, 32, 48],
-    "select_window": [512, 1024, 2048, 4096, 8192, 16384],
+    "select_window": [256, 512, 1024, 2048, 4096, 8192, 16384],
     "refine_items": [4096, 8192, 16384, 32768, 65536],
     "min_chunk": [512, 1024, 2048, 4096, 8192],
     "refine_chunk": [16384, 32768, 65536, 131072],
