@@ -108,3 +108,11 @@ def test_gpu_features_match_restatement(ctx):
             assert np.abs(F[q, 2:34] - f[2:34]).sum() <= 4.0 / R.NPAIR
             assert F[q, 34] == pytest.approx(f[34], abs=1e-6)  # log10(n) / 6
             assert F[q, 35] == pytest.approx(f[35])             # k / 4
+
+
+def test_pair_generator_is_splitmix64():
+    """The feature pairs come from splitmix64 (Steele, Lea, Flood 2014): the restatement's
+    mixer reproduces the published output stream for state 0."""
+    gamma = 0x9E3779B97F4A7C15
+    outs = [R._mix64((q * gamma) & R.M64) for q in range(3)]
+    assert outs == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
