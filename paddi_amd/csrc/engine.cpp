@@ -203,6 +203,7 @@ Engine::Engine(int device) : device_(device)
     HIPC(hipHostMalloc(&pin_out_, PASS_OUT_HEAD + (size_t)SCAP * sizeof(Surv),
                        hipHostMallocDefault));
     HIPC(hipEventCreateWithFlags(&ser_ev_, hipEventDisableTiming));
+    HIPC(hipEventCreateWithFlags(&sync_ev_, hipEventDisableTiming));
     const char *dbg = std::getenv("PADDI_DEBUG");
     debug_ = dbg && dbg[0] == '1';
 }
@@ -214,6 +215,7 @@ Engine::~Engine()
     drop_graphs();
     for (auto &e : ev_) (void)hipEventDestroy(e);
     (void)hipEventDestroy(ser_ev_);
+    (void)hipEventDestroy(sync_ev_);
     for (void *p : {pin_, pin_par_, pin_out_, pin_ser_, pin_x_})
         if (p) (void)hipHostFree(p);
     (void)hipStreamDestroy(stream_);
@@ -317,6 +319,19 @@ void Engine::build_batch(Batch &B, const std::vector<const Series *> &ss, int mi
     HIPC(hipStreamSynchronize(stream_));  // host vectors above must outlive the copies
 }
 
+// Wait for everything queued on stream_: the engine's round trips are short, so the host
+// spins on an event instead of sleeping in hipStreamSynchronize (whose wake-up costs tens of
+// microseconds per round trip).
+void Engine::wait()
+{
+    HIPC(hipEventRecord(sync_ev_, stream_));
+    for (;;) {
+        const hipError_t e = hipEventQuery(sync_ev_);
+        if (e == hipSuccess) return;
+        if (e != hipErrorNotReady) HIPC(e);
+    }
+}
+
 // Series table (N at the current length) to the device, ordered on stream_ before the
 // kernels that follow; the host array may change right after the call.
 void Engine::upload_ser(Batch &B)
@@ -340,7 +355,6 @@ void Engine::prefix(Batch &B)
 void Engine::derive(Batch &B, int m, paddi_stats *st)
 {
     for (int q = 0; q < B.nser; q++) B.host[(size_t)q].N = B.n[(size_t)q] - m + 1;
-    upload_ser(B);
     colv_.reserve((size_t)B.rows * sizeof(float4));
     st64_.reserve((size_t)B.rows * sizeof(double2));
     HIPC(hipEventRecord(ev_[4], stream_));
@@ -496,7 +510,7 @@ void Engine::bound_candidates(Batch &B, int m, const std::vector<char> &need, in
     HIPC(hipGetLastError());
     std::vector<unsigned long long> keys((size_t)nwords);
     HIPC(hipMemcpyAsync(keys.data(), bkey_.p, (size_t)nwords * 8, hipMemcpyDeviceToHost, stream_));
-    HIPC(hipStreamSynchronize(stream_));
+    wait();
     collect_derive();
     if (st) {
         float a = 0.f;
@@ -599,7 +613,7 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
     } else {
         enqueue_pass(Q, lds, parb, rows, nwords, scap, true);
     }
-    HIPC(hipStreamSynchronize(stream_));
+    wait();
     collect_derive();
     const PassCtl C = *(const PassCtl *)pin_out_;
     double pairs_sel = 0.0, pairs_ref = 0.0;
@@ -754,7 +768,7 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
     }
     HIPC(hipEventRecord(ev_[7], stream_));
     HIPC(hipMemcpyAsync(hx, dx, nk * 8, hipMemcpyDeviceToHost, stream_));
-    HIPC(hipStreamSynchronize(stream_));
+    wait();
     std::vector<unsigned long long> keys(nc, ~0ull);
     for (size_t q = 0; q < nc; q++)
         for (int b = 0; b < EXACT_SLOTS; b++)

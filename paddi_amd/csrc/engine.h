@@ -150,6 +150,7 @@ private:
     ScanPtrs fill_pass(Batch &B, int m, const std::vector<double> &r2, bool witness, int64_t WA,
                        bool refine, size_t &parb);
     void collect_derive();
+    void wait();
     paddi_stats *derive_st_ = nullptr;   // derive events not yet read
     struct GraphEnt {
         std::vector<uintptr_t> sig;   // everything baked into the graph
@@ -173,6 +174,7 @@ private:
     void *pin_par_ = nullptr, *pin_out_ = nullptr, *pin_ser_ = nullptr;
     size_t pin_par_bytes_ = 0, pin_ser_bytes_ = 0;
     hipEvent_t ser_ev_ = nullptr;
+    hipEvent_t sync_ev_ = nullptr;  // wait(): completion of the work queued so far
     DevBuf colv_, st64_, alive_, best_, wit_[2], blist_, par_, pout_;
     DevBuf xarg_;                  // exact-refine arguments and results (one block)
     DevBuf bkey_;                  // per-block candidate keys of bound_candidates

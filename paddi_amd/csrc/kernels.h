@@ -14,7 +14,9 @@ constexpr int EXACT_SLOTS = 64;  // result slots per candidate (blocks spread th
 constexpr int64_t XPAD = 2048; // zero padding after each series (floats); k_exact reads up
                                // to n + EXACT_SUB + 3, the scans up to n + 63
 
-// One series of a batch, as the kernels see it (rebuilt per length: N and thr change).
+// One series of a batch, as the kernels see it.  The batch's table is uploaded once (k_derive,
+// k_exact and the feature kernels take N = n - m + 1 from n and m); a pass carries its own
+// copy with N and thr of that pass (N = 0 switches a series off).
 struct SerDev {
     int64_t xoff;   // offset of x_s in the concatenated series buffer
     int64_t n;      // points

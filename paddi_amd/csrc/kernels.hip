@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
     }
     const SerDev S = ser[sid];
     const int64_t i = g - S.roff;
-    if (i >= S.N) {
+    if (i >= S.n - m + 1) {
         colv[g] = make_float4(0.f, 0.f, 0.f, 0.f);
         st64[g] = make_double2(0.0, 0.0);
         return;
@@ -856,7 +856,7 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     const SerDev S = E.ser[E.gser[grp]];
     const float *x = E.x + S.xoff;
     const double2 *st64 = E.st64 + S.roff;
-    const int64_t N = S.N;
+    const int64_t N = S.n - m + 1;
     const int64_t cblk = (item - E.goff[grp]) * EXACT_COLS;
     for (int u = 0; u < NU; u++) {
         const int64_t i = E.cand[u0 + (u < nU ? u : 0)];

@@ -90,8 +90,6 @@ bool learned_available()
 void Engine::features(Batch &B, int m, int k, std::vector<float> &X)
 {
     const int ns = B.nser;
-    for (int q = 0; q < ns; q++) B.host[(size_t)q].N = B.n[(size_t)q] - m + 1;
-    upload_ser(B);
     feat_.reserve((size_t)ns * 33 * 4);
     float *hist = feat_.as<float>();
     float *gstd = hist + (size_t)ns * 32;

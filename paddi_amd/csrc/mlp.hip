@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_feat_pairs(const float *__restrict__ x,
     const int p = wave % NPAIR;
     if (sid >= nser) return;
     const SerDev S = ser[sid];
-    const int64_t N = S.N;
+    const int64_t N = S.n - m + 1;
     if (N <= 2 * (int64_t)m + 2) return;
     const uint64_t h0 = mix64(seed ^ ((uint64_t)sid << 32) ^ (uint64_t)p);
     const uint64_t h1 = mix64(h0);
