@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpaddi_hip.so")
 
-HIP_SOURCES = ["kernels.hip", "batched.hip", "mlp.hip", "engine.cpp", "learned.cpp"]  # the library
+HIP_SOURCES = ["kernels.hip", "mlp.hip", "engine.cpp", "learned.cpp"]  # the library
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall",
     # packed FP32 VALU ops are disabled on purpose (hardware notes)

@@ -1,13 +1,14 @@
 // Host side of the discord engine and its C ABI (include/paddi.h).
 //
-// Per-(m, r) pass over a batch of series (the single-GPU slice of the reference's missing
-// do_PALMAD_one_length, called at src/main.cu:144,176,219):
-//   k_derive        window statistics of length m from the series' prefix sums
+// Per length: k_derive (window statistics of length m from the series' prefix sums), then
+// per (m, r) pass over a batch of series (the single-GPU slice of the reference's missing
+// do_PALMAD_one_length, called at src/main.cu:144,176,219), one host round trip:
+//   k_pass_init     alive bits, row maxima, witness slots, counters
 //   k_scan_select   64-row blocks scan their witness tiles and a bounded window to their right
-//   k_list_blocks   blocks with live rows
-//   k_scan_refine   live blocks x column chunks over all remaining columns
+//   k_list_blocks   blocks with live rows (before each refine round)
+//   k_scan_refine   live blocks x column chunks, three rounds of growing distance
 //   k_survivors     surviving rows with their single-precision NN distance
-//   TopK            exact top-k: interval ordering + k_exact (double) where it matters
+// then TopK: exact top-k by interval ordering + k_exact (double) where it matters.
 //
 // Exactness.  The scans prune at ED^2 <= r2 - delta, where delta bounds the error of the
 // single-precision distance, so the surviving set contains every row whose exact NN distance
@@ -20,11 +21,13 @@
 //   plain   each series alone, each length with the r-descent of src/main.cu:136-233.
 //   fast    all series of the call in every launch; per length, r comes from the exact
 //           double-precision NN distances (at the new length) of the previous length's
-//           discords -- a lower bound of the new k-th distance whenever they stay mutually
-//           non-overlapping -- or, for the first length, from the learned proposal (or a
-//           geometric probe descent without a model); blocks first rescan the tiles that
-//           pruned their rows at the previous length.  The result is the same discord set:
-//           the top-k does not depend on r once r is below the k-th distance.
+//           discords and their backward shifts -- a lower bound of the new k-th distance
+//           whenever they stay mutually non-overlapping -- or, for the first length, from
+//           the learned proposal raised to the exact bound of the best-scored candidate
+//           rows (learned form) or a geometric probe descent (batched form); blocks first
+//           rescan the tiles that pruned their rows at the previous length.  The result is
+//           the same discord set: the top-k does not depend on r once r is below the k-th
+//           distance.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -119,7 +119,6 @@ void launch_scan_refine(const ScanPtrs &Q, const PassDev *P, const int *blist, P
                         int round, unsigned grid, size_t lds, hipStream_t st);
 __global__ void k_pass_init(ScanPtrs Q, const PassDev *__restrict__ P, int64_t rows,
                             int64_t nwords, PassCtl *__restrict__ ctl);
-__global__ void k_fill_int(int *__restrict__ p, int64_t n, int v);
 __global__ void k_list_blocks(const uint64_t *__restrict__ alive, int64_t nwords,
                               int *__restrict__ list, int *__restrict__ count);
 // A surviving row: global row and single-precision squared NN distance.

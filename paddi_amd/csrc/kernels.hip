@@ -750,12 +750,6 @@ __global__ __launch_bounds__(256) void k_pass_init(ScanPtrs Q, const PassDev *__
     if (g < (int64_t)(sizeof(PassCtl) / sizeof(int))) reinterpret_cast<int *>(ctl)[g] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_fill_int(int *__restrict__ p, int64_t n, int v)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 __global__ __launch_bounds__(256) void k_list_blocks(const uint64_t *__restrict__ alive,
                                                      int64_t nwords, int *__restrict__ list,
                                                      int *__restrict__ count)
