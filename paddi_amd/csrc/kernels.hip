@@ -10,7 +10,11 @@
 //   k_scan_refine   exhaustive scan of surviving blocks over the remaining columns, split in
 //                   column chunks, early abandoning (replaces gpu_local_discords_refine,
 //                   src/PD3_kernels.cuh:276-528, and closes its left-of-segment gap)
+//   k_pass_init     per-pass state: alive bits, row maxima, witness slots, counters
+//   k_list_blocks   live blocks before each refine round
 //   k_survivors     compaction of surviving rows (replaces sets_processing.hpp:49-113)
+//   k_block_best    per block, the row with the largest NN-distance bound (first-length
+//                   candidate scoring)
 //   k_exact         double-precision nearest-neighbour distance of a few candidates
 //                   (replaces gpu_multigpus_discords_refine, src/PD3_kernels.cuh:665-759)
 //
@@ -31,8 +35,9 @@
 // s+t).  At step j every lane is on the same column c+j, so column terms are wave-uniform
 // (one broadcast LDS read per step) and each lane receives its diagonal predecessor
 // cov(s+t-1, c+j-1) from lane t-1 through DPP wave_shr:1; lane 0 takes a directly computed
-// seed cov(s, c+j).  Row pruning, row maxima and the alive mask are lane-local; column
-// pruning is a ballot.
+// seed cov(s, c+j).  The seeds of 256 columns at a time are one product on the matrix cores
+// (super_seeds).  Row pruning, row maxima and the alive mask are lane-local; column pruning
+// is a ballot.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
