@@ -813,12 +813,15 @@ __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__
 
 // ---------------------------------------------------------------------------------------
 // Exact (FP64) nearest-neighbour squared distance of candidate groups.  A group is up to
-// MAXU candidates of one series; block = (group, 256-column chunk of that series), found by
-// binary search in goff (prefix sums of chunk counts).  The pair value is formed as
-// sum_k (x[i+k]-mu_i)(x[j+k]-mu_j) in ascending k, times (invn_i*invn_j), which is bitwise
-// symmetric in (i, j): mutual nearest neighbours tie exactly and the tie goes to the lower
-// index.  outkey[u][b mod EXACT_SLOTS] receives the atomic minimum of ord_key64(ED^2) of
-// candidate u over blocks b (the host takes the minimum over the slots).
+// MAXU candidates of one series; block = (group, EXACT_COLS-column chunk of that series),
+// found by binary search in goff (prefix sums of chunk counts).  The pair value is the
+// reference's formula in double precision (src/cpu_kernels/cpu_kernels.cpp:55):
+// ED^2 = 2m (1 - (dot - m (mu_i mu_j)) (invn_i invn_j)), dot = sum_k x[i+k] x[j+k] by FMAs in
+// ascending k.  Every product in it is commutative, so the value is bitwise symmetric in
+// (i, j): mutual nearest neighbours tie exactly and the tie goes to the lower index.  (The
+// raw dot cancels against m mu_i mu_j; in double precision the loss is a few digits of 16.)
+// outkey[u][b mod EXACT_SLOTS] receives the atomic minimum of ord_key64(ED^2) of candidate u
+// over blocks b (the host takes the minimum over the slots).
 
 // NU = the group size of the launch (a template parameter, so the inner loop is straight
 // code); a smaller group fills its spare rows with copies of its first candidate and ignores
@@ -827,9 +830,8 @@ __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__
 // Thread t owns the 4 adjacent columns c + 4t + q of a sub-tile and slides a window of 8
 // doubles over x: per 4 steps of k it reads 4 new doubles (two 16-byte LDS reads, laid out
 // 4 doubles per 48 bytes so that the 64 lanes hit distinct banks) and the candidates' next 4
-// values (broadcast), for 4 (1 + NU) FP64 VALU.  Per column the sum still runs in ascending
-// k with one FMA per term; candidate rows are zero-padded to a multiple of 4, which adds
-// exact zeros.
+// values (broadcast), for 4 NU FP64 FMAs.  Per column the sum runs in ascending k with one
+// FMA per term; candidate rows are zero-padded to a multiple of 4, which adds exact zeros.
 __device__ __forceinline__ int xs_at(int p) { return 6 * (p >> 2) + (p & 3); }
 
 template <int NU>
@@ -839,8 +841,8 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     const int m = E.m;
     const int m4 = (m + 3) & ~3;
     double *xs = dsh;                               // padded layout, exact_xs_doubles(m)
-    double *A = dsh + exact_xs_doubles(m);          // [NU][m4] centred candidate windows
-    __shared__ double ainv[MAXU];
+    double *A = dsh + exact_xs_doubles(m);          // [NU][m4] candidate windows
+    __shared__ double ainv[MAXU], amu[MAXU];
     __shared__ unsigned long long wmin[MAXU][4];
     const int t = threadIdx.x;
     int lo = 0, hi = E.ngroups - 1;
@@ -860,8 +862,11 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     for (int u = 0; u < NU; u++) {
         const int64_t i = E.cand[u0 + (u < nU ? u : 0)];
         const double2 st = st64[i];
-        if (t == 0) ainv[u] = st.y;
-        for (int k = t; k < m4; k += 256) A[u * m4 + k] = k < m ? (double)x[i + k] - st.x : 0.0;
+        if (t == 0) {
+            ainv[u] = st.y;
+            amu[u] = st.x;
+        }
+        for (int k = t; k < m4; k += 256) A[u * m4 + k] = k < m ? (double)x[i + k] : 0.0;
     }
     unsigned long long kmin[NU];
 #pragma unroll
@@ -906,15 +911,11 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
                 a[u][0] = p0.x; a[u][1] = p0.y; a[u][2] = p1.x; a[u][3] = p1.y;
             }
 #pragma unroll
-            for (int kk = 0; kk < 4; kk++) {
-                double b[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) b[q] = w[kk + q] - muj[q];
+            for (int kk = 0; kk < 4; kk++)
 #pragma unroll
                 for (int u = 0; u < NU; u++)
 #pragma unroll
-                    for (int q = 0; q < 4; q++) acc[u][q] = fma(a[u][kk], b[q], acc[u][q]);
-            }
+                    for (int q = 0; q < 4; q++) acc[u][q] = fma(a[u][kk], w[kk + q], acc[u][q]);
 #pragma unroll
             for (int q = 0; q < 4; q++) w[q] = w[q + 4];
         }
@@ -928,7 +929,8 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
                 const bool valid = (j < N) && ((j - i >= m - 1) || (i - j >= m + 1));
                 if (valid) {
                     if (ainv[u] > 0.0 && invj[q] > 0.0)
-                        ed = 2.0 * m * (1.0 - acc[u][q] * (ainv[u] * invj[q]));
+                        ed = 2.0 * m * (1.0 - (acc[u][q] - m * (amu[u] * muj[q])) *
+                                                  (ainv[u] * invj[q]));
                     else
                         ed = 2.0 * m;
                 }
