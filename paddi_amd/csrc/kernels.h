@@ -8,11 +8,10 @@ namespace paddi {
 constexpr int PB = 4096;       // prefix-sum block (points); also the largest supported m
 constexpr int MAXM = 2048;     // largest subsequence length the scan kernels accept
 constexpr int MAXU = 8;        // candidates per exact-refine group
-constexpr int EXACT_SUB = 1024;  // columns per k_exact sub-tile (4 per thread)
-constexpr int EXACT_COLS = 4096; // columns per k_exact block (4 sub-tiles)
+constexpr int EXACT_COLS = 4096; // columns per k_exact block (sub-tiles of 256 exact_qc)
 constexpr int EXACT_SLOTS = 64;  // result slots per candidate (blocks spread their minima)
-constexpr int64_t XPAD = 2048; // zero padding after each series (floats); k_exact reads up
-                               // to n + EXACT_SUB + 3, the scans up to n + 63
+constexpr int64_t XPAD = 2048; // zero padding after each series (floats); the scans read
+                               // up to n + 63
 
 // One series of a batch, as the kernels see it.  The batch's table is uploaded once (k_derive,
 // k_exact and the feature kernels take N = n - m + 1 from n and m); a pass carries its own
@@ -154,22 +153,26 @@ inline size_t scan_lds_bytes(int m)
     const int scratch = ((64 + m + 4 + 3) & ~3) + ((m + 3) & ~3);
     return (size_t)(2 * 64 * 4 + 256 + ((K4 + 20 + 3) & ~3) + (ring > scratch ? ring : scratch)) * 4;
 }
-// Doubles of k_exact's staged series window: EXACT_SUB + m4 + 8 values (m4 = m rounded up
-// to 4), 4 values per 6 doubles.
-__host__ __device__ inline int exact_xs_doubles(int m)
+// Columns per thread of k_exact for groups of nU candidates: 8 for one or two candidates, else
+// 4 (the accumulators of 8 columns for more would cut the waves a SIMD holds).
+__host__ __device__ constexpr int exact_qc(int nU) { return nU <= 2 ? 8 : 4; }
+// Doubles of k_exact's staged series window: 256 qc + m4 + 4 values (m4 = m rounded up to
+// 4), qc values per qc + 2 doubles.
+__host__ __device__ inline int exact_xs_doubles(int m, int qc)
 {
-    return 6 * ((EXACT_SUB + ((m + 3) & ~3) + 8) / 4);
+    return (qc + 2) * ((256 * qc + ((m + 3) & ~3) + 4) / qc + 1);
 }
 // LDS bytes of k_exact for groups of nU candidates of length m.
 inline size_t exact_lds_bytes(int m, int nU)
 {
-    return ((size_t)exact_xs_doubles(m) + (size_t)nU * ((m + 3) & ~3)) * 8;
+    return ((size_t)exact_xs_doubles(m, exact_qc(nU)) + (size_t)nU * ((m + 3) & ~3)) * 8;
 }
 // Candidates per exact group that fit 64 KiB of LDS at length m (>= 1 for m <= MAXM).
 inline int exact_max_u(int m)
 {
-    int u = (int)((65536 / 8 - exact_xs_doubles(m)) / ((m + 3) & ~3));
-    return u < 1 ? 1 : (u > MAXU ? MAXU : u);
+    for (int u = MAXU; u > 1; u--)
+        if (exact_lds_bytes(m, u) <= 65536) return u;
+    return 1;
 }
 
 }  // namespace paddi

@@ -827,21 +827,65 @@ __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__
 // code); a smaller group fills its spare rows with copies of its first candidate and ignores
 // their results.
 //
-// Thread t owns the 4 adjacent columns c + 4t + q of a sub-tile and slides a window of 8
-// doubles over x: per 4 steps of k it reads 4 new doubles (two 16-byte LDS reads, laid out
-// 4 doubles per 48 bytes so that the 64 lanes hit distinct banks) and the candidates' next 4
-// values (broadcast), for 4 NU FP64 FMAs.  Per column the sum runs in ascending k with one
-// FMA per term; candidate rows are zero-padded to a multiple of 4, which adds exact zeros.
-__device__ __forceinline__ int xs_at(int p) { return 6 * (p >> 2) + (p & 3); }
+// Thread t owns the QC = exact_qc(NU) adjacent columns c + QC t + q of a sub-tile of 256 QC
+// columns and keeps a window of QC + 4 doubles of x in a register ring: per 4 steps of k it
+// reads 4 new doubles (two 16-byte LDS reads; the staged window holds QC values per QC + 2
+// doubles, so the lanes' reads, QC + 2 doubles apart, fall in distinct banks) and the
+// candidates' next 4 values (broadcast), for 4 QC NU FP64 FMAs.  Value x[c + QC t + j] sits in
+// ring slot j mod (QC + 4); the k loop is unrolled over one turn of the ring, so every slot
+// index is a constant and the window never moves between registers.  Per column the sum runs
+// in ascending k with one FMA per term; candidate rows are zero-padded to a multiple of 4,
+// which adds exact zeros.
+template <int QC>
+__device__ __forceinline__ int xs_at(int p)
+{
+    return (QC + 2) * (int)((unsigned)p / QC) + (int)((unsigned)p % QC);
+}
+
+// One step of 4 k: xt = xs + (QC + 2) t, thread t's group of the staged window; the offset of
+// value QC t + k0 + QC from there is wave-uniform.
+template <int NU, int QC, int SLOT>
+__device__ __forceinline__ void exact_step(const double *xt, const double *A, int m4, int k0,
+                                           double (&w)[QC + 4], double (&acc)[NU][QC])
+{
+    constexpr int WR = QC + 4;
+    {
+        const unsigned p = (unsigned)k0 + QC;
+        const double2 *g = reinterpret_cast<const double2 *>(xt + (QC + 2) * (p / QC) + p % QC);
+        const double2 lo2 = g[0], hi2 = g[1];
+        w[(SLOT + QC) % WR] = lo2.x;
+        w[(SLOT + QC + 1) % WR] = lo2.y;
+        w[(SLOT + QC + 2) % WR] = hi2.x;
+        w[(SLOT + QC + 3) % WR] = hi2.y;
+    }
+    double a[NU][4];
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+        const double2 *ap = reinterpret_cast<const double2 *>(A + u * m4 + k0);
+        const double2 p0 = ap[0], p1 = ap[1];
+        a[u][0] = p0.x; a[u][1] = p0.y; a[u][2] = p1.x; a[u][3] = p1.y;
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++)
+#pragma unroll
+        for (int u = 0; u < NU; u++)
+#pragma unroll
+            for (int q = 0; q < QC; q++)
+                acc[u][q] = fma(a[u][kk], w[(SLOT + kk + q) % WR], acc[u][q]);
+}
 
 template <int NU>
 __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
 {
+    constexpr int QC = exact_qc(NU);
+    constexpr int SUBC = 256 * QC;   // columns per sub-tile
+    constexpr int WR = QC + 4;       // window ring
+    constexpr int UR = WR / 4;       // k steps of 4 per turn of the ring
     extern __shared__ double dsh[];
     const int m = E.m;
     const int m4 = (m + 3) & ~3;
-    double *xs = dsh;                               // padded layout, exact_xs_doubles(m)
-    double *A = dsh + exact_xs_doubles(m);          // [NU][m4] candidate windows
+    double *xs = dsh;                                   // exact_xs_doubles(m, QC)
+    double *A = dsh + exact_xs_doubles(m, QC);          // [NU][m4] candidate windows
     __shared__ double ainv[MAXU], amu[MAXU];
     __shared__ unsigned long long wmin[MAXU][4];
     const int t = threadIdx.x;
@@ -871,60 +915,58 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     unsigned long long kmin[NU];
 #pragma unroll
     for (int u = 0; u < NU; u++) kmin[u] = ~0ull;
-    for (int sub = 0; sub < EXACT_COLS / EXACT_SUB; sub++) {
-        const int64_t c = cblk + sub * EXACT_SUB;
+    const int lim = SUBC + m4 + 4;  // staged values per sub-tile
+    for (int sub = 0; sub < EXACT_COLS / SUBC; sub++) {
+        const int64_t c = cblk + sub * SUBC;
         if (c >= N) break;
+        // Values at or past n only meet columns >= N (ignored) and are staged as zeros.
+        const int avail = (int)(S.n - c < lim ? S.n - c : lim);
         __syncthreads();  // previous sub-tile's readers are done with xs
-        for (int k = t; k < EXACT_SUB + m4 + 4; k += 256) xs[xs_at(k)] = (double)x[c + k];
-        __syncthreads();
-        double muj[4], invj[4];
+        // four loads in flight per thread, then the stores
+        for (int k = t; k < lim; k += 1024) {
+            float v[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int64_t j = c + 4 * t + q;
-            const double2 sj = (j < N) ? st64[j] : make_double2(0.0, 0.0);
-            muj[q] = sj.x;
-            invj[q] = sj.y;
+            for (int p = 0; p < 4; p++) v[p] = (k + 256 * p < avail) ? x[c + k + 256 * p] : 0.f;
+#pragma unroll
+            for (int p = 0; p < 4; p++)
+                if (k + 256 * p < lim) xs[xs_at<QC>(k + 256 * p)] = (double)v[p];
         }
-        double acc[NU][4];
+        __syncthreads();
+        double acc[NU][QC];
 #pragma unroll
         for (int u = 0; u < NU; u++)
 #pragma unroll
-            for (int q = 0; q < 4; q++) acc[u][q] = 0.0;
-        // window w[0..7] = x[c + 4t + k0 .. + 7]
-        double w[8];
-        {
-            const double2 *g = reinterpret_cast<const double2 *>(xs + 6 * t);
-            const double2 lo2 = g[0], hi2 = g[1];
-            w[0] = lo2.x; w[1] = lo2.y; w[2] = hi2.x; w[3] = hi2.y;
+            for (int q = 0; q < QC; q++) acc[u][q] = 0.0;
+        double w[WR];
+#pragma unroll
+        for (int q = 0; q < QC; q += 2) {
+            const double2 v = *reinterpret_cast<const double2 *>(xs + xs_at<QC>(QC * t + q));
+            w[q] = v.x;
+            w[q + 1] = v.y;
         }
-        for (int k0 = 0; k0 < m4; k0 += 4) {
-            {
-                const double2 *g = reinterpret_cast<const double2 *>(xs + 6 * (t + (k0 >> 2) + 1));
-                const double2 lo2 = g[0], hi2 = g[1];
-                w[4] = lo2.x; w[5] = lo2.y; w[6] = hi2.x; w[7] = hi2.y;
-            }
-            double a[NU][4];
+        const double *xt = xs + (QC + 2) * t;
+        int k0 = 0;
+        for (; k0 + 4 * UR <= m4; k0 += 4 * UR) {
+            exact_step<NU, QC, 0>(xt, A, m4, k0, w, acc);
+            exact_step<NU, QC, 4>(xt, A, m4, k0 + 4, w, acc);
+            if (UR == 3) exact_step<NU, QC, 8 % WR>(xt, A, m4, k0 + 8, w, acc);
+        }
+        if (k0 < m4) exact_step<NU, QC, 0>(xt, A, m4, k0, w, acc);
+        if (UR == 3 && k0 + 4 < m4) exact_step<NU, QC, 4>(xt, A, m4, k0 + 4, w, acc);
+        double muj[QC], invj[QC];
 #pragma unroll
-            for (int u = 0; u < NU; u++) {
-                const double2 *ap = reinterpret_cast<const double2 *>(A + u * m4 + k0);
-                const double2 p0 = ap[0], p1 = ap[1];
-                a[u][0] = p0.x; a[u][1] = p0.y; a[u][2] = p1.x; a[u][3] = p1.y;
-            }
-#pragma unroll
-            for (int kk = 0; kk < 4; kk++)
-#pragma unroll
-                for (int u = 0; u < NU; u++)
-#pragma unroll
-                    for (int q = 0; q < 4; q++) acc[u][q] = fma(a[u][kk], w[kk + q], acc[u][q]);
-#pragma unroll
-            for (int q = 0; q < 4; q++) w[q] = w[q + 4];
+        for (int q = 0; q < QC; q++) {
+            const int64_t j = c + QC * t + q;
+            const double2 sj = (j < N) ? st64[j] : make_double2(0.0, 0.0);
+            muj[q] = sj.x;
+            invj[q] = sj.y;
         }
 #pragma unroll
         for (int u = 0; u < NU; u++) {
             const int64_t i = E.cand[u0 + (u < nU ? u : 0)];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int64_t j = c + 4 * t + q;
+            for (int q = 0; q < QC; q++) {
+                const int64_t j = c + QC * t + q;
                 double ed = DBL_MAX;
                 const bool valid = (j < N) && ((j - i >= m - 1) || (i - j >= m + 1));
                 if (valid) {

@@ -138,6 +138,21 @@ def test_degenerate_windows(ctx):
         assert_same_discords(got, expected(ref), f"flat m={m}")
 
 
+@pytest.mark.parametrize("mode", ["plain", "batched", "learned"])
+def test_mutual_neighbours_tie_to_the_lower_index(ctx, mode):
+    """Rows 426 and 2691 (m = 24) are each other's nearest neighbour: the oracle's distances
+    tie exactly and the greedy top-k takes the lower index first.  The engine's double-
+    precision pair value is bitwise symmetric too, so it reports the same tie in the same
+    order (an evaluation that depended on which row is the candidate would order the two by
+    rounding noise)."""
+    x = synth.rw_planted(4000, 196, 2, 48, 10.0)
+    ref = O.palmad(x, 24, 24, 1, 3)
+    assert list(ref[0]["idx"][1:]) == [426, 2691] and ref[0]["d2"][1] == ref[0]["d2"][2]
+    got = ctx.discords(x, 24, 24, 1, 3, mode=mode)
+    assert_same_discords(got, expected(ref), mode)
+    assert got[1].nn_dist2 == got[2].nn_dist2
+
+
 def test_deterministic_replay(ctx):
     x = synth.rw_planted(30000, 60, 3, 64, 16.0)
     a = ctx.discords(x, 60, 66, 1, 3)
