@@ -318,6 +318,109 @@ __device__ __forceinline__ void tile_steps(const float4 *tile, RowState &R, int6
     R.mx = mx;
 }
 
+// The unmasked tile of k_scan_select with the column terms as DPP operands.  tile_steps
+// reads the 16 bytes of column j from LDS in every step, the same address for all 64 lanes.
+// Here lane l reads column 16g + (l mod 16) once per 16 steps, and step j takes column j's
+// terms from lane j mod 16 of each row of 16 lanes as the first operand of an FMA or multiply
+// (DPP row_newbcast: no instruction of its own).  The step's increment
+// a = dg_i df_j + df_i dg_j (+ the seed on lane 0, whose row terms are zero: e0 = 1 there,
+// 0 elsewhere) does not depend on the chain; the chain itself is one add per step,
+// cov_i = cov_{i-1} (shifted in by DPP wave_shr, zero on lane 0) + a.  The increment is summed
+// in another order than tile_steps' two FMAs, a rounding difference of the order of one unit
+// in the last place per step.  This costs one VALU instruction per step more than tile_steps
+// and saves 60 of its 64 LDS reads per tile: k_scan_select is faster with it, k_scan_refine
+// (whose loop is bound by VALU issue) is not and keeps tile_steps.
+// Written as VALU assembly because the compiler does not fold DPP broadcasts into FMAs.
+// Hazards: a DPP operand needs two wait states after a VALU write of it.  A block starts with
+// s_nop 1 (its operands may have been written just before it); within a block the chain
+// value read by the next step's shifted add was written at least three instructions earlier.
+#define PD_CTL(B) " row_newbcast:" #B " row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+#define PD_CHAIN(A, B)                                                                         \
+    "v_mul_f32_dpp " A ", %[cx], %[rdg]" PD_CTL(B)                                               \
+    "v_fmac_f32_dpp " A ", %[cy], %[rdf]" PD_CTL(B)                                              \
+    "v_fmac_f32_dpp " A ", %[cw], %[e0]" PD_CTL(B)                                               \
+    "v_add_f32_dpp %[cov], %[cov], " A " wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+#define PD_STEP(A, J, H, B)                                                                    \
+    PD_CHAIN(A, B)                                                                               \
+    "v_mul_f32_dpp " J ", %[cz], %[cov]" PD_CTL(B)                                               \
+    "v_cmp_le_f32_e64 " H ", %[thr], " J "\n"
+// Four steps (columns b0..b0+3 of the current group of 16): chain value and row maximum in
+// place, and the four hit masks (rows of the step's column that are within range).
+#define PD_QUAD_BODY(B0, B1, B2, B3)                                                           \
+    "s_nop 1\n"                                                                                  \
+    PD_STEP("%[a0]", "%[j0]", "%[h0]", B0)                                                       \
+    PD_STEP("%[a1]", "%[j1]", "%[h1]", B1)                                                       \
+    "v_max3_f32 %[mx], %[mx], %[j0], %[j1]\n"                                                    \
+    PD_STEP("%[a0]", "%[j0]", "%[h2]", B2)                                                       \
+    PD_STEP("%[a1]", "%[j1]", "%[h3]", B3)                                                       \
+    "v_max3_f32 %[mx], %[mx], %[j0], %[j1]\n"
+#define PD_QUAD_IN                                                                             \
+    [cw] "v"(cd.w), [cx] "v"(cd.x), [cy] "v"(cd.y), [cz] "v"(cd.z), [e0] "v"(e0),              \
+        [rdf] "v"(rdf), [rdg] "v"(rdg), [thr] "v"(thr)
+#define PD_QUAD_COLS(B0, B1, B2, B3)                                                           \
+    asm(PD_QUAD_BODY(B0, B1, B2, B3)                                                            \
+        : [cov] "+v"(cov), [mx] "+v"(mx), [a0] "=&v"(a0), [a1] "=&v"(a1), [j0] "=&v"(j0),      \
+          [j1] "=&v"(j1), [h0] "=&s"(h[0]), [h1] "=&s"(h[1]), [h2] "=&s"(h[2]),                \
+          [h3] "=&s"(h[3])                                                                      \
+        : PD_QUAD_IN)
+// Four warm-up steps: the chain only (the next step's increment separates the adds).
+#define PD_WARM4(B0, B1, B2, B3)                                                               \
+    asm("s_nop 1\n" PD_CHAIN("%[a0]", B0) PD_CHAIN("%[a1]", B1) PD_CHAIN("%[a0]", B2)          \
+            PD_CHAIN("%[a1]", B3)                                                               \
+        : [cov] "+v"(cov), [a0] "=&v"(a0), [a1] "=&v"(a1)                                       \
+        : [cw] "v"(cd.w), [cx] "v"(cd.x), [cy] "v"(cd.y), [e0] "v"(e0), [rdf] "v"(rdf),        \
+          [rdg] "v"(rdg))
+
+__device__ __forceinline__ void tile_steps_bc(const float4 *tile, RowState &R, uint64_t &rowhit,
+                                              uint64_t &colhit)
+{
+    const int t = threadIdx.x;
+    float cov = R.cov, mx = R.mx;
+    const float thr = R.thr, rdf = R.rdf, rdg = R.rdg;
+    const float e0 = t == 0 ? 1.f : 0.f;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const float4 cd = tile[16 * g + (t & 15)];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            float a0, a1, j0, j1;
+            uint64_t h[4];
+            if (q == 0) PD_QUAD_COLS(0, 1, 2, 3);
+            if (q == 1) PD_QUAD_COLS(4, 5, 6, 7);
+            if (q == 2) PD_QUAD_COLS(8, 9, 10, 11);
+            if (q == 3) PD_QUAD_COLS(12, 13, 14, 15);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                rowhit |= h[i];
+                colhit |= (h[i] != 0) ? (1ull << (16 * g + 4 * q + i)) : 0ull;
+            }
+        }
+    }
+    R.cov = cov;
+    R.mx = mx;
+}
+
+__device__ __forceinline__ void warm_steps_bc(const float4 *tile, RowState &R)
+{
+    const int t = threadIdx.x;
+    float cov = R.cov;
+    const float rdf = R.rdf, rdg = R.rdg;
+    const float e0 = t == 0 ? 1.f : 0.f;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const float4 cd = tile[16 * g + (t & 15)];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            float a0, a1;
+            if (q == 0) PD_WARM4(0, 1, 2, 3);
+            if (q == 1) PD_WARM4(4, 5, 6, 7);
+            if (q == 2) PD_WARM4(8, 9, 10, 11);
+            if (q == 3) PD_WARM4(12, 13, 14, 15);
+        }
+    }
+    R.cov = cov;
+}
+
 __device__ __forceinline__ int ring_at(int f) { return f + (f >> 4); }
 
 // Seeds of the 256 columns c .. c+255 of a super-tile on the matrix cores, as one
@@ -438,7 +541,10 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
             tile[t] = make_float4(cv.x, cv.y, cv.z, seed);
             __syncthreads();
             if (ct < c0) {  // the warm-up tile
-                warm_steps(tile, R);
+                if (COLS)
+                    warm_steps_bc(tile, R);
+                else
+                    warm_steps(tile, R);
                 continue;
             }
             R.tiles++;
@@ -448,6 +554,8 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
             if (masked) {
                 const int64_t lim = c1 < N ? c1 : N;
                 tile_steps<true, COLS>(tile, R, s, ct, N, lim, m, rowhit, colhit);
+            } else if (COLS) {
+                tile_steps_bc(tile, R, rowhit, colhit);
             } else {
                 tile_steps<false, COLS>(tile, R, s, ct, N, N, m, rowhit, colhit);
             }
