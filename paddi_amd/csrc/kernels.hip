@@ -834,8 +834,9 @@ void launch_scan_refine(const ScanPtrs &Q, const PassDev *P, const int *blist, P
 
 // Start of a pass: alive bits of every global row (1 for real rows, < N of their series;
 // 0 for padding and for series switched off), row maxima to -inf, this length's witness
-// tiles to none, counters to zero.  Fill kernel rather than memset nodes, so that a
-// captured graph replays it faithfully.
+// tiles of the pass's series to none (a series switched off keeps those of its last pass at
+// this length, which the next length reads), counters to zero.  Fill kernel rather than
+// memset nodes, so that a captured graph replays it faithfully.
 __global__ __launch_bounds__(256) void k_pass_init(ScanPtrs Q, const PassDev *__restrict__ P,
                                                    int64_t rows, int64_t nwords,
                                                    PassCtl *__restrict__ ctl)
@@ -845,7 +846,10 @@ __global__ __launch_bounds__(256) void k_pass_init(ScanPtrs Q, const PassDev *__
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g < rows) {
         A.best[g] = (int)0x807fffff;  // ord_key(-inf)
-        if (A.wit) A.wit[g] = -1;
+        if (A.wit) {
+            const int sid = A.bser[g >> 6];
+            if (sid < 0 || A.ser[sid].N > 0) A.wit[g] = -1;
+        }
     }
     if (g < nwords) {
         const int sid = A.bser[g];
