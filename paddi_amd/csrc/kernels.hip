@@ -297,9 +297,11 @@ __device__ __forceinline__ void tile_steps(const float4 *tile, RowState &R, int6
         const bool hit = cj >= thr;
         if (!MASKED) {
             mx = fmaxf(mx, cj);
-            const uint64_t h = __builtin_amdgcn_ballot_w64(hit);
-            rowhit |= h;
-            if (COLS) colhit |= (h != 0) ? (1ull << j) : 0ull;
+            if (COLS) {
+                const uint64_t h = __builtin_amdgcn_ballot_w64(hit);
+                rowhit |= h;
+                colhit |= (h != 0) ? (1ull << j) : 0ull;
+            }
         } else {
             const int d = dbase + j;
             const bool colin = j < colmax;
@@ -314,6 +316,10 @@ __device__ __forceinline__ void tile_steps(const float4 *tile, RowState &R, int6
             }
         }
     }
+    // Without column pruning the steps need not test the rows: a live row's running maximum
+    // is below thr when the tile starts (a hit retires the row), so the row is hit in this
+    // tile iff its maximum has reached thr by the end of it.
+    if (!MASKED && !COLS) rowhit |= __builtin_amdgcn_ballot_w64(mx >= thr);
     R.cov = cov;
     R.mx = mx;
 }
