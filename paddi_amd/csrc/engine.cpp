@@ -471,6 +471,8 @@ ScanPtrs Engine::fill_pass(Batch &B, int m, const std::vector<double> &r2, bool 
     A.witness_max = witness_tiles_;
     A.wsel = witness ? witcur_ : -1;
     A.rsel = (witness && wread_valid_) ? (witcur_ ^ 1) : -1;
+    A.some_off = 0;
+    for (int q = 0; q < B.nser; q++) A.some_off |= r2[(size_t)q] < 0.0;
     // Refine rounds over cyclic offsets [U0 + WA, N + U0) from each block, nearest first;
     // between rounds the live blocks are listed again, so far columns are scanned only for
     // the few blocks the near ones did not kill.

@@ -70,6 +70,7 @@ struct ScanScal {
     int m;
     int witness_max;
     int wsel, rsel;           // witness buffer written / read (0, 1; -1 = none)
+    int some_off;             // 1 when the pass switches some series off
 };
 
 // Everything one pass reads that changes between passes: one host-to-device copy (or one

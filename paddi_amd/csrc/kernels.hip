@@ -853,7 +853,7 @@ __global__ __launch_bounds__(256) void k_pass_init(ScanPtrs Q, const PassDev *__
     if (g < rows) {
         A.best[g] = (int)0x807fffff;  // ord_key(-inf)
         if (A.wit) {
-            const int sid = A.bser[g >> 6];
+            const int sid = P->S.some_off ? A.bser[g >> 6] : -1;
             if (sid < 0 || A.ser[sid].N > 0) A.wit[g] = -1;
         }
     }
