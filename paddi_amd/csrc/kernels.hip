@@ -731,7 +731,9 @@ __device__ __forceinline__ ScanArgs load_args(const ScanPtrs &Q, const PassDev *
     return A;
 }
 
-__global__ __launch_bounds__(64) void k_scan_select(ScanPtrs Q, const PassDev *__restrict__ P)
+// Five waves per SIMD (a 96-register budget; a few bytes spill to scratch) measured faster
+// than the four the unconstrained allocation leaves room for; so for k_scan_refine.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_scan_select(ScanPtrs Q, const PassDev *__restrict__ P)
 {
     const ScanArgs A = load_args(Q, P);
     const int64_t gb = blockIdx.x;
@@ -797,7 +799,7 @@ __device__ __forceinline__ void refine_item(const ScanArgs &A, int64_t gb, int64
 // keeps about R.items work items per round whatever that count is.  The grid is fixed, so a
 // captured graph can replay it: at least 1.5 R.items workgroups and more than the number of
 // blocks; workgroup b takes item b, and the workgroups beyond the round's items exit at once.
-__global__ __launch_bounds__(64) void k_scan_refine(ScanPtrs Q, const PassDev *__restrict__ P,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_scan_refine(ScanPtrs Q, const PassDev *__restrict__ P,
                                                     const int *__restrict__ blist,
                                                     PassCtl *__restrict__ ctl, int round)
 {
