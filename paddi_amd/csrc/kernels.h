@@ -157,16 +157,20 @@ inline size_t scan_lds_bytes(int m)
 // Columns per thread of k_exact for groups of nU candidates: 8 for one or two candidates, else
 // 4 (the accumulators of 8 columns for more would cut the waves a SIMD holds).
 __host__ __device__ constexpr int exact_qc(int nU) { return nU <= 2 ? 8 : 4; }
-// Doubles of k_exact's staged series window: 256 qc + m4 + 4 values (m4 = m rounded up to
-// 4), qc values per qc + 2 doubles.
-__host__ __device__ inline int exact_xs_doubles(int m, int qc)
+// Candidate window length of k_exact, zero-padded: to a multiple of 16 for groups of three
+// (their values enter as DPP broadcasts of 16 lanes), else of 4.
+__host__ __device__ constexpr int exact_mk(int m, int nU) { return nU == 3 ? (m + 15) & ~15 : (m + 3) & ~3; }
+// Doubles of k_exact's staged series window: 256 qc + mk + 4 values, qc values per qc + 2
+// doubles.
+__host__ __device__ inline int exact_xs_doubles(int m, int nU)
 {
-    return (qc + 2) * ((256 * qc + ((m + 3) & ~3) + 4) / qc + 1);
+    const int qc = exact_qc(nU);
+    return (qc + 2) * ((256 * qc + exact_mk(m, nU) + 4) / qc + 1);
 }
 // LDS bytes of k_exact for groups of nU candidates of length m.
 inline size_t exact_lds_bytes(int m, int nU)
 {
-    return ((size_t)exact_xs_doubles(m, exact_qc(nU)) + (size_t)nU * ((m + 3) & ~3)) * 8;
+    return ((size_t)exact_xs_doubles(m, nU) + (size_t)nU * exact_mk(m, nU)) * 8;
 }
 // Candidates per exact group that fit 64 KiB of LDS at length m (>= 1 for m <= MAXM).
 inline int exact_max_u(int m)

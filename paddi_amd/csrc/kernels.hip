@@ -994,6 +994,52 @@ __device__ __forceinline__ void exact_step(const double *xt, const double *A, in
                 acc[u][q] = fma(a[u][kk], w[(SLOT + kk + q) % WR], acc[u][q]);
 }
 
+// Groups of three: per 16 steps of k each lane holds value k16 + (lane mod 16) of every
+// candidate, and step k takes it from lane k mod 16 of each 16-lane row as the first operand
+// of the FMA (DPP row_newbcast, no instruction of its own), so the candidates cost one LDS
+// read per 16 steps instead of a broadcast read per 2 steps.  Written as VALU assembly (the
+// compiler does not fold DPP broadcasts into FMAs); the block starts with s_nop 1, as the
+// broadcast operands need two wait states after a VALU write of them.  Per accumulator the
+// FMAs still run in ascending k.
+#define XF(U, Q, W, K)                                                                         \
+    "v_fmac_f64_dpp %[c" #U #Q "], %[a" #U "], %[w" #W "] row_newbcast:" #K                     \
+    " row_mask:0xf bank_mask:0xf\n"
+#define XK0(U, K) XF(U, 0, 0, K) XF(U, 1, 1, K) XF(U, 2, 2, K) XF(U, 3, 3, K)
+#define XK1(U, K) XF(U, 0, 1, K) XF(U, 1, 2, K) XF(U, 2, 3, K) XF(U, 3, 4, K)
+#define XK2(U, K) XF(U, 0, 2, K) XF(U, 1, 3, K) XF(U, 2, 4, K) XF(U, 3, 5, K)
+#define XK3(U, K) XF(U, 0, 3, K) XF(U, 1, 4, K) XF(U, 2, 5, K) XF(U, 3, 6, K)
+#define XQ3(K0, K1, K2, K3)                                                                    \
+    asm("s_nop 1\n" XK0(0, K0) XK0(1, K0) XK0(2, K0) XK1(0, K1) XK1(1, K1) XK1(2, K1)           \
+            XK2(0, K2) XK2(1, K2) XK2(2, K2) XK3(0, K3) XK3(1, K3) XK3(2, K3)                   \
+        : [c00] "+v"(acc[0][0]), [c01] "+v"(acc[0][1]), [c02] "+v"(acc[0][2]),                 \
+          [c03] "+v"(acc[0][3]), [c10] "+v"(acc[1][0]), [c11] "+v"(acc[1][1]),                 \
+          [c12] "+v"(acc[1][2]), [c13] "+v"(acc[1][3]), [c20] "+v"(acc[2][0]),                 \
+          [c21] "+v"(acc[2][1]), [c22] "+v"(acc[2][2]), [c23] "+v"(acc[2][3])                  \
+        : [a0] "v"(a0), [a1] "v"(a1), [a2] "v"(a2), [w0] "v"(w[(SLOT + 0) % 8]),               \
+          [w1] "v"(w[(SLOT + 1) % 8]), [w2] "v"(w[(SLOT + 2) % 8]), [w3] "v"(w[(SLOT + 3) % 8]), \
+          [w4] "v"(w[(SLOT + 4) % 8]), [w5] "v"(w[(SLOT + 5) % 8]), [w6] "v"(w[(SLOT + 6) % 8]))
+
+// Four steps of k (k0 = k16 + B) for a group of three: the window's 4 new values, then the
+// 48 FMAs with the candidates' values of lanes B .. B + 3.
+template <int SLOT, int B>
+__device__ __forceinline__ void exact_quad3(const double *xt, int k0, double a0, double a1,
+                                            double a2, double (&w)[8], double (&acc)[3][4])
+{
+    {
+        const unsigned p = (unsigned)k0 + 4;
+        const double2 *g = reinterpret_cast<const double2 *>(xt + 6 * (p / 4));
+        const double2 lo2 = g[0], hi2 = g[1];
+        w[(SLOT + 4) % 8] = lo2.x;
+        w[(SLOT + 5) % 8] = lo2.y;
+        w[(SLOT + 6) % 8] = hi2.x;
+        w[(SLOT + 7) % 8] = hi2.y;
+    }
+    if (B == 0) XQ3(0, 1, 2, 3);
+    if (B == 4) XQ3(4, 5, 6, 7);
+    if (B == 8) XQ3(8, 9, 10, 11);
+    if (B == 12) XQ3(12, 13, 14, 15);
+}
+
 template <int NU>
 __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
 {
@@ -1003,9 +1049,9 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     constexpr int UR = WR / 4;       // k steps of 4 per turn of the ring
     extern __shared__ double dsh[];
     const int m = E.m;
-    const int m4 = (m + 3) & ~3;
-    double *xs = dsh;                                   // exact_xs_doubles(m, QC)
-    double *A = dsh + exact_xs_doubles(m, QC);          // [NU][m4] candidate windows
+    const int m4 = exact_mk(m, NU);                     // a multiple of 4 (of 16 for NU = 3)
+    double *xs = dsh;                                   // exact_xs_doubles(m, NU)
+    double *A = dsh + exact_xs_doubles(m, NU);          // [NU][m4] candidate windows
     __shared__ double ainv[MAXU], amu[MAXU];
     __shared__ unsigned long long wmin[MAXU][4];
     const int t = threadIdx.x;
@@ -1065,14 +1111,25 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
             w[q + 1] = v.y;
         }
         const double *xt = xs + (QC + 2) * t;
-        int k0 = 0;
-        for (; k0 + 4 * UR <= m4; k0 += 4 * UR) {
-            exact_step<NU, QC, 0>(xt, A, m4, k0, w, acc);
-            exact_step<NU, QC, 4>(xt, A, m4, k0 + 4, w, acc);
-            if (UR == 3) exact_step<NU, QC, 8 % WR>(xt, A, m4, k0 + 8, w, acc);
+        if constexpr (NU == 3) {
+            for (int k16 = 0; k16 < m4; k16 += 16) {
+                const double a0 = A[k16 + (t & 15)], a1 = A[m4 + k16 + (t & 15)],
+                             a2 = A[2 * m4 + k16 + (t & 15)];
+                exact_quad3<0, 0>(xt, k16, a0, a1, a2, w, acc);
+                exact_quad3<4, 4>(xt, k16 + 4, a0, a1, a2, w, acc);
+                exact_quad3<0, 8>(xt, k16 + 8, a0, a1, a2, w, acc);
+                exact_quad3<4, 12>(xt, k16 + 12, a0, a1, a2, w, acc);
+            }
+        } else {
+            int k0 = 0;
+            for (; k0 + 4 * UR <= m4; k0 += 4 * UR) {
+                exact_step<NU, QC, 0>(xt, A, m4, k0, w, acc);
+                exact_step<NU, QC, 4>(xt, A, m4, k0 + 4, w, acc);
+                if (UR == 3) exact_step<NU, QC, 8 % WR>(xt, A, m4, k0 + 8, w, acc);
+            }
+            if (k0 < m4) exact_step<NU, QC, 0>(xt, A, m4, k0, w, acc);
+            if (UR == 3 && k0 + 4 < m4) exact_step<NU, QC, 4>(xt, A, m4, k0 + 4, w, acc);
         }
-        if (k0 < m4) exact_step<NU, QC, 0>(xt, A, m4, k0, w, acc);
-        if (UR == 3 && k0 + 4 < m4) exact_step<NU, QC, 4>(xt, A, m4, k0 + 4, w, acc);
         double muj[QC], invj[QC];
 #pragma unroll
         for (int q = 0; q < QC; q++) {
