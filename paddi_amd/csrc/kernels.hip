@@ -33,11 +33,12 @@
 //
 // Layout.  One workgroup = one wave64 = one block of 64 consecutive rows (lane t owns row
 // s+t).  At step j every lane is on the same column c+j, so column terms are wave-uniform
-// (one broadcast LDS read per step) and each lane receives its diagonal predecessor
+// (k_scan_refine: one broadcast LDS read per step; k_scan_select: one read per 16 steps, the
+// terms then taken as DPP row broadcasts) and each lane receives its diagonal predecessor
 // cov(s+t-1, c+j-1) from lane t-1 through DPP wave_shr:1; lane 0 takes a directly computed
 // seed cov(s, c+j).  The seeds of 256 columns at a time are one product on the matrix cores
 // (super_seeds).  Row pruning, row maxima and the alive mask are lane-local; column pruning
-// is a ballot.
+// is a ballot.  k_exact takes the candidates of groups of three as DPP row broadcasts too.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
