@@ -172,12 +172,15 @@ inline size_t exact_lds_bytes(int m, int nU)
 {
     return ((size_t)exact_xs_doubles(m, nU) + (size_t)nU * exact_mk(m, nU)) * 8;
 }
-// Candidates per exact group that fit 64 KiB of LDS at length m (>= 1 for m <= MAXM).
+// Candidates per exact group at length m: the largest u such that groups of every size up to u
+// fit 63 KiB of dynamic LDS (64 KiB less the kernel's static arrays; the footprint is not
+// monotone in the group size: groups of three pad the candidates to 16 values); >= 1 for
+// m <= MAXM.
 inline int exact_max_u(int m)
 {
-    for (int u = MAXU; u > 1; u--)
-        if (exact_lds_bytes(m, u) <= 65536) return u;
-    return 1;
+    int u = 1;
+    while (u < MAXU && exact_lds_bytes(m, u + 1) <= 65536 - 1024) u++;
+    return u;
 }
 
 }  // namespace paddi

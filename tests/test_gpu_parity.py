@@ -138,6 +138,16 @@ def test_degenerate_windows(ctx):
         assert_same_discords(got, expected(ref), f"flat m={m}")
 
 
+@pytest.mark.parametrize("mode", ["plain", "learned"])
+def test_long_windows_match_oracle(ctx, mode):
+    """m near 1000: the scans' windows and seed products span more than 1000 values, and the
+    exact refinement's LDS bounds its group size (groups of up to four here)."""
+    x = synth.rw_planted(6000, 77, 2, 1000, 12.0)
+    ref = O.palmad(x, 1016, 1020, 4, 2)
+    got = ctx.discords(x, 1016, 1020, 4, 2, mode=mode)
+    assert_same_discords(got, expected(ref), mode)
+
+
 @pytest.mark.parametrize("mode", ["plain", "batched", "learned"])
 def test_mutual_neighbours_tie_to_the_lower_index(ctx, mode):
     """Rows 426 and 2691 (m = 24) are each other's nearest neighbour: the oracle's distances
