@@ -10,12 +10,14 @@
 //   k_survivors     surviving rows with their single-precision NN distance
 // then TopK: exact top-k by interval ordering + k_exact (double) where it matters.
 //
-// Exactness.  The scans prune at ED^2 <= r2 - delta, where delta bounds the error of the
-// single-precision distance, so the surviving set contains every row whose exact NN distance
-// exceeds r2, and each survivor's single-precision distance a_i is within delta of the exact
-// one.  TopK ranks survivors by exact distance, computing the exact (double) distance only
-// for candidates whose rank the +-delta intervals leave undecided and for every accepted
-// discord (whose reported distance is then the double-precision value).
+// Exactness.  The scans prune at ED^2 <= r2 - delta_c, where delta_c bounds the error of the
+// cell's single-precision distance: delta (the bound on data of uniform scale) times the
+// ratio of the largest window scale its chain crossed to the cell's own (per 64-row block
+// and 64-column tile, k_pass_init), so the surviving set contains every row whose exact NN
+// distance exceeds r2, and each survivor carries bounds alo <= exact <= a.  TopK ranks
+// survivors by exact distance, computing the exact (double) distance only for candidates
+// whose rank those intervals leave undecided and for every accepted discord (whose reported
+// distance is then the double-precision value).
 //
 // Forms.
 //   plain   each series alone, each length with the r-descent of src/main.cu:136-233.
@@ -88,11 +90,19 @@ void DevBuf::release()
     bytes = 0;
 }
 
+// Rounding of a scan bound's conversion to a squared distance d (the bound's last addition
+// and scaling by invn_i in the scan, then 2m (1 - c) in k_survivors, all in single
+// precision): at most 10m + 2|d| units of 2^-24 (|c| <= 1 contributes the 10m), with a
+// factor 4 of room.
+double conv_ed2(int m, double d) { return 4.0 * (10.0 * m + 2.0 * std::fabs(d)) * 6.0e-8; }
+
 double delta_ed2(int m)
 {
-    // Bound on |single - exact| squared distance: correlation error of a seed (m-term FP32
-    // dot product on locally centred data) plus up to 63 recurrence steps, with a wide
-    // safety factor over the largest error measured on the test workloads.
+    // Bound on |single - exact| squared distance where the windows a cell's chain crossed
+    // are of the scale of the cell's own: correlation error of a seed (m-term FP32 dot
+    // product on locally centred data) plus up to 63 recurrence steps, with a wide safety
+    // factor over the largest error measured on the test workloads.  The scans scale it by
+    // the ratio of the largest scale crossed to the cell's (k_pass_init).
     const double rho_err = 4.0e-5 + 2.0e-7 * m;
     return 2.0 * m * rho_err;
 }
@@ -103,8 +113,10 @@ static inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * 
 // ------------------------------------------------------------------------------------------
 // Exact greedy top-k over the survivors of one series.
 //
-// Every survivor c has an interval [lo_c, hi_c] = [a_c - delta, a_c + delta] holding its
-// exact distance (a point once refined).  T = argmax hi; the exact maximum lies in
+// Every survivor c has an interval [lo_c, hi_c] = [alo_c - conv, ahi_c + conv] (conv_ed2)
+// holding its
+// exact distance (a point once refined): the scans' bounds, widened by the rounding of
+// their conversion to distances.  T = argmax hi; the exact maximum lies in
 // U = {c : hi_c >= lo_T}.  Once U is all exact, its maximum (ties: lower index) is the next
 // candidate in exact order; it is accepted when its distance exceeds r2, and every
 // candidate overlapping it (|i - a| < m) is dropped, since all of them rank below it.
@@ -117,7 +129,6 @@ void TopK::reset(std::vector<Cand> &&c, double r2_, int m_, int k_)
     r2 = r2_;
     m = m_;
     k = k_;
-    delta = delta_ed2(m_);
     done = false;
 }
 
@@ -125,10 +136,12 @@ bool TopK::advance(std::vector<size_t> &need)
 {
     need.clear();
     auto lo = [&](const Cand &c) {
-        return c.exact ? c.v : (c.a >= 3.0e38f ? INFINITY : (double)c.a - delta);
+        return c.exact ? c.v
+                       : (c.a >= 3.0e38f ? INFINITY : (double)c.alo - conv_ed2(m, c.alo));
     };
     auto hi = [&](const Cand &c) {
-        return c.exact ? c.v : (c.a >= 3.0e38f ? INFINITY : (double)c.a + delta);
+        return c.exact ? c.v
+                       : (c.a >= 3.0e38f ? INFINITY : (double)c.ahi + conv_ed2(m, c.ahi));
     };
     while (!done) {
         if ((int)acc.size() >= k) { done = true; break; }
@@ -360,10 +373,13 @@ void Engine::derive(Batch &B, int m, paddi_stats *st)
     for (int q = 0; q < B.nser; q++) B.host[(size_t)q].N = B.n[(size_t)q] - m + 1;
     colv_.reserve((size_t)B.rows * sizeof(float4));
     st64_.reserve((size_t)B.rows * sizeof(double2));
+    tmax_.reserve((size_t)B.nblocks * sizeof(float2));
+    rho_.reserve((size_t)B.nblocks * sizeof(float));
     HIPC(hipEventRecord(ev_[4], stream_));
     hipLaunchKernelGGL(k_derive, dim3(grid1(B.rows, 256)), dim3(256), 0, stream_,
                        B.x.as<float>(), B.ser.as<SerDev>(), B.bser.as<int>(), B.P.as<double2>(),
-                       B.refs.as<double>(), B.rows, m, colv_.as<float4>(), st64_.as<double2>());
+                       B.refs.as<double>(), B.rows, m, colv_.as<float4>(), st64_.as<double2>(),
+                       tmax_.as<float2>());
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ev_[5], stream_));
     derive_st_ = st;  // timed at the next synchronisation (collect_derive)
@@ -437,6 +453,8 @@ ScanPtrs Engine::fill_pass(Batch &B, int m, const std::vector<double> &r2, bool 
     parb = sizeof(PassDev) + (size_t)B.nser * sizeof(SerDev);
     alive_.reserve((size_t)nwords * 8);
     best_.reserve((size_t)rows * 4);
+    besth_.reserve((size_t)rows * 4);
+    bestl_.reserve((size_t)rows * 4);
     blist_.reserve((size_t)nwords * 4);
     pout_.reserve(PASS_OUT_HEAD + (size_t)rows * sizeof(Surv));
     wit_[0].reserve((size_t)rows * 4);
@@ -450,7 +468,7 @@ ScanPtrs Engine::fill_pass(Batch &B, int m, const std::vector<double> &r2, bool 
     for (int q = 0; q < B.nser; q++) {
         S[q] = B.host[(size_t)q];
         S[q].N = r2[(size_t)q] < 0.0 ? 0 : B.n[(size_t)q] - m + 1;
-        S[q].thr = (float)(1.0 - (r2[(size_t)q] - delta) / (2.0 * m));
+        S[q].thr = (float)(1.0 - r2[(size_t)q] / (2.0 * m));
         nmax = std::max(nmax, S[q].N);
     }
     ScanPtrs Q{};
@@ -458,8 +476,12 @@ ScanPtrs Engine::fill_pass(Batch &B, int m, const std::vector<double> &r2, bool 
     Q.ser = (const SerDev *)((char *)par_.p + sizeof(PassDev));
     Q.bser = B.bser.as<int>();
     Q.colv = colv_.as<float4>();
+    Q.tmax = tmax_.as<float2>();
+    Q.rho = rho_.as<float>();
     Q.alive = alive_.as<uint64_t>();
     Q.best = best_.as<int>();
+    Q.besth = besth_.as<int>();
+    Q.bestl = bestl_.as<int>();
     Q.wit0 = wit_[0].as<int>();
     Q.wit1 = wit_[1].as<int>();
     Q.pairs = &pout_.as<PassCtl>()->pairs[0][0];
@@ -471,6 +493,7 @@ ScanPtrs Engine::fill_pass(Batch &B, int m, const std::vector<double> &r2, bool 
     A.witness_max = witness_tiles_;
     A.wsel = witness ? witcur_ : -1;
     A.rsel = (witness && wread_valid_) ? (witcur_ ^ 1) : -1;
+    A.dk = (float)(delta / (2.0 * m));
     A.some_off = 0;
     for (int q = 0; q < B.nser; q++) A.some_off |= r2[(size_t)q] < 0.0;
     // Refine rounds over cyclic offsets [U0 + WA, N + U0) from each block, nearest first;
@@ -593,7 +616,8 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
             (uintptr_t)par_.p,   (uintptr_t)pin_par_, (uintptr_t)pin_out_, (uintptr_t)pout_.p,
             (uintptr_t)alive_.p, (uintptr_t)blist_.p,
             (uintptr_t)Q.x,      (uintptr_t)Q.bser,   (uintptr_t)Q.colv,   (uintptr_t)Q.best,
-            (uintptr_t)Q.wit0,   (uintptr_t)Q.wit1,   (uintptr_t)rows,     (uintptr_t)nwords,
+            (uintptr_t)Q.wit0,   (uintptr_t)Q.wit1,   (uintptr_t)Q.besth,  (uintptr_t)Q.bestl,
+            (uintptr_t)Q.tmax,   (uintptr_t)Q.rho,    (uintptr_t)rows,     (uintptr_t)nwords,
             (uintptr_t)parb,     (uintptr_t)scap,     (uintptr_t)refine_waves_};
         GraphEnt &G = graphs_[{0, lds}];
         if (!G.exec || G.sig != sig) {
@@ -653,7 +677,9 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
         for (int q = 0; q < ns; q++) {
             const int64_t g = sv[(size_t)q].g;
             const int sid = B.bser_host[(size_t)(g >> 6)];
-            out[(size_t)sid].push_back(Cand{g - B.roff[(size_t)sid], sv[(size_t)q].d2, 0.0, false});
+            out[(size_t)sid].push_back(
+                Cand{g - B.roff[(size_t)sid], sv[(size_t)q].d2, 0.0, false, sv[(size_t)q].d2lo,
+                     sv[(size_t)q].d2hi});
         }
     }
     if (st) {
@@ -1243,13 +1269,12 @@ paddi_status paddi_range_discords(paddi_ctx *ctx, const float *T, int64_t n, int
         E.pass(B, m, {(double)r2}, false, surv, nullptr);
         std::fill(is_discord, is_discord + N, 0);
         std::fill(nn_dist2, nn_dist2 + N, 3.4028235e38f);
-        const double delta = paddi::delta_ed2(m);
         std::vector<std::pair<int, int64_t>> amb;
         for (const paddi::Cand &c : surv[0]) {
-            if (c.a >= 3.0e38f || (double)c.a > (double)r2 + delta) {
+            if (c.a >= 3.0e38f || (double)c.alo - paddi::conv_ed2(m, c.alo) > (double)r2) {
                 is_discord[c.i] = 1;
                 nn_dist2[c.i] = c.a;
-            } else if ((double)c.a > (double)r2 - delta) {
+            } else if ((double)c.ahi + paddi::conv_ed2(m, c.ahi) > (double)r2) {
                 amb.emplace_back(0, c.i);
             }
         }

@@ -17,6 +17,7 @@ namespace paddi {
 
 const char *status_str(paddi_status s);
 double delta_ed2(int m);
+double conv_ed2(int m, double d);
 
 constexpr int64_t SCAP = 8192;   // survivors returned with a pass's counters (more: a 2nd copy)
 constexpr int64_t BOUND_WINDOW = 1024;  // columns behind the first-length candidates' bounds
@@ -41,12 +42,14 @@ struct Series {
     DevBuf x;  // n floats
 };
 
-// Surviving row: single-precision distance a, exact distance v once computed.
+// Surviving row: single-precision NN distance a with bounds alo <= exact <= ahi (up to
+// conv_ed2), exact distance v once computed.
 struct Cand {
     int64_t i;
     float a;
     double v;
     bool exact;
+    float alo, ahi;
 };
 
 // Exact greedy top-k of one series at one (m, r2) (see engine.cpp).
@@ -54,7 +57,7 @@ struct TopK {
     std::vector<Cand> cands;
     std::vector<char> gone;
     std::vector<Cand> acc;
-    double r2 = 0.0, delta = 0.0;
+    double r2 = 0.0;
     int m = 0, k = 0;
     int spec = 4;      // candidates refined per round (speculative beyond the undecided set)
     bool done = false;
@@ -175,7 +178,8 @@ private:
     size_t pin_par_bytes_ = 0, pin_ser_bytes_ = 0;
     hipEvent_t ser_ev_ = nullptr;
     hipEvent_t sync_ev_ = nullptr;  // wait(): completion of the work queued so far
-    DevBuf colv_, st64_, alive_, best_, wit_[2], blist_, par_, pout_;
+    DevBuf colv_, st64_, alive_, best_, besth_, bestl_, wit_[2], blist_, par_, pout_;
+    DevBuf tmax_, rho_;            // per global block: row scales, tile column factors
     DevBuf xarg_;                  // exact-refine arguments and results (one block)
     DevBuf bkey_;                  // per-block candidate keys of bound_candidates
     void *pin_x_ = nullptr;        // its pinned staging

@@ -22,7 +22,8 @@ struct SerDev {
     int64_t roff;   // first global row (multiple of 64)
     int64_t pboff;  // first prefix block
     int64_t N;      // windows at this length, n - m + 1
-    float thr;      // prune when correlation >= thr (= 1 - (r2 - delta)/(2m))
+    float thr;      // 1 - r2/(2m): prune row i at column j when cov invn_i invn_j reaches it
+                    // plus the cell's error bound (per tile, see ScanScal::dk)
     int32_t pad;
 };
 
@@ -31,8 +32,11 @@ struct ScanArgs {
     const SerDev *ser;
     const int *bser;          // global block -> series (-1 = padding block)
     const float4 *colv;       // global rows: (df, dg, invn, mu) in single precision
+    const float2 *tmax;       // global blocks: (max sqrt(M2), max invn) over the block's rows
+    const float *rho;         // global blocks: column factor of a tile (k_pass_init)
     uint64_t *alive;          // one word per global block
     int *best;                // global rows: ord_key of the max correlation
+    int *besth, *bestl;       // global rows: ord_key of (max correlation +- its error bound)
     int *wit;                 // global rows: pruning tile recorded at this length (or null)
     const int *wit_read;      // global rows: tiles recorded at the previous length (or null)
     unsigned long long *pairs;  // [0 .. PAIR_SLOTS) k_scan_select, [PAIR_SLOTS ..) k_scan_refine
@@ -41,6 +45,7 @@ struct ScanArgs {
     int64_t U0;               // first cyclic offset scanned: round_up(m + 64, 64)
     int m;
     int witness_max;          // witness ranges scanned first
+    float dk;                 // delta(m) / (2m): error bound per unit of block and tile scale
 };
 
 // Refine rounds of one pass: round r scans cyclic offsets [bound[r], bound[r+1]) of the
@@ -58,8 +63,10 @@ struct ScanPtrs {
     const SerDev *ser;        // the pass's series table (N = 0 switches a series off)
     const int *bser;
     const float4 *colv;
+    const float2 *tmax;
+    float *rho;
     uint64_t *alive;
-    int *best;
+    int *best, *besth, *bestl;
     int *wit0, *wit1;         // the two witness buffers (one written, one read per length)
     unsigned long long *pairs;
 };
@@ -71,6 +78,7 @@ struct ScanScal {
     int witness_max;
     int wsel, rsel;           // witness buffer written / read (0, 1; -1 = none)
     int some_off;             // 1 when the pass switches some series off
+    float dk;                 // delta(m) / (2m) (ScanArgs::dk)
 };
 
 // Everything one pass reads that changes between passes: one host-to-device copy (or one
@@ -111,7 +119,8 @@ __global__ void k_prefix(const float *__restrict__ x, const SerDev *__restrict__
 __global__ void k_derive(const float *__restrict__ x, const SerDev *__restrict__ ser,
                          const int *__restrict__ bser, const double2 *__restrict__ P,
                          const double *__restrict__ refs, int64_t rows, int m,
-                         float4 *__restrict__ colv, double2 *__restrict__ st64);
+                         float4 *__restrict__ colv, double2 *__restrict__ st64,
+                         float2 *__restrict__ tmax);
 // The scan kernels: one 64-row block per workgroup (one wave); LDS from scan_lds_bytes(m).
 void launch_scan_select(const ScanPtrs &Q, const PassDev *P, unsigned grid, size_t lds,
                         hipStream_t st);
@@ -121,11 +130,12 @@ __global__ void k_pass_init(ScanPtrs Q, const PassDev *__restrict__ P, int64_t r
                             int64_t nwords, PassCtl *__restrict__ ctl);
 __global__ void k_list_blocks(const uint64_t *__restrict__ alive, int64_t nwords,
                               int *__restrict__ list, int *__restrict__ count);
-// A surviving row: global row and single-precision squared NN distance.
+// A surviving row: global row, single-precision squared NN distance d2 and bounds
+// d2lo <= exact <= d2hi (up to the rounding of their conversion).
 struct Surv {
     int64_t g;
     float d2;
-    int32_t pad;
+    float d2lo, d2hi;
 };
 // Pass results as one device block: the counters, then the survivors (one copy to the host).
 constexpr size_t PASS_OUT_HEAD = 1536;

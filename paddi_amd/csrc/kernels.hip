@@ -143,7 +143,7 @@ __device__ __forceinline__ double2 window_sums(const double2 *__restrict__ P,
 
 __device__ __forceinline__ void window_stats(const double2 *__restrict__ P,
                                              const double *__restrict__ refs, int64_t i, int m,
-                                             double *mu, double *invn)
+                                             double *mu, double *invn, double *m2 = nullptr)
 {
     double cb;
     double2 s = window_sums(P, refs, i, m, &cb);
@@ -152,48 +152,78 @@ __device__ __forceinline__ void window_stats(const double2 *__restrict__ P,
     *mu = cb + s.x / m;
     // zero spread (up to rounding of the sums) = degenerate window
     *invn = (M2 > 1e-13 * (s.y + q) && M2 > 0.0) ? 1.0 / sqrt(M2) : 0.0;
+    if (m2) *m2 = M2 > 0.0 ? M2 : 0.0;
 }
 
-// Per-length terms for every global row; padding rows get zeros.
+// DPP row operation of the wave: lanes of the rows in row_mask read src through ctrl, the
+// others (and lanes without a source) keep their own value.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+
+// Maximum over the 64 lanes (every lane active): pairs, quads, half rows, rows by quad_perm
+// and mirror moves, then rows 1 and 3 take the last lane of rows 0 and 2 (row_bcast:15) and
+// rows 2 and 3 that of row 1 (row_bcast:31), which leaves the total in lane 63.
+__device__ __forceinline__ float wave_max(float v)
+{
+    v = fmaxf(v, dpp_f<0xB1, 0xF>(v));   // quad_perm [1,0,3,2]
+    v = fmaxf(v, dpp_f<0x4E, 0xF>(v));   // quad_perm [2,3,0,1]
+    v = fmaxf(v, dpp_f<0x141, 0xF>(v));  // row_half_mirror
+    v = fmaxf(v, dpp_f<0x140, 0xF>(v));  // row_mirror
+    v = fmaxf(v, dpp_f<0x142, 0xA>(v));  // row_bcast:15
+    v = fmaxf(v, dpp_f<0x143, 0xC>(v));  // row_bcast:31
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// Per-length terms for every global row (padding rows get zeros), and per 64-row block
+// (one wave; rows is a multiple of 64) the largest sqrt(M2) and invn of its rows.
 __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
                                                 const SerDev *__restrict__ ser,
                                                 const int *__restrict__ bser,
                                                 const double2 *__restrict__ P,
                                                 const double *__restrict__ refs, int64_t rows,
                                                 int m, float4 *__restrict__ colv,
-                                                double2 *__restrict__ st64)
+                                                double2 *__restrict__ st64,
+                                                float2 *__restrict__ tmax)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= rows) return;
-    const int sid = bser[g >> 6];
-    if (sid < 0) {
-        colv[g] = make_float4(0.f, 0.f, 0.f, 0.f);
-        st64[g] = make_double2(0.0, 0.0);
-        return;
+    float4 cv = make_float4(0.f, 0.f, 0.f, 0.f);
+    double2 sv = make_double2(0.0, 0.0);
+    float sd = 0.f;
+    const int sid = g < rows ? bser[g >> 6] : -1;
+    if (sid >= 0) {
+        const SerDev S = ser[sid];
+        const int64_t i = g - S.roff;
+        if (i < S.n - m + 1) {
+            const float *xs = x + S.xoff;
+            const double2 *Ps = P + S.pboff * (PB + 1);
+            const double *rs = refs + S.pboff;
+            double mu, invn, M2;
+            window_stats(Ps, rs, i, m, &mu, &invn, &M2);
+            double df = 0.0, dg = 0.0;
+            if (i > 0) {
+                double mup, invp;
+                window_stats(Ps, rs, i - 1, m, &mup, &invp);
+                const double xin = (double)xs[i + m - 1], xout = (double)xs[i - 1];
+                df = 0.5 * (xin - xout);
+                dg = (xin - mu) + (xout - mup);
+            }
+            cv = make_float4((float)df, (float)dg, (float)invn, (float)mu);
+            sv = make_double2(mu, invn);
+            sd = sqrtf((float)M2);
+        }
     }
-    const SerDev S = ser[sid];
-    const int64_t i = g - S.roff;
-    if (i >= S.n - m + 1) {
-        colv[g] = make_float4(0.f, 0.f, 0.f, 0.f);
-        st64[g] = make_double2(0.0, 0.0);
-        return;
+    if (g < rows) {
+        colv[g] = cv;
+        st64[g] = sv;
     }
-    const float *xs = x + S.xoff;
-    const double2 *Ps = P + S.pboff * (PB + 1);
-    const double *rs = refs + S.pboff;
-    double mu, invn;
-    window_stats(Ps, rs, i, m, &mu, &invn);
-    double df = 0.0, dg = 0.0;
-    if (i > 0) {
-        double mup, invp;
-        window_stats(Ps, rs, i - 1, m, &mup, &invp);
-        const double xin = (double)xs[i + m - 1], xout = (double)xs[i - 1];
-        df = 0.5 * (xin - xout);
-        dg = (xin - mu) + (xout - mup);
-    }
-    colv[g] = make_float4((float)df, (float)dg, (float)invn, (float)mu);
-    st64[g] = make_double2(mu, invn);
+    const float a = wave_max(sd), b = wave_max(cv.z);
+    if ((threadIdx.x & 63) == 0 && g < rows) tmax[g >> 6] = make_float2(a, b);
 }
+
 
 // ---------------------------------------------------------------------------------------
 // Scan kernels (all indices below are local to one series).
@@ -254,8 +284,11 @@ __device__ __forceinline__ float stage_centred(float *dst, const float *__restri
 struct RowState {
     float cov;     // chain value of this lane (row s+t) at the last processed column
     float rdf, rdg;// row recurrence terms (0 on lane 0, which takes seeds)
-    float thr;     // prune when cov*invn_j >= thr
+    float thr0;    // (1 - r2/(2m)) / invn_i
+    float thr;     // the current tile's threshold: prune when cov*invn_j >= thr
+    float kb;      // dk * (largest sqrt(M2) of the block's rows), wave-uniform
     float mx;      // max over valid columns of cov*invn_j
+    float mxl, mxh;// max over tiles of (the tile's max of cov*invn_j -+ the tile's bound)
     uint64_t alive;// wave-uniform alive mask of the 64 rows
     uint32_t tiles;// 64x64 tiles evaluated (pair counter)
     float qsum;    // sum of the staged, centred first row (L.qz)
@@ -265,11 +298,14 @@ struct RowState {
 struct View {
     const float *x;
     const float4 *colv;
+    const float2 *tmax;
+    const float *rho;
     uint64_t *alive;
-    int *best;
+    int *best, *besth, *bestl;
     int *wit;
     int64_t N;
     int m;
+    float dk;
 };
 
 // climit: first column not to evaluate (min of the scan end and N); rows at or beyond N
@@ -278,11 +314,11 @@ struct View {
 // far columns it scans belong to blocks that are almost all dead by then).
 template <bool MASKED, bool COLS>
 __device__ __forceinline__ void tile_steps(const float4 *tile, RowState &R, int64_t s, int64_t c,
-                                           int64_t N, int64_t climit, int m, uint64_t &rowhit,
-                                           uint64_t &colhit)
+                                           int64_t N, int64_t climit, int m, float inc,
+                                           uint64_t &rowhit, uint64_t &colhit)
 {
     const int t = threadIdx.x;
-    float cov = R.cov, mx = R.mx;
+    float cov = R.cov, mx = -INFINITY;
     const float thr = R.thr, rdf = R.rdf, rdg = R.rdg;
     const int dbase = (int)(c - s) - t;
     const bool rowin = (s + t) < N;
@@ -317,12 +353,13 @@ __device__ __forceinline__ void tile_steps(const float4 *tile, RowState &R, int6
             }
         }
     }
-    // Without column pruning the steps need not test the rows: a live row's running maximum
-    // is below thr when the tile starts (a hit retires the row), so the row is hit in this
-    // tile iff its maximum has reached thr by the end of it.
+    // Without column pruning the steps need not test the rows: the row is hit in this tile
+    // iff the tile's maximum has reached thr by the end of it.
     if (!MASKED && !COLS) rowhit |= __builtin_amdgcn_ballot_w64(mx >= thr);
     R.cov = cov;
-    R.mx = mx;
+    R.mx = fmaxf(R.mx, mx);
+    R.mxl = fmaxf(R.mxl, mx - inc);
+    R.mxh = fmaxf(R.mxh, mx + inc);
 }
 
 // The unmasked tile of k_scan_select with the column terms as DPP operands.  tile_steps
@@ -378,11 +415,11 @@ __device__ __forceinline__ void tile_steps(const float4 *tile, RowState &R, int6
         : [cw] "v"(cd.w), [cx] "v"(cd.x), [cy] "v"(cd.y), [e0] "v"(e0), [rdf] "v"(rdf),        \
           [rdg] "v"(rdg))
 
-__device__ __forceinline__ void tile_steps_bc(const float4 *tile, RowState &R, uint64_t &rowhit,
-                                              uint64_t &colhit)
+__device__ __forceinline__ void tile_steps_bc(const float4 *tile, RowState &R, float inc,
+                                              uint64_t &rowhit, uint64_t &colhit)
 {
     const int t = threadIdx.x;
-    float cov = R.cov, mx = R.mx;
+    float cov = R.cov, mx = -INFINITY;
     const float thr = R.thr, rdf = R.rdf, rdg = R.rdg;
     const float e0 = t == 0 ? 1.f : 0.f;
 #pragma unroll
@@ -404,7 +441,9 @@ __device__ __forceinline__ void tile_steps_bc(const float4 *tile, RowState &R, u
         }
     }
     R.cov = cov;
-    R.mx = mx;
+    R.mx = fmaxf(R.mx, mx);
+    R.mxl = fmaxf(R.mxl, mx - inc);
+    R.mxh = fmaxf(R.mxh, mx + inc);
 }
 
 __device__ __forceinline__ void warm_steps_bc(const float4 *tile, RowState &R)
@@ -555,16 +594,20 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
                 continue;
             }
             R.tiles++;
+            // The rows' thresholds raised by the tile's error bound on cov*invn_j (ct is a
+            // multiple of 64: the tile is block ct/64's columns).
+            const float inc = R.kb * V.rho[ct >> 6];
+            R.thr = R.thr0 + inc;
             const bool masked = (ct - s - 63 <= m && ct - s + 63 >= -m) || (ct + WAVE > N) ||
                                 (s + WAVE > N) || (ct + WAVE > c1);
             uint64_t rowhit = 0, colhit = 0;
             if (masked) {
                 const int64_t lim = c1 < N ? c1 : N;
-                tile_steps<true, COLS>(tile, R, s, ct, N, lim, m, rowhit, colhit);
+                tile_steps<true, COLS>(tile, R, s, ct, N, lim, m, inc, rowhit, colhit);
             } else if (COLS) {
-                tile_steps_bc(tile, R, rowhit, colhit);
+                tile_steps_bc(tile, R, inc, rowhit, colhit);
             } else {
-                tile_steps<false, COLS>(tile, R, s, ct, N, N, m, rowhit, colhit);
+                tile_steps<false, COLS>(tile, R, s, ct, N, N, m, inc, rowhit, colhit);
             }
             if (V.wit) {
                 if (((rowhit & R.alive) >> t) & 1ull) V.wit[s + t] = (int)ct;
@@ -640,11 +683,15 @@ __device__ __forceinline__ void load_rows(RowState &R, const View &V, int64_t s,
     R.rdf = (t == 0) ? 0.f : rv.x;
     R.rdg = (t == 0) ? 0.f : rv.y;
     const float inv = rv.z;
+    // a degenerate row is at 2m from every column: pruned iff 2m <= r2, exactly
     if (inv > 0.f)
-        R.thr = thr_rho / inv;
+        R.thr0 = thr_rho / inv;
     else
-        R.thr = (thr_rho <= 0.f) ? -INFINITY : INFINITY;
+        R.thr0 = (thr_rho <= 0.f) ? -INFINITY : INFINITY;
+    R.thr = R.thr0;
     R.mx = -INFINITY;
+    R.mxl = -INFINITY;
+    R.mxh = -INFINITY;
     R.tiles = 0;
 }
 
@@ -653,6 +700,7 @@ __device__ __forceinline__ void load_block(const ScanLds &L, RowState &R, const 
                                            int64_t s, float thr_rho)
 {
     load_rows(R, V, s, thr_rho);
+    R.kb = V.dk * V.tmax[s >> 6].x;
     const int t = threadIdx.x;
     if (t < 16) L.qz[t] = 0.f;
     for (int k = 16 + V.m + t; k < L.K4 + 20; k += WAVE) L.qz[k] = 0.f;
@@ -663,8 +711,10 @@ __device__ __forceinline__ void flush_best(const RowState &R, const View &V, int
 {
     const int t = threadIdx.x;
     if (((R.alive >> t) & 1ull) && (s + t) < V.N && R.mx > -INFINITY) {
-        const float rho = R.mx * V.colv[s + t].z;
-        atomicMax(V.best + s + t, ord_key(rho));
+        const float inv = V.colv[s + t].z;
+        atomicMax(V.best + s + t, ord_key(R.mx * inv));
+        atomicMax(V.besth + s + t, ord_key(R.mxh * inv));
+        atomicMax(V.bestl + s + t, ord_key(R.mxl * inv));
     }
 }
 
@@ -699,8 +749,13 @@ __device__ __forceinline__ int make_view(const ScanArgs &A, int64_t gb, View &V,
     const SerDev S = A.ser[sid];
     V.x = A.x + S.xoff;
     V.colv = A.colv + S.roff;
+    V.tmax = A.tmax + (S.roff >> 6);
+    V.rho = A.rho + (S.roff >> 6);
     V.alive = A.alive + (S.roff >> 6);
     V.best = A.best + S.roff;
+    V.besth = A.besth + S.roff;
+    V.bestl = A.bestl + S.roff;
+    V.dk = A.dk;
     V.wit = A.wit ? A.wit + S.roff : nullptr;
     V.N = S.N;
     V.m = A.m;
@@ -719,8 +774,12 @@ __device__ __forceinline__ ScanArgs load_args(const ScanPtrs &Q, const PassDev *
     A.ser = Q.ser;
     A.bser = Q.bser;
     A.colv = Q.colv;
+    A.tmax = Q.tmax;
+    A.rho = Q.rho;
     A.alive = Q.alive;
     A.best = Q.best;
+    A.besth = Q.besth;
+    A.bestl = Q.bestl;
     A.wit = S.wsel == 0 ? Q.wit0 : (S.wsel == 1 ? Q.wit1 : nullptr);
     A.wit_read = S.rsel == 0 ? Q.wit0 : (S.rsel == 1 ? Q.wit1 : nullptr);
     A.pairs = Q.pairs;
@@ -729,6 +788,7 @@ __device__ __forceinline__ ScanArgs load_args(const ScanPtrs &Q, const PassDev *
     A.U0 = S.U0;
     A.m = S.m;
     A.witness_max = S.witness_max;
+    A.dk = S.dk;
     return A;
 }
 
@@ -842,7 +902,8 @@ void launch_scan_refine(const ScanPtrs &Q, const PassDev *P, const int *blist, P
 // ---------------------------------------------------------------------------------------
 
 // Start of a pass: alive bits of every global row (1 for real rows, < N of their series;
-// 0 for padding and for series switched off), row maxima to -inf, this length's witness
+// 0 for padding and for series switched off), the column factor of every block's tile,
+// row maxima to -inf, this length's witness
 // tiles of the pass's series to none (a series switched off keeps those of its last pass at
 // this length, which the next length reads), counters to zero.  Fill kernel rather than
 // memset nodes, so that a captured graph replays it faithfully.
@@ -855,6 +916,8 @@ __global__ __launch_bounds__(256) void k_pass_init(ScanPtrs Q, const PassDev *__
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g < rows) {
         A.best[g] = (int)0x807fffff;  // ord_key(-inf)
+        A.besth[g] = (int)0x807fffff;
+        A.bestl[g] = (int)0x807fffff;
         if (A.wit) {
             const int sid = P->S.some_off ? A.bser[g >> 6] : -1;
             if (sid < 0 || A.ser[sid].N > 0) A.wit[g] = -1;
@@ -872,6 +935,17 @@ __global__ __launch_bounds__(256) void k_pass_init(ScanPtrs Q, const PassDev *__
                 v = (~0ull) >> (64 - (S.N - r0));
         }
         alive[g] = v;
+        // A scan tile over the 64 columns of block g holds chains started at most 63 columns
+        // earlier, so the centred products it sums are bounded through sqrt(M2) over blocks
+        // g-1 and g; its correlations are those products times invn of g's columns.
+        float r = 0.f;
+        if (sid >= 0) {
+            const float2 cur = Q.tmax[g];
+            float sm = cur.x;
+            if (g > 0 && A.bser[g - 1] == sid) sm = fmaxf(sm, Q.tmax[g - 1].x);
+            r = sm * cur.y;
+        }
+        Q.rho[g] = r;
     }
     if (g < (int64_t)(sizeof(PassCtl) / sizeof(int))) reinterpret_cast<int *>(ctl)[g] = 0;
 }
@@ -913,7 +987,8 @@ __global__ __launch_bounds__(256) void k_block_best(ScanPtrs Q, const PassDev *_
     if (lane == 0) out[b] = key;
 }
 
-// Surviving rows -> (global row, FP32 squared distance); FLT_MAX when no valid pair was seen.
+// Surviving rows -> (global row, bounds of the squared NN distance from the single-precision
+// scans); FLT_MAX when no valid pair was seen.
 __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__restrict__ P,
                                                    int64_t rows, PassCtl *__restrict__ ctl,
                                                    Surv *__restrict__ out, int64_t cap)
@@ -923,13 +998,15 @@ __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__
     const ScanArgs A = load_args(Q, P);
     if (!((A.alive[g >> 6] >> (g & 63)) & 1ull)) return;
     const int k = A.best[g];
-    float d;
-    if (k == ord_key(-INFINITY))
-        d = FLT_MAX;
-    else
-        d = 2.0f * (float)A.m * (1.0f - ord_val(k));
+    float d = FLT_MAX, dlo = FLT_MAX, dhi = FLT_MAX;
+    if (k != ord_key(-INFINITY)) {
+        const float m2 = 2.0f * (float)A.m;
+        d = m2 * (1.0f - ord_val(k));
+        dlo = m2 * (1.0f - ord_val(A.besth[g]));
+        dhi = m2 * (1.0f - ord_val(A.bestl[g]));
+    }
     const int p = atomicAdd(&ctl->survivors, 1);
-    if (p < cap) out[p] = Surv{g, d, 0};
+    if (p < cap) out[p] = Surv{g, d, dlo, dhi};
 }
 
 // ---------------------------------------------------------------------------------------

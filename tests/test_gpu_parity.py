@@ -138,6 +138,44 @@ def test_degenerate_windows(ctx):
         assert_same_discords(got, expected(ref), f"flat m={m}")
 
 
+def ill_conditioned_series():
+    """Series whose window scales change by orders of magnitude within a few windows."""
+    t = np.arange(12000.0)
+    rng = np.random.default_rng(4)
+    square = np.where((t // 150) % 2 == 0, 10.0, -10.0) + rng.normal(0.0, 0.01, t.size)
+    spikes = synth.ar2_mix(12000, 7).astype(np.float64)
+    spikes[3000] += 1e5
+    spikes[8000] -= 1e5 / 3
+    offset = synth.ar2_mix(12000, 5).astype(np.float64) + 1e4
+    return [("square", square.astype(np.float32), 40, 44),
+            ("spikes", spikes.astype(np.float32), 48, 52),
+            ("offset", offset.astype(np.float32), 40, 44)]
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched", "learned"])
+def test_ill_conditioned_series_match_oracle(ctx, mode):
+    """A noisy square wave (plateaus of spread 0.01 between jumps of 20), spikes of 1e5 on
+    unit-scale noise, and an offset of 1e4: a single-precision chain that crosses a jump
+    carries rounding of the jump's scale into cells of the plateau's, thousands of times
+    smaller, so the pruning margin has to grow with that ratio (a fixed margin misses
+    discords on the square wave at m = 42 and 44)."""
+    for name, x, lo, hi in ill_conditioned_series():
+        ref = O.palmad(x, lo, hi, 2, 3)
+        got = ctx.discords(x, lo, hi, 2, 3, mode=mode)
+        assert_same_discords(got, expected(ref), f"{name} {mode}")
+
+
+def test_ill_conditioned_range_flags(ctx):
+    name, x, m, _ = ill_conditioned_series()[0]
+    nn_ref = O.profile(x, m)
+    for q in (0.5, 0.99, 0.999):
+        r2 = float(np.quantile(nn_ref, q))
+        flag, _ = ctx.range_discords(x, m, r2)
+        cand, _ = O.range_discords(x, m, r2)
+        near = np.abs(nn_ref - r2) <= 1e-7 * r2
+        assert np.array_equal((flag == 1)[~near], (cand == 1)[~near]), (name, q)
+
+
 @pytest.mark.parametrize("mode", ["plain", "learned"])
 def test_long_windows_match_oracle(ctx, mode):
     """m near 1000: the scans' windows and seed products span more than 1000 values, and the
