@@ -288,7 +288,8 @@ struct RowState {
     float thr;     // the current tile's threshold: prune when cov*invn_j >= thr
     float kb;      // dk * (largest sqrt(M2) of the block's rows), wave-uniform
     float mx;      // max over valid columns of cov*invn_j
-    float mxl, mxh;// max over tiles of (the tile's max of cov*invn_j -+ the tile's bound)
+    float mt;      // the same over the current super-tile
+    float mxl, mxh;// max over super-tiles of (their max of cov*invn_j -+ their bound)
     uint64_t alive;// wave-uniform alive mask of the 64 rows
     uint32_t tiles;// 64x64 tiles evaluated (pair counter)
     float qsum;    // sum of the staged, centred first row (L.qz)
@@ -314,11 +315,11 @@ struct View {
 // far columns it scans belong to blocks that are almost all dead by then).
 template <bool MASKED, bool COLS>
 __device__ __forceinline__ void tile_steps(const float4 *tile, RowState &R, int64_t s, int64_t c,
-                                           int64_t N, int64_t climit, int m, float inc,
-                                           uint64_t &rowhit, uint64_t &colhit)
+                                           int64_t N, int64_t climit, int m, uint64_t &rowhit,
+                                           uint64_t &colhit)
 {
     const int t = threadIdx.x;
-    float cov = R.cov, mx = -INFINITY;
+    float cov = R.cov, mx = R.mt;
     const float thr = R.thr, rdf = R.rdf, rdg = R.rdg;
     const int dbase = (int)(c - s) - t;
     const bool rowin = (s + t) < N;
@@ -353,13 +354,12 @@ __device__ __forceinline__ void tile_steps(const float4 *tile, RowState &R, int6
             }
         }
     }
-    // Without column pruning the steps need not test the rows: the row is hit in this tile
-    // iff the tile's maximum has reached thr by the end of it.
+    // Without column pruning the steps need not test the rows: thr holds over the super-tile
+    // and a live row's maximum over it is below thr when the tile starts (a hit retires the
+    // row), so the row is hit in this tile iff that maximum has reached thr by its end.
     if (!MASKED && !COLS) rowhit |= __builtin_amdgcn_ballot_w64(mx >= thr);
     R.cov = cov;
-    R.mx = fmaxf(R.mx, mx);
-    R.mxl = fmaxf(R.mxl, mx - inc);
-    R.mxh = fmaxf(R.mxh, mx + inc);
+    R.mt = mx;
 }
 
 // The unmasked tile of k_scan_select with the column terms as DPP operands.  tile_steps
@@ -415,11 +415,11 @@ __device__ __forceinline__ void tile_steps(const float4 *tile, RowState &R, int6
         : [cw] "v"(cd.w), [cx] "v"(cd.x), [cy] "v"(cd.y), [e0] "v"(e0), [rdf] "v"(rdf),        \
           [rdg] "v"(rdg))
 
-__device__ __forceinline__ void tile_steps_bc(const float4 *tile, RowState &R, float inc,
-                                              uint64_t &rowhit, uint64_t &colhit)
+__device__ __forceinline__ void tile_steps_bc(const float4 *tile, RowState &R, uint64_t &rowhit,
+                                              uint64_t &colhit)
 {
     const int t = threadIdx.x;
-    float cov = R.cov, mx = -INFINITY;
+    float cov = R.cov, mx = R.mt;
     const float thr = R.thr, rdf = R.rdf, rdg = R.rdg;
     const float e0 = t == 0 ? 1.f : 0.f;
 #pragma unroll
@@ -441,9 +441,7 @@ __device__ __forceinline__ void tile_steps_bc(const float4 *tile, RowState &R, f
         }
     }
     R.cov = cov;
-    R.mx = fmaxf(R.mx, mx);
-    R.mxl = fmaxf(R.mxl, mx - inc);
-    R.mxh = fmaxf(R.mxh, mx + inc);
+    R.mt = mx;
 }
 
 __device__ __forceinline__ void warm_steps_bc(const float4 *tile, RowState &R)
@@ -574,6 +572,17 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
         const int r0 = (int)(c % W);
         __syncthreads();  // ring writes of the previous super-tile are visible
         super_seeds(L, r0);
+        // The rows' thresholds raised by the super-tile's error bound on cov*invn_j: the
+        // largest factor of its tiles (tiles start at multiples of 64, so tile T is block
+        // (c + 64 T) / 64's columns; positive floats order as their bits).
+        uint32_t rb = 0;
+        for (int T = 0; T < 4 && c + 64 * T < c1; T++) {
+            const uint32_t b = __float_as_uint(V.rho[(c >> 6) + T]);
+            rb = rb > b ? rb : b;
+        }
+        const float inc = R.kb * __uint_as_float(rb);
+        R.thr = R.thr0 + inc;
+        R.mt = -INFINITY;
         __syncthreads();
         for (int T = 0; T < 4; T++) {
             const int64_t ct = c + 64 * T;
@@ -594,20 +603,16 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
                 continue;
             }
             R.tiles++;
-            // The rows' thresholds raised by the tile's error bound on cov*invn_j (ct is a
-            // multiple of 64: the tile is block ct/64's columns).
-            const float inc = R.kb * V.rho[ct >> 6];
-            R.thr = R.thr0 + inc;
             const bool masked = (ct - s - 63 <= m && ct - s + 63 >= -m) || (ct + WAVE > N) ||
                                 (s + WAVE > N) || (ct + WAVE > c1);
             uint64_t rowhit = 0, colhit = 0;
             if (masked) {
                 const int64_t lim = c1 < N ? c1 : N;
-                tile_steps<true, COLS>(tile, R, s, ct, N, lim, m, inc, rowhit, colhit);
+                tile_steps<true, COLS>(tile, R, s, ct, N, lim, m, rowhit, colhit);
             } else if (COLS) {
-                tile_steps_bc(tile, R, inc, rowhit, colhit);
+                tile_steps_bc(tile, R, rowhit, colhit);
             } else {
-                tile_steps<false, COLS>(tile, R, s, ct, N, N, m, inc, rowhit, colhit);
+                tile_steps<false, COLS>(tile, R, s, ct, N, N, m, rowhit, colhit);
             }
             if (V.wit) {
                 if (((rowhit & R.alive) >> t) & 1ull) V.wit[s + t] = (int)ct;
@@ -629,6 +634,9 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
             }
             if (R.alive == 0) return false;
         }
+        R.mx = fmaxf(R.mx, R.mt);
+        R.mxl = fmaxf(R.mxl, R.mt - inc);
+        R.mxh = fmaxf(R.mxh, R.mt + inc);
         // The next window: its new columns [c + 256 + K4, c + 512 + K4) replace columns
         // before c + 256 (W >= 256 + K4), then the one after is requested.
         if (c + 256 < c1) {
