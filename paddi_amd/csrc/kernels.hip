@@ -178,6 +178,30 @@ __device__ __forceinline__ float wave_max(float v)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// DPP row operation with zero fill: lanes of the rows in ROWS and banks (groups of 4 lanes
+// of a row) in BANKS read src through CTRL, all others and lanes without a source get 0.
+template <int CTRL, int ROWS, int BANKS>
+__device__ __forceinline__ float dpp_z(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, BANKS, true));
+}
+
+// Inclusive prefix maximum over lanes 0..t of non-negative values (every lane active; 0 is
+// the identity): within each row of 16 lanes the three preceding lanes (row_shr:1..3), then
+// 4 back for banks 1-3 and 8 back for banks 2-3, which completes the row; then rows 1 and 3
+// take the last lane of rows 0 and 2 (row_bcast:15) and rows 2 and 3 that of row 1
+// (row_bcast:31).
+__device__ __forceinline__ float wave_prefix_max(float x)
+{
+    float v = fmaxf(fmaxf(x, dpp_z<0x111, 0xF, 0xF>(x)),
+                    fmaxf(dpp_z<0x112, 0xF, 0xF>(x), dpp_z<0x113, 0xF, 0xF>(x)));
+    v = fmaxf(v, dpp_z<0x114, 0xF, 0xE>(v));
+    v = fmaxf(v, dpp_z<0x118, 0xF, 0xC>(v));
+    v = fmaxf(v, dpp_z<0x142, 0xA, 0xF>(v));
+    v = fmaxf(v, dpp_z<0x143, 0xC, 0xF>(v));
+    return v;
+}
+
 // Per-length terms for every global row (padding rows get zeros), and per 64-row block
 // (one wave; rows is a multiple of 64) the largest sqrt(M2) and invn of its rows.
 __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
@@ -286,7 +310,7 @@ struct RowState {
     float rdf, rdg;// row recurrence terms (0 on lane 0, which takes seeds)
     float thr0;    // (1 - r2/(2m)) / invn_i
     float thr;     // the current tile's threshold: prune when cov*invn_j >= thr
-    float kb;      // dk * (largest sqrt(M2) of the block's rows), wave-uniform
+    float kb;      // dk * (largest sqrt(M2) of rows s..s+t, the rows this lane's chains cross)
     float mx;      // max over valid columns of cov*invn_j
     float mt;      // the same over the current super-tile
     float mxl, mxh;// max over super-tiles of (their max of cov*invn_j -+ their bound)
@@ -299,7 +323,6 @@ struct RowState {
 struct View {
     const float *x;
     const float4 *colv;
-    const float2 *tmax;
     const float *rho;
     uint64_t *alive;
     int *best, *besth, *bestl;
@@ -573,8 +596,9 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
         __syncthreads();  // ring writes of the previous super-tile are visible
         super_seeds(L, r0);
         // The rows' thresholds raised by the super-tile's error bound on cov*invn_j: the
-        // largest factor of its tiles (tiles start at multiples of 64, so tile T is block
-        // (c + 64 T) / 64's columns; positive floats order as their bits).
+        // lane's row scale times the largest column factor of its tiles (tiles start at
+        // multiples of 64, so tile T is block (c + 64 T) / 64's columns; positive floats
+        // order as their bits).
         uint32_t rb = 0;
         for (int T = 0; T < 4 && c + 64 * T < c1; T++) {
             const uint32_t b = __float_as_uint(V.rho[(c >> 6) + T]);
@@ -708,8 +732,11 @@ __device__ __forceinline__ void load_block(const ScanLds &L, RowState &R, const 
                                            int64_t s, float thr_rho)
 {
     load_rows(R, V, s, thr_rho);
-    R.kb = V.dk * V.tmax[s >> 6].x;
     const int t = threadIdx.x;
+    // sqrt(M2) of row s+t is 1/invn (0 for degenerate and padding rows); prefix maximum
+    // over lanes 0..t
+    const float inv = V.colv[s + t].z;
+    R.kb = V.dk * wave_prefix_max(inv > 0.f ? 1.0f / inv : 0.f);
     if (t < 16) L.qz[t] = 0.f;
     for (int k = 16 + V.m + t; k < L.K4 + 20; k += WAVE) L.qz[k] = 0.f;
     R.qsum = stage_centred(L.qz + 16, V.x, s, V.m, V.colv[s].w);
@@ -757,7 +784,6 @@ __device__ __forceinline__ int make_view(const ScanArgs &A, int64_t gb, View &V,
     const SerDev S = A.ser[sid];
     V.x = A.x + S.xoff;
     V.colv = A.colv + S.roff;
-    V.tmax = A.tmax + (S.roff >> 6);
     V.rho = A.rho + (S.roff >> 6);
     V.alive = A.alive + (S.roff >> 6);
     V.best = A.best + S.roff;
