@@ -309,7 +309,7 @@ struct RowState {
     float cov;     // chain value of this lane (row s+t) at the last processed column
     float rdf, rdg;// row recurrence terms (0 on lane 0, which takes seeds)
     float thr0;    // (1 - r2/(2m)) / invn_i
-    float thr;     // the current tile's threshold: prune when cov*invn_j >= thr
+    float thr;     // the current super-tile's threshold: prune when cov*invn_j >= thr
     float kb;      // dk * (largest sqrt(M2) of rows s..s+t, the rows this lane's chains cross)
     float mx;      // max over valid columns of cov*invn_j
     float mt;      // the same over the current super-tile
