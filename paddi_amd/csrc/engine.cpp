@@ -34,6 +34,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -279,6 +280,7 @@ Series *Engine::upload(const float *T, int64_t n)
 
 void Engine::build_batch(Batch &B, const std::vector<const Series *> &ss, int minL)
 {
+    derive_m2_ = 0;  // statistics computed ahead belong to the previous row layout
     B.nser = (int)ss.size();
     B.minL = minL;
     B.n.assign(ss.size(), 0);
@@ -368,18 +370,25 @@ void Engine::prefix(Batch &B)
     HIPC(hipGetLastError());
 }
 
-void Engine::derive(Batch &B, int m, paddi_stats *st)
+void Engine::derive(Batch &B, int m, paddi_stats *st, int m2)
 {
     for (int q = 0; q < B.nser; q++) B.host[(size_t)q].N = B.n[(size_t)q] - m + 1;
     colv_.reserve((size_t)B.rows * sizeof(float4));
     st64_.reserve((size_t)B.rows * sizeof(double2));
     tmax_.reserve((size_t)B.nblocks * sizeof(float2));
+    // the previous derive's second length is this one: its statistics are already there
+    const bool have =
+        derive_m2_ == m && derive_m2_ > 0 && st64n_.bytes >= (size_t)B.rows * sizeof(double2);
+    if (have) st64_.swap(st64n_);
+    if (m2) st64n_.reserve((size_t)B.rows * sizeof(double2));
+    derive_m2_ = m2;
     rho_.reserve((size_t)B.nblocks * sizeof(float));
     HIPC(hipEventRecord(ev_[4], stream_));
     hipLaunchKernelGGL(k_derive, dim3(grid1(B.rows, 256)), dim3(256), 0, stream_,
                        B.x.as<float>(), B.ser.as<SerDev>(), B.bser.as<int>(), B.P.as<double2>(),
-                       B.refs.as<double>(), B.rows, m, colv_.as<float4>(), st64_.as<double2>(),
-                       tmax_.as<float2>());
+                       B.refs.as<double>(), B.rows, m, colv_.as<float4>(),
+                       have ? nullptr : st64_.as<double2>(), tmax_.as<float2>(), m2,
+                       st64n_.as<double2>());
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ev_[5], stream_));
     derive_st_ = st;  // timed at the next synchronisation (collect_derive)
@@ -700,10 +709,14 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
 
 // Exact (double) NN distance of (series, local row) requests.
 void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &req,
-                   std::vector<double> &out, paddi_stats *st)
+                   std::vector<double> &out, paddi_stats *st, int m2,
+                   std::vector<double> *out2)
 {
     out.assign(req.size(), 0.0);
+    if (out2) out2->assign(req.size(), DBL_MAX);
     if (req.empty()) return;
+    // the second length needs its statistics from the last derive
+    if (!out2 || m2 <= m || m2 > m + EXACT_EXT || m2 != derive_m2_) m2 = 0;
     // the series table on the device is derive(m)'s: every series at N = n - m + 1
     const int umax = exact_max_u(m);
     std::vector<size_t> order(req.size());
@@ -759,11 +772,11 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
         goff.push_back(L.blocks);
         launches.push_back(L);
     }
-    // one block: outkey (EXACT_SLOTS per candidate, set to the maximum key) | cand | goff |
-    // gfirst | gser, staged in pinned memory and copied in one piece
+    // one block: outkey (EXACT_SLOTS per candidate, set to the maximum key) | outkey2 (with
+    // m2) | cand | goff | gfirst | gser, staged in pinned memory and copied in one piece
     const size_t nc = cand.size(), ngx = gfirst.size();
-    const size_t nk = nc * EXACT_SLOTS;
-    const size_t o_cand = nk * 8, o_goff = o_cand + nc * 8, o_gfirst = o_goff + ngx * 8,
+    const size_t nk = nc * EXACT_SLOTS, nkeys = m2 ? 2 * nk : nk;
+    const size_t o_cand = nkeys * 8, o_goff = o_cand + nc * 8, o_gfirst = o_goff + ngx * 8,
                  o_gser = o_gfirst + ngx * 4, xbytes = o_gser + ngx * 4;
     if (xbytes > pin_x_bytes_) {
         if (pin_x_) HIPC(hipHostFree(pin_x_));
@@ -773,7 +786,7 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
         pin_x_bytes_ = xbytes * 2;
     }
     char *hx = (char *)pin_x_;
-    std::memset(hx, 0xff, nk * 8);
+    std::memset(hx, 0xff, nkeys * 8);
     std::memcpy(hx + o_cand, cand.data(), nc * 8);
     std::memcpy(hx + o_goff, goff.data(), ngx * 8);
     std::memcpy(hx + o_gfirst, gfirst.data(), ngx * 4);
@@ -792,25 +805,31 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
         E.gser = (const int *)(dx + o_gser) + L.off;
         E.goff = (const int64_t *)(dx + o_goff) + L.off;
         E.outkey = (unsigned long long *)dx;
+        E.st64n = st64n_.as<double2>();
+        E.outkey2 = (unsigned long long *)dx + nk;
         E.ngroups = L.ngroups;
         E.m = m;
+        E.m2 = m2;
         launch_exact(E, L.nu, (unsigned)L.blocks, exact_lds_bytes(m, L.nu), stream_);
         HIPC(hipGetLastError());
     }
     HIPC(hipEventRecord(ev_[7], stream_));
-    HIPC(hipMemcpyAsync(hx, dx, nk * 8, hipMemcpyDeviceToHost, stream_));
+    HIPC(hipMemcpyAsync(hx, dx, nkeys * 8, hipMemcpyDeviceToHost, stream_));
     wait();
-    std::vector<unsigned long long> keys(nc, ~0ull);
-    for (size_t q = 0; q < nc; q++)
-        for (int b = 0; b < EXACT_SLOTS; b++)
-            keys[q] = std::min(keys[q], ((const unsigned long long *)hx)[q * EXACT_SLOTS + b]);
     collect_derive();
-    for (size_t q = 0; q < slot.size(); q++) {
-        const unsigned long long k = keys[q];
+    // minimum over the slots, decoded from ord_key64
+    auto value = [&](size_t base, size_t q) {
+        unsigned long long k = ~0ull;
+        for (int b = 0; b < EXACT_SLOTS; b++)
+            k = std::min(k, ((const unsigned long long *)hx)[base + q * EXACT_SLOTS + b]);
         const unsigned long long bits = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
         double d;
         std::memcpy(&d, &bits, 8);
-        out[slot[q]] = d;
+        return d;
+    };
+    for (size_t q = 0; q < slot.size(); q++) {
+        out[slot[q]] = value(0, q);
+        if (m2) (*out2)[slot[q]] = value(nk, q);
     }
     if (st) {
         float a = 0.f;
@@ -821,7 +840,7 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
     }
 }
 
-void Engine::topk_all(Batch &B, int m, std::vector<TopK> &tk, paddi_stats *st)
+void Engine::topk_all(Batch &B, int m, std::vector<TopK> &tk, paddi_stats *st, int m2)
 {
     for (;;) {
         std::vector<std::pair<int, int64_t>> req;
@@ -837,12 +856,13 @@ void Engine::topk_all(Batch &B, int m, std::vector<TopK> &tk, paddi_stats *st)
                 }
         }
         if (req.empty()) break;
-        std::vector<double> v;
-        exact(B, m, req, v, st);
+        std::vector<double> v, v2;
+        exact(B, m, req, v, st, m2, &v2);
         for (size_t u = 0; u < who.size(); u++) {
             Cand &c = tk[(size_t)who[u].first].cands[who[u].second];
             c.exact = true;
             c.v = v[u];
+            c.v2 = v2[u];
         }
     }
 }
@@ -957,14 +977,19 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
     const int max_att = o.max_attempts > 0 ? o.max_attempts : 64;
     const bool learned = o.mode == PADDI_MODE_LEARNED;
     std::vector<std::vector<int64_t>> prev((size_t)ns);
+    // With the previous discords evaluated at i only, their exact distances at the next
+    // length come with the exact refinements of this one (k_exact's second length).
+    const bool ahead = prev_shift_ == 0 && step <= EXACT_EXT;
+    std::vector<std::vector<double>> prev_v((size_t)ns);  // prev's distances at this length
     std::vector<std::vector<paddi_discord>> res((size_t)ns);
     std::vector<std::vector<Cand>> surv;
     std::vector<TopK> tk((size_t)ns);
     wread_valid_ = false;
     for (int j = 0; j < nlen; j++) {
         const int m = minL + j * step;
+        const int m_next = (ahead && j + 1 < nlen) ? m + step : 0;
         len_begin(st);
-        derive(B, m, st);
+        derive(B, m, st, m_next);
         std::vector<double> r2((size_t)ns, 0.0);
         std::vector<char> from_prev((size_t)ns, 0);
         // r from the previous length's discords, evaluated exactly at this length.  A
@@ -977,8 +1002,27 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
             std::vector<int> grp;  // request -> (series, discord) group
             int ngrp = 0;
             std::vector<int> gser;
+            std::vector<double> mn((size_t)ns, INFINITY);
+            std::vector<int> cnt((size_t)ns, 0);
+            std::vector<char> done((size_t)ns, 0);
+            for (int q = 0; q < ns; q++) {
+                const std::vector<double> &pv = prev_v[(size_t)q];
+                if ((int)prev[(size_t)q].size() < k || pv.size() != prev[(size_t)q].size())
+                    continue;
+                bool ok = true;
+                for (double v : pv) ok &= v >= 0.0;
+                if (!ok) continue;
+                // the same rule as below, from the values computed ahead
+                const int64_t Nq = B.n[(size_t)q] - m + 1;
+                for (size_t u = 0; u < pv.size(); u++) {
+                    if (prev[(size_t)q][u] >= Nq) continue;
+                    cnt[(size_t)q]++;
+                    if (pv[u] < 3.0e38) mn[(size_t)q] = std::min(mn[(size_t)q], pv[u]);
+                }
+                done[(size_t)q] = 1;
+            }
             for (int q = 0; q < ns; q++)
-                if ((int)prev[(size_t)q].size() >= k)
+                if ((int)prev[(size_t)q].size() >= k && !done[(size_t)q])
                     for (int64_t i : prev[(size_t)q]) {
                         const int64_t Nq = B.n[(size_t)q] - m + 1;
                         bool any = false;
@@ -999,8 +1043,6 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
             std::vector<double> best((size_t)ngrp, -INFINITY);
             for (size_t u = 0; u < req.size(); u++)
                 if (v[u] < 3.0e38) best[(size_t)grp[u]] = std::max(best[(size_t)grp[u]], v[u]);
-            std::vector<double> mn((size_t)ns, INFINITY);
-            std::vector<int> cnt((size_t)ns, 0);
             for (int g = 0; g < ngrp; g++) {
                 const int q = gser[(size_t)g];
                 mn[(size_t)q] = std::min(mn[(size_t)q], best[(size_t)g] > -INFINITY
@@ -1056,7 +1098,7 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
                 } else {
                     tk[(size_t)q].done = true;
                 }
-            topk_all(B, m, tk, st);
+            topk_all(B, m, tk, st, m_next);
             for (int q = 0; q < ns; q++) {
                 if (!active[(size_t)q]) continue;
                 att[(size_t)q]++;
@@ -1071,7 +1113,11 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
                                      acc.empty() ? 0.0 : acc.back().v, att[(size_t)q]);
                     emit(res[(size_t)q], acc, m, q, r2[(size_t)q]);
                     prev[(size_t)q].clear();
-                    for (const Cand &c : acc) prev[(size_t)q].push_back(c.i);
+                    prev_v[(size_t)q].clear();
+                    for (const Cand &c : acc) {
+                        prev[(size_t)q].push_back(c.i);
+                        if (m_next) prev_v[(size_t)q].push_back(c.v2);
+                    }
                     active[(size_t)q] = 0;
                 } else {
                     // below the k-th distance the answer is the same: step r down, not

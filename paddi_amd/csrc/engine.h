@@ -34,6 +34,11 @@ struct DevBuf {
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     template <class T> T *as() const { return (T *)p; }
+    void swap(DevBuf &o)
+    {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+    }
 };
 
 // A series uploaded to HBM (paddi_series_upload).
@@ -43,13 +48,14 @@ struct Series {
 };
 
 // Surviving row: single-precision NN distance a with bounds alo <= exact <= ahi (up to
-// conv_ed2), exact distance v once computed.
+// conv_ed2), exact distance v once computed (and v2 at the next length, when asked for).
 struct Cand {
     int64_t i;
     float a;
     double v;
     bool exact;
     float alo, ahi;
+    double v2 = -1.0;
 };
 
 // Exact greedy top-k of one series at one (m, r2) (see engine.cpp).
@@ -88,7 +94,8 @@ public:
     Series *upload(const float *T, int64_t n);
     void build_batch(Batch &B, const std::vector<const Series *> &ss, int minL);
     void prefix(Batch &B);
-    void derive(Batch &B, int m, paddi_stats *st);
+    // Window statistics at m (and at m2 > 0 for k_exact's second length).
+    void derive(Batch &B, int m, paddi_stats *st, int m2 = 0);
     // One (m, r2_s) pass over every series of the batch; survivors per series.
     void pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
               std::vector<std::vector<Cand>> &out, paddi_stats *st);
@@ -98,11 +105,14 @@ public:
     // First-length candidates of the series in `need` (see engine.cpp).
     void bound_candidates(Batch &B, int m, const std::vector<char> &need, int count,
                           std::vector<std::vector<int64_t>> &cand, paddi_stats *st);
-    // Exact NN ED^2 of (series, local row) requests at length m (derive(m) must be current).
+    // Exact NN ED^2 of (series, local row) requests at length m (derive(m) must be current);
+    // with m2 in (m, m + EXACT_EXT] also at m2 into out2 (DBL_MAX: none).
     void exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &req,
-               std::vector<double> &out, paddi_stats *st);
-    // Run the exact top-k of every series to completion (batched exact refines).
-    void topk_all(Batch &B, int m, std::vector<TopK> &tk, paddi_stats *st);
+               std::vector<double> &out, paddi_stats *st, int m2 = 0,
+               std::vector<double> *out2 = nullptr);
+    // Run the exact top-k of every series to completion (batched exact refines); with m2,
+    // every refined candidate also gets its exact distance at m2.
+    void topk_all(Batch &B, int m, std::vector<TopK> &tk, paddi_stats *st, int m2 = 0);
 
     void run(const std::vector<const Series *> &ss, int minL, int maxL, int step, int k,
              const paddi_opts &o, std::vector<paddi_discord> &out, paddi_stats *st);
@@ -180,6 +190,8 @@ private:
     hipEvent_t sync_ev_ = nullptr;  // wait(): completion of the work queued so far
     DevBuf colv_, st64_, alive_, best_, besth_, bestl_, wit_[2], blist_, par_, pout_;
     DevBuf tmax_, rho_;            // per global block: row scales, tile column factors
+    DevBuf st64n_;                 // (mu, invn) at the last derive's second length
+    int derive_m2_ = 0;            // that length (0: none)
     DevBuf xarg_;                  // exact-refine arguments and results (one block)
     DevBuf bkey_;                  // per-block candidate keys of bound_candidates
     void *pin_x_ = nullptr;        // its pinned staging

@@ -109,9 +109,15 @@ struct ExactArgs {
     const int *gser;          // series of group g
     const int64_t *goff;      // first block of group g (prefix sums of 256-column chunks)
     unsigned long long *outkey;  // [candidate][EXACT_SLOTS]: minima by block slot
+    // Optional second length m2 in (m, m + 4]: the same candidates' distances at m2 from the
+    // same dot products extended by m2 - m terms
+    const double2 *st64n;     // global rows: (mu, invn) at m2 (k_derive)
+    unsigned long long *outkey2; // as outkey, at m2
     int ngroups;
     int m;
+    int m2;                   // 0: one length
 };
+constexpr int EXACT_EXT = 4;  // largest m2 - m
 
 __global__ void k_prefix(const float *__restrict__ x, const SerDev *__restrict__ ser,
                          const int *__restrict__ pser, double2 *__restrict__ P,
@@ -120,7 +126,7 @@ __global__ void k_derive(const float *__restrict__ x, const SerDev *__restrict__
                          const int *__restrict__ bser, const double2 *__restrict__ P,
                          const double *__restrict__ refs, int64_t rows, int m,
                          float4 *__restrict__ colv, double2 *__restrict__ st64,
-                         float2 *__restrict__ tmax);
+                         float2 *__restrict__ tmax, int m2, double2 *__restrict__ st64n);
 // The scan kernels: one 64-row block per workgroup (one wave); LDS from scan_lds_bytes(m).
 void launch_scan_select(const ScanPtrs &Q, const PassDev *P, unsigned grid, size_t lds,
                         hipStream_t st);

@@ -203,7 +203,9 @@ __device__ __forceinline__ float wave_prefix_max(float x)
 }
 
 // Per-length terms for every global row (padding rows get zeros), and per 64-row block
-// (one wave; rows is a multiple of 64) the largest sqrt(M2) and invn of its rows.
+// (one wave; rows is a multiple of 64) the largest sqrt(M2) and invn of its rows; with
+// m2 > 0 also (mu, invn) at length m2 into st64n (k_exact's second length).  st64 = null
+// skips (mu, invn) at m (already there from the previous length's st64n).
 __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
                                                 const SerDev *__restrict__ ser,
                                                 const int *__restrict__ bser,
@@ -211,11 +213,12 @@ __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
                                                 const double *__restrict__ refs, int64_t rows,
                                                 int m, float4 *__restrict__ colv,
                                                 double2 *__restrict__ st64,
-                                                float2 *__restrict__ tmax)
+                                                float2 *__restrict__ tmax, int m2,
+                                                double2 *__restrict__ st64n)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float4 cv = make_float4(0.f, 0.f, 0.f, 0.f);
-    double2 sv = make_double2(0.0, 0.0);
+    double2 sv = make_double2(0.0, 0.0), sv2 = make_double2(0.0, 0.0);
     float sd = 0.f;
     const int sid = g < rows ? bser[g >> 6] : -1;
     if (sid >= 0) {
@@ -238,11 +241,17 @@ __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
             cv = make_float4((float)df, (float)dg, (float)invn, (float)mu);
             sv = make_double2(mu, invn);
             sd = sqrtf((float)M2);
+            if (m2 && i < S.n - m2 + 1) {
+                double mu2, inv2;
+                window_stats(Ps, rs, i, m2, &mu2, &inv2);
+                sv2 = make_double2(mu2, inv2);
+            }
         }
     }
     if (g < rows) {
         colv[g] = cv;
-        st64[g] = sv;
+        if (st64) st64[g] = sv;
+        if (m2) st64n[g] = sv2;
     }
     const float a = wave_max(sd), b = wave_max(cv.z);
     if ((threadIdx.x & 63) == 0 && g < rows) tmax[g >> 6] = make_float2(a, b);
@@ -1068,6 +1077,15 @@ __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__
 // index is a constant and the window never moves between registers.  Per column the sum runs
 // in ascending k with one FMA per term; candidate rows are zero-padded to a multiple of 4,
 // which adds exact zeros.
+// The pair distance from its dot product (both rows' statistics at length m).
+__device__ __forceinline__ double pair_ed2(double dot, int m, double mui, double invi,
+                                           double muj, double invj)
+{
+    if (invi > 0.0 && invj > 0.0)
+        return 2.0 * m * (1.0 - (dot - m * (mui * muj)) * (invi * invj));
+    return 2.0 * m;
+}
+
 template <int QC>
 __device__ __forceinline__ int xs_at(int p)
 {
@@ -1152,7 +1170,9 @@ __device__ __forceinline__ void exact_quad3(const double *xt, int k0, double a0,
     if (B == 12) XQ3(12, 13, 14, 15);
 }
 
-template <int NU>
+// DUAL: also the distances at the second length E.m2 (a separate instantiation: its
+// epilogue needs more registers, which would cost the one-length launches occupancy).
+template <int NU, bool DUAL>
 __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
 {
     constexpr int QC = exact_qc(NU);
@@ -1164,7 +1184,8 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     const int m4 = exact_mk(m, NU);                     // a multiple of 4 (of 16 for NU = 3)
     double *xs = dsh;                                   // exact_xs_doubles(m, NU)
     double *A = dsh + exact_xs_doubles(m, NU);          // [NU][m4] candidate windows
-    __shared__ double ainv[MAXU], amu[MAXU];
+    __shared__ double ainv[MAXU], amu[MAXU], ainv2[MAXU], amu2[MAXU], aext[MAXU][EXACT_EXT];
+    __shared__ int64_t cidx[MAXU];
     __shared__ unsigned long long wmin[MAXU][4];
     const int t = threadIdx.x;
     int lo = 0, hi = E.ngroups - 1;
@@ -1180,6 +1201,10 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     const float *x = E.x + S.xoff;
     const double2 *st64 = E.st64 + S.roff;
     const int64_t N = S.n - m + 1;
+    const int m2 = DUAL ? E.m2 : 0;
+    const int ext = DUAL ? m2 - m : 0;
+    const int64_t N2 = DUAL ? S.n - m2 + 1 : 0;
+    const double2 *st64n = E.st64n + S.roff;
     const int64_t cblk = (item - E.goff[grp]) * EXACT_COLS;
     for (int u = 0; u < NU; u++) {
         const int64_t i = E.cand[u0 + (u < nU ? u : 0)];
@@ -1187,12 +1212,18 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
         if (t == 0) {
             ainv[u] = st.y;
             amu[u] = st.x;
+            const double2 s2 = i < N2 ? st64n[i] : make_double2(0.0, 0.0);
+            amu2[u] = s2.x;
+            ainv2[u] = s2.y;
+            cidx[u] = i;
+            for (int e = 0; e < EXACT_EXT; e++)
+                aext[u][e] = (e < ext && i < N2) ? (double)x[i + m + e] : 0.0;
         }
         for (int k = t; k < m4; k += 256) A[u * m4 + k] = k < m ? (double)x[i + k] : 0.0;
     }
-    unsigned long long kmin[NU];
+    unsigned long long kmin[NU], kmin2[NU];
 #pragma unroll
-    for (int u = 0; u < NU; u++) kmin[u] = ~0ull;
+    for (int u = 0; u < NU; u++) kmin[u] = kmin2[u] = ~0ull;
     const int lim = SUBC + m4 + 4;  // staged values per sub-tile
     for (int sub = 0; sub < EXACT_COLS / SUBC; sub++) {
         const int64_t c = cblk + sub * SUBC;
@@ -1242,31 +1273,43 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
             if (k0 < m4) exact_step<NU, QC, 0>(xt, A, m4, k0, w, acc);
             if (UR == 3 && k0 + 4 < m4) exact_step<NU, QC, 4>(xt, A, m4, k0 + 4, w, acc);
         }
-        double muj[QC], invj[QC];
+        // Per column: its statistics, then every candidate's distance at m and, with m2, the
+        // dot product at m2 as the same FMA chain continued over k = m .. m2-1 (the
+        // zero-padded terms between add exact zeros), so each value is bitwise the one a
+        // launch at m2 computes.  Staged values reach column offset 256 QC + m4 + 3 >=
+        // 256 QC - 1 + m2 - 1.
 #pragma unroll
         for (int q = 0; q < QC; q++) {
             const int64_t j = c + QC * t + q;
             const double2 sj = (j < N) ? st64[j] : make_double2(0.0, 0.0);
-            muj[q] = sj.x;
-            invj[q] = sj.y;
-        }
+            double2 sj2 = make_double2(0.0, 0.0);
+            double xe[EXACT_EXT];
+            if (DUAL) {
+                if (j < N2) sj2 = st64n[j];
 #pragma unroll
-        for (int u = 0; u < NU; u++) {
-            const int64_t i = E.cand[u0 + (u < nU ? u : 0)];
+                for (int e = 0; e < EXACT_EXT; e++)
+                    xe[e] = e < ext ? xs[xs_at<QC>(QC * t + q + m + e)] : 0.0;
+            }
 #pragma unroll
-            for (int q = 0; q < QC; q++) {
-                const int64_t j = c + QC * t + q;
-                double ed = DBL_MAX;
+            for (int u = 0; u < NU; u++) {
+                const int64_t i = cidx[u];
                 const bool valid = (j < N) && ((j - i >= m - 1) || (i - j >= m + 1));
-                if (valid) {
-                    if (ainv[u] > 0.0 && invj[q] > 0.0)
-                        ed = 2.0 * m * (1.0 - (acc[u][q] - m * (amu[u] * muj[q])) *
-                                                  (ainv[u] * invj[q]));
-                    else
-                        ed = 2.0 * m;
-                }
+                const double ed =
+                    valid ? pair_ed2(acc[u][q], m, amu[u], ainv[u], sj.x, sj.y) : DBL_MAX;
                 const unsigned long long key = ord_key64(ed);
                 kmin[u] = key < kmin[u] ? key : kmin[u];
+                if (DUAL) {
+                    double a2 = acc[u][q];
+#pragma unroll
+                    for (int e = 0; e < EXACT_EXT; e++)
+                        if (e < ext) a2 = fma(aext[u][e], xe[e], a2);
+                    const bool valid2 =
+                        (i < N2) && (j < N2) && ((j - i >= m2 - 1) || (i - j >= m2 + 1));
+                    const double ed2 =
+                        valid2 ? pair_ed2(a2, m2, amu2[u], ainv2[u], sj2.x, sj2.y) : DBL_MAX;
+                    const unsigned long long key2 = ord_key64(ed2);
+                    kmin2[u] = key2 < kmin2[u] ? key2 : kmin2[u];
+                }
             }
         }
     }
@@ -1287,20 +1330,49 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
         for (int q = 1; q < 4; q++) k = wmin[t][q] < k ? wmin[t][q] : k;
         atomicMin(E.outkey + (size_t)(u0 + t) * EXACT_SLOTS + (blockIdx.x & (EXACT_SLOTS - 1)), k);
     }
+    if (DUAL) {
+        __syncthreads();  // wmin is read above
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+            unsigned long long key = kmin2[u];
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                const unsigned long long other = __shfl_xor(key, o, 64);
+                key = other < key ? other : key;
+            }
+            if (lane == 0) wmin[u][w] = key;
+        }
+        __syncthreads();
+        if (t < nU) {
+            unsigned long long k = wmin[t][0];
+            for (int q = 1; q < 4; q++) k = wmin[t][q] < k ? wmin[t][q] : k;
+            atomicMin(E.outkey2 + (size_t)(u0 + t) * EXACT_SLOTS + (blockIdx.x & (EXACT_SLOTS - 1)),
+                      k);
+        }
+    }
+}
+
+template <bool DUAL>
+static void launch_exact_t(const ExactArgs &E, int nu, unsigned grid, size_t lds, hipStream_t st)
+{
+    switch (nu) {
+    case 1: hipLaunchKernelGGL((k_exact<1, DUAL>), dim3(grid), dim3(256), lds, st, E); break;
+    case 2: hipLaunchKernelGGL((k_exact<2, DUAL>), dim3(grid), dim3(256), lds, st, E); break;
+    case 3: hipLaunchKernelGGL((k_exact<3, DUAL>), dim3(grid), dim3(256), lds, st, E); break;
+    case 4: hipLaunchKernelGGL((k_exact<4, DUAL>), dim3(grid), dim3(256), lds, st, E); break;
+    case 5: hipLaunchKernelGGL((k_exact<5, DUAL>), dim3(grid), dim3(256), lds, st, E); break;
+    case 6: hipLaunchKernelGGL((k_exact<6, DUAL>), dim3(grid), dim3(256), lds, st, E); break;
+    case 7: hipLaunchKernelGGL((k_exact<7, DUAL>), dim3(grid), dim3(256), lds, st, E); break;
+    default: hipLaunchKernelGGL((k_exact<8, DUAL>), dim3(grid), dim3(256), lds, st, E); break;
+    }
 }
 
 void launch_exact(const ExactArgs &E, int nu, unsigned grid, size_t lds, hipStream_t st)
 {
-    switch (nu) {
-    case 1: hipLaunchKernelGGL((k_exact<1>), dim3(grid), dim3(256), lds, st, E); break;
-    case 2: hipLaunchKernelGGL((k_exact<2>), dim3(grid), dim3(256), lds, st, E); break;
-    case 3: hipLaunchKernelGGL((k_exact<3>), dim3(grid), dim3(256), lds, st, E); break;
-    case 4: hipLaunchKernelGGL((k_exact<4>), dim3(grid), dim3(256), lds, st, E); break;
-    case 5: hipLaunchKernelGGL((k_exact<5>), dim3(grid), dim3(256), lds, st, E); break;
-    case 6: hipLaunchKernelGGL((k_exact<6>), dim3(grid), dim3(256), lds, st, E); break;
-    case 7: hipLaunchKernelGGL((k_exact<7>), dim3(grid), dim3(256), lds, st, E); break;
-    default: hipLaunchKernelGGL((k_exact<8>), dim3(grid), dim3(256), lds, st, E); break;
-    }
+    if (E.m2)
+        launch_exact_t<true>(E, nu, grid, lds, st);
+    else
+        launch_exact_t<false>(E, nu, grid, lds, st);
 }
 
 }  // namespace paddi

@@ -85,3 +85,23 @@ def test_graph_replays_match_eager_and_oracle(ctx, restore):
         assert [(g.index, g.length) for g in mine] == [(i, m) for i, m, _ in exp]
         for g, (_, _, d) in zip(mine, exp):
             assert abs(g.nn_dist2 - d) <= RTOL * d
+
+
+@pytest.mark.parametrize("mode", ["batched", "learned"])
+@pytest.mark.parametrize("step", [1, 4])
+def test_next_length_range_from_the_exact_refinements(ctx, restore, mode, step):
+    """With prev_shift = 0 a length's r comes from its previous discords' distances at this
+    length, computed by the previous length's exact refinements (a second length in the same
+    launches) instead of a launch of its own: over several lengths, steps of 1 and 4 and two
+    uneven series the discords stay the oracle's."""
+    series = [synth.rw_planted(9000, 61, 2, 64, 12.0), synth.ar2_mix(7000, 62)]
+    minL, maxL, k = 48, 64, 3
+    ctx.set_tunables(prev_shift=0)
+    got = ctx.discords_batch(series, minL, maxL, step, k, mode=mode)
+    for q, x in enumerate(series):
+        ref = O.palmad(x, minL, maxL, step, k)
+        exp = [(int(i), r["m"], float(d)) for r in ref for i, d in zip(r["idx"], r["d2"])]
+        mine = [g for g in got if g.series == q]
+        assert [(g.index, g.length) for g in mine] == [(i, m) for i, m, _ in exp], (q, step)
+        for g, (_, _, d) in zip(mine, exp):
+            assert abs(g.nn_dist2 - d) <= RTOL * d
