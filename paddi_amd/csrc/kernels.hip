@@ -1280,6 +1280,9 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
         // 256 QC - 1 + m2 - 1.
 #pragma unroll
         for (int q = 0; q < QC; q++) {
+            // keep each column's loads in its own iteration (hoisted, the second length's
+            // terms of every column would take registers from the main loop's occupancy)
+            if (DUAL) __builtin_amdgcn_sched_barrier(0);
             const int64_t j = c + QC * t + q;
             const double2 sj = (j < N) ? st64[j] : make_double2(0.0, 0.0);
             double2 sj2 = make_double2(0.0, 0.0);
