@@ -114,6 +114,9 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t);
 paddi_status paddi_compute_statistics(paddi_ctx *ctx, const float *T, int64_t n, int32_t m,
                                       float *mean, float *std);
 
+/* Every entry point that takes series returns PADDI_ERR_INVALID when a series holds a NaN
+ * or an infinite value. */
+
 /* Range discords at squared range r2 over the n-m+1 windows of T.
  * Replaces find_discords_cpu (src/cpu_kernels/cpu_kernels.cpp:33-78) and the
  * single-GPU do_PD3 + refine pipeline (src/PD3.cuh:15-61):

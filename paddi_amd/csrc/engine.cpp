@@ -216,7 +216,7 @@ Engine::Engine(int device) : device_(device)
     HIPC(hipSetDevice(device));
     HIPC(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     for (auto &e : ev_) HIPC(hipEventCreate(&e));
-    HIPC(hipHostMalloc(&pin_, 64, hipHostMallocDefault));
+    HIPC(hipHostMalloc(&pin_, 64, hipHostMallocDefault));  // k_prefix's non-finite flag
     HIPC(hipHostMalloc(&pin_out_, PASS_OUT_HEAD + (size_t)SCAP * sizeof(Surv),
                        hipHostMallocDefault));
     HIPC(hipEventCreateWithFlags(&ser_ev_, hipEventDisableTiming));
@@ -364,10 +364,16 @@ void Engine::upload_ser(Batch &B)
 
 void Engine::prefix(Batch &B)
 {
+    flag_.reserve(64);
+    HIPC(hipMemsetAsync(flag_.p, 0, sizeof(int), stream_));
     hipLaunchKernelGGL(k_prefix, dim3((unsigned)B.npb), dim3(256), 0, stream_,
                        B.x.as<float>(), B.ser.as<SerDev>(), B.pser.as<int>(), B.P.as<double2>(),
-                       B.refs.as<double>());
+                       B.refs.as<double>(), flag_.as<int>());
     HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(pin_, flag_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    wait();
+    if (*(volatile int *)pin_)
+        throw HipError{PADDI_ERR_INVALID, "the series holds a NaN or infinite value"};
 }
 
 void Engine::derive(Batch &B, int m, paddi_stats *st, int m2)

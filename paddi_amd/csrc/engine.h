@@ -181,6 +181,7 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[8] = {};
     void *pin_ = nullptr;
+    DevBuf flag_;                  // k_prefix's non-finite flag
     // pinned staging: pass parameters (PassDev + series table), pass results (PassCtl +
     // the first SCAP survivors), series table of derive/exact; ser_ev_ marks the end of
     // the last copy out of pin_ser_

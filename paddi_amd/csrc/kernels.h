@@ -119,9 +119,10 @@ struct ExactArgs {
 };
 constexpr int EXACT_EXT = 4;  // largest m2 - m
 
+// Blocked prefix sums of every series; *bad becomes nonzero if a value is NaN or infinite.
 __global__ void k_prefix(const float *__restrict__ x, const SerDev *__restrict__ ser,
                          const int *__restrict__ pser, double2 *__restrict__ P,
-                         double *__restrict__ refs);
+                         double *__restrict__ refs, int *__restrict__ bad);
 __global__ void k_derive(const float *__restrict__ x, const SerDev *__restrict__ ser,
                          const int *__restrict__ bser, const double2 *__restrict__ P,
                          const double *__restrict__ refs, int64_t rows, int m,

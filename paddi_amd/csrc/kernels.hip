@@ -75,7 +75,8 @@ __device__ __forceinline__ unsigned long long ord_key64(double d)
 __global__ __launch_bounds__(256) void k_prefix(const float *__restrict__ x,
                                                 const SerDev *__restrict__ ser,
                                                 const int *__restrict__ pser,
-                                                double2 *__restrict__ P, double *__restrict__ refs)
+                                                double2 *__restrict__ P, double *__restrict__ refs,
+                                                int *__restrict__ bad)
 {
     const int t = threadIdx.x;
     const int gb = blockIdx.x;
@@ -88,10 +89,13 @@ __global__ __launch_bounds__(256) void k_prefix(const float *__restrict__ x,
     constexpr int PER = PB / 256;
     double v1[PER], v2[PER];
     double s1 = 0.0, s2 = 0.0;
+    bool nonfinite = false;
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         int64_t g = base + (int64_t)t * PER + q;
-        double y = (g < n) ? ((double)xs[g] - c) : 0.0;
+        const float xv = (g < n) ? xs[g] : 0.f;
+        nonfinite |= !isfinite(xv);
+        double y = (g < n) ? ((double)xv - c) : 0.0;
         v1[q] = s1;
         v2[q] = s2;
         s1 += y;
@@ -116,6 +120,7 @@ __global__ __launch_bounds__(256) void k_prefix(const float *__restrict__ x,
         Pb[t * PER + q] = make_double2(o1 + v1[q], o2 + v2[q]);
     if (t == 255) Pb[PB] = make_double2(o1 + s1, o2 + s2);
     if (t == 0) refs[gb] = c;
+    if (nonfinite) atomicOr(bad, 1);
 }
 
 // (sum, sum of squares) of (x - c_b) over the window [i, i+m) of one series; m <= PB.

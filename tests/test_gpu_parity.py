@@ -322,3 +322,21 @@ def test_batched_c1_matches_cached_oracle(ctx):
         ref = json.load(f)
     got = ctx.discords(w["series"][0], w["minL"], w["maxL"], w["step"], w["k"], mode="batched")
     assert_same_discords(got, expected(ref), "c1 batched")
+
+
+def test_non_finite_series_is_rejected(ctx):
+    """A NaN or infinity anywhere in a series is an invalid argument (status 1), found while
+    the prefix sums are built; the context stays usable."""
+    from paddi_amd.engine import PaddiError
+    x = synth.ar2_mix(3000, 5)
+    for bad in (np.nan, np.inf, -np.inf):
+        y = x.copy()
+        y[1234] = bad
+        with pytest.raises(PaddiError) as e:
+            ctx.discords(y, 32, 34, 1, 2)
+        assert e.value.status == 1
+        with pytest.raises(PaddiError) as e:
+            ctx.range_discords(y, 32, 10.0)
+        assert e.value.status == 1
+    ref = O.palmad(x, 32, 34, 1, 2)
+    assert_same_discords(ctx.discords(x, 32, 34, 1, 2), expected(ref), "after rejection")
