@@ -14,7 +14,7 @@
 // cell's single-precision distance: delta (the bound on data of uniform scale) times the
 // ratio of the largest window scale its chain crossed to the cell's own (per 64-row block
 // and 64-column tile, k_pass_init), so the surviving set contains every row whose exact NN
-// distance exceeds r2, and each survivor carries bounds alo <= exact <= a.  TopK ranks
+// distance exceeds r2, and each survivor carries bounds alo <= exact <= ahi.  TopK ranks
 // survivors by exact distance, computing the exact (double) distance only for candidates
 // whose rank those intervals leave undecided and for every accepted discord (whose reported
 // distance is then the double-precision value).

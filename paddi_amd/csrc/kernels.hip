@@ -1035,8 +1035,8 @@ __global__ __launch_bounds__(256) void k_block_best(ScanPtrs Q, const PassDev *_
     if (lane == 0) out[b] = key;
 }
 
-// Surviving rows -> (global row, bounds of the squared NN distance from the single-precision
-// scans); FLT_MAX when no valid pair was seen.
+// Surviving rows -> (global row, the single-precision squared NN distance and its bounds);
+// FLT_MAX when no valid pair was seen.
 __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__restrict__ P,
                                                    int64_t rows, PassCtl *__restrict__ ctl,
                                                    Surv *__restrict__ out, int64_t cap)
