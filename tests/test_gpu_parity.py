@@ -340,3 +340,18 @@ def test_non_finite_series_is_rejected(ctx):
         assert e.value.status == 1
     ref = O.palmad(x, 32, 34, 1, 2)
     assert_same_discords(ctx.discords(x, 32, 34, 1, 2), expected(ref), "after rejection")
+
+
+@pytest.mark.parametrize("mode", ["batched", "learned"])
+def test_batched_edge_series_match_oracle(ctx, mode):
+    """One call mixing a series with a single window at the longest length, series shorter
+    than one 64-row block, a ragged one and normal ones, with k larger than some of them
+    can hold: each series' discords equal the oracle's on it alone."""
+    minL, maxL, step, k = 20, 26, 2, 4
+    series = [synth.ar2_mix(maxL + 1, 11), synth.ar2_mix(70, 12), synth.ar2_mix(131, 13),
+              synth.rw_planted(3000, 14, 1, 24, 8.0), synth.ecg_like(2500, 15, 1, 0)]
+    got = ctx.discords_batch(series, minL, maxL, step, k, mode=mode)
+    for q, x in enumerate(series):
+        ref = O.palmad(x, minL, maxL, step, k)
+        mine = [g for g in got if g.series == q]
+        assert_same_discords(mine, expected(ref), f"series {q} n={len(x)}")
