@@ -355,3 +355,13 @@ def test_batched_edge_series_match_oracle(ctx, mode):
         ref = O.palmad(x, minL, maxL, step, k)
         mine = [g for g in got if g.series == q]
         assert_same_discords(mine, expected(ref), f"series {q} n={len(x)}")
+
+
+@pytest.mark.parametrize("mode", ["batched", "learned"])
+def test_large_k_matches_oracle(ctx, mode):
+    """Top-12 per length: the greedy top-k takes many exact rounds and drops many trivial
+    matches, and the next length's range comes from twelve previous discords."""
+    x = synth.rw_planted(20000, 91, 4, 48, 10.0)
+    ref = O.palmad(x, 48, 52, 1, 12)
+    got = ctx.discords(x, 48, 52, 1, 12, mode=mode)
+    assert_same_discords(got, expected(ref), mode)
