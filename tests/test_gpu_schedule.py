@@ -105,3 +105,16 @@ def test_next_length_range_from_the_exact_refinements(ctx, restore, mode, step):
         assert [(g.index, g.length) for g in mine] == [(i, m) for i, m, _ in exp], (q, step)
         for g, (_, _, d) in zip(mine, exp):
             assert abs(g.nn_dist2 - d) <= RTOL * d
+
+
+def test_next_length_range_with_large_groups(ctx, restore):
+    """Top-8 with prev_shift = 0 and speculate = 8: exact launches of up to eight candidates
+    that also evaluate them at the next length (the widest two-length kernels)."""
+    x = synth.rw_planted(16000, 93, 4, 40, 10.0)
+    ctx.set_tunables(prev_shift=0, speculate=8)
+    ref = O.palmad(x, 40, 46, 1, 8)
+    exp = [(int(i), r["m"], float(d)) for r in ref for i, d in zip(r["idx"], r["d2"])]
+    got = ctx.discords(x, 40, 46, 1, 8, mode="learned")
+    assert [(g.index, g.length) for g in got] == [(i, m) for i, m, _ in exp]
+    for g, (_, _, d) in zip(got, exp):
+        assert abs(g.nn_dist2 - d) <= RTOL * d
