@@ -87,16 +87,17 @@ def test_graph_replays_match_eager_and_oracle(ctx, restore):
             assert abs(g.nn_dist2 - d) <= RTOL * d
 
 
+@pytest.mark.parametrize("graph", [0, 1])
 @pytest.mark.parametrize("mode", ["batched", "learned"])
 @pytest.mark.parametrize("step", [1, 4])
-def test_next_length_range_from_the_exact_refinements(ctx, restore, mode, step):
+def test_next_length_range_from_the_exact_refinements(ctx, restore, mode, step, graph):
     """With prev_shift = 0 a length's r comes from its previous discords' distances at this
     length, computed by the previous length's exact refinements (a second length in the same
-    launches) instead of a launch of its own: over several lengths, steps of 1 and 4 and two
-    uneven series the discords stay the oracle's."""
+    launches) instead of a launch of its own: over several lengths, steps of 1 and 4, two
+    uneven series, eager or replayed passes, the discords stay the oracle's."""
     series = [synth.rw_planted(9000, 61, 2, 64, 12.0), synth.ar2_mix(7000, 62)]
     minL, maxL, k = 48, 64, 3
-    ctx.set_tunables(prev_shift=0)
+    ctx.set_tunables(prev_shift=0, use_graph=graph)
     got = ctx.discords_batch(series, minL, maxL, step, k, mode=mode)
     for q, x in enumerate(series):
         ref = O.palmad(x, minL, maxL, step, k)
