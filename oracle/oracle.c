@@ -89,12 +89,10 @@ static inline int valid_pair(int64_t i, int64_t j, int32_t m)
     return d >= m;
 }
 
-static inline double ed2_f64(const float *T, const double *mean, const double *std,
-                             int64_t i, int64_t j, int32_t m)
+/* The pair value from its dot product (cpu_kernels.cpp:55 in double precision). */
+static inline double ed2_from_dot(double dot, const double *mean, const double *std,
+                                  int64_t i, int64_t j, int32_t m)
 {
-    double dot = 0.0;
-    for (int32_t k = 0; k < m; k++)
-        dot += (double)T[i + k] * (double)T[j + k];
     /* Products of the two rows' statistics are formed first, so the value is bitwise
      * symmetric in (i, j): mutual nearest neighbours tie exactly and the tie goes to the
      * lower index, as find_top1_discord's strict '>' intends (sets_processing.hpp:28). */
@@ -102,6 +100,66 @@ static inline double ed2_f64(const float *T, const double *mean, const double *s
     if (!(den > 0.0))
         return 2.0 * m;
     return 2.0 * m * (1.0 - (dot - m * (mean[i] * mean[j])) / den);
+}
+
+static inline double ed2_f64(const float *T, const double *mean, const double *std,
+                             int64_t i, int64_t j, int32_t m)
+{
+    double dot = 0.0;
+    for (int32_t k = 0; k < m; k++)
+        dot += (double)T[i + k] * (double)T[j + k];
+    return ed2_from_dot(dot, mean, std, i, j, m);
+}
+
+/*
+ * Row i against columns j0 .. j1-1 in ascending order, as the loop of
+ * cpu_kernels.cpp:47-75: valid pairs with ED^2 > r lower *nn, the first valid
+ * pair with ED^2 <= r ends the scan.  Returns that column, or -1.  Columns are
+ * evaluated eight at a time (eight independent dot products, each summed in
+ * ascending k exactly as ed2_f64 does) and then consumed in order, so every
+ * pair value and every decision is bitwise the one-column-at-a-time loop's;
+ * the block only lets the compiler run the eight sums side by side.
+ */
+#define OJB 8
+static int64_t scan_row(const float *T, const double *mean, const double *std, int64_t i,
+                        int64_t j0, int64_t j1, int64_t N, double r, int32_t m, double *nn)
+{
+    int64_t j = j0;
+    while (j < j1) {
+        if (j + OJB <= j1 && j + OJB <= N) {
+            int any = 0;
+            for (int q = 0; q < OJB; q++) any |= valid_pair(i, j + q, m);
+            if (!any) { j += OJB; continue; }
+            double acc[OJB];
+            for (int q = 0; q < OJB; q++) acc[q] = 0.0;
+            const float *ti = T + i, *tj = T + j;
+            for (int32_t k = 0; k < m; k++) {
+                const double xi = (double)ti[k];
+                for (int q = 0; q < OJB; q++) acc[q] += xi * (double)tj[k + q];
+            }
+            for (int q = 0; q < OJB; q++) {
+                if (!valid_pair(i, j + q, m)) continue;
+                const double ed = ed2_from_dot(acc[q], mean, std, i, j + q, m);
+                if (ed > r) {
+                    if (ed < *nn) *nn = ed;
+                } else {
+                    return j + q;
+                }
+            }
+            j += OJB;
+        } else {
+            if (valid_pair(i, j, m)) {
+                const double ed = ed2_f64(T, mean, std, i, j, m);
+                if (ed > r) {
+                    if (ed < *nn) *nn = ed;
+                } else {
+                    return j;
+                }
+            }
+            j++;
+        }
+    }
+    return -1;
 }
 
 /*
@@ -119,19 +177,46 @@ void oracle_find_discords_f64(const float *T, const double *mean, const double *
 #endif
     #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t i = 0; i < N; i++) {
-        int64_t j = 0;
-        int not_pruned = 1;
-        while (not_pruned && j < N) {
-            if (valid_pair(i, j, m)) {
-                double ed = ed2_f64(T, mean, std, i, j, m);
-                if (ed > r) {
-                    if (ed < nnDist[i]) nnDist[i] = ed;
-                } else {
-                    cand[i] = 0;
-                    not_pruned = 0;
-                }
+        if (scan_row(T, mean, std, i, 0, N, N, r, m, &nnDist[i]) >= 0) cand[i] = 0;
+    }
+}
+
+/*
+ * The same range-discord set, for large series (the certificates of long
+ * searches): cand[i] and, where cand[i] == 1, nnDist[i] equal
+ * oracle_find_discords_f64's bit for bit, since neither depends on the order
+ * in which a row meets its columns (cand[i] == 1 iff no valid pair has
+ * ED^2 <= r, and nnDist[i] is then the minimum over all valid pairs).  Each
+ * thread walks runs of 256 consecutive rows and lets row i first try column
+ * p + 1, where p is the column that pruned row i - 1 (windows i-1 and p are
+ * close, so windows i and p + 1, which share m - 1 points with them, usually
+ * are too); if that pair does not prune, the row is scanned from column 0 as
+ * above.  nnDist of pruned rows is unspecified (partial minima), as in the
+ * reference.
+ */
+void oracle_find_discords_hint_f64(const float *T, const double *mean, const double *std,
+                                   int32_t *cand, double *nnDist, int64_t N, double r,
+                                   int32_t m, int32_t nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    const int64_t RUN = 256;
+    const int64_t nruns = (N + RUN - 1) / RUN;
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t q = 0; q < nruns; q++) {
+        int64_t p = -1;
+        const int64_t e = (q + 1) * RUN < N ? (q + 1) * RUN : N;
+        for (int64_t i = q * RUN; i < e; i++) {
+            int64_t h = p >= 0 ? p + 1 : -1;
+            int64_t hit = -1;
+            if (h >= 0 && h < N && valid_pair(i, h, m)) {
+                double dummy = DBL_MAX;
+                hit = scan_row(T, mean, std, i, h, h + 1, N, r, m, &dummy);
             }
-            j++;
+            if (hit < 0) hit = scan_row(T, mean, std, i, 0, N, N, r, m, &nnDist[i]);
+            if (hit >= 0) cand[i] = 0;
+            p = hit;
         }
     }
 }
@@ -172,11 +257,7 @@ void oracle_profile_f64(const float *T, const double *mean, const double *std,
     #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t i = 0; i < N; i++) {
         double best = DBL_MAX;
-        for (int64_t j = 0; j < N; j++) {
-            if (!valid_pair(i, j, m)) continue;
-            double ed = ed2_f64(T, mean, std, i, j, m);
-            if (ed < best) best = ed;
-        }
+        (void)scan_row(T, mean, std, i, 0, N, N, -INFINITY, m, &best);
         nnDist[i] = best;
     }
 }

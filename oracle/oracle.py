@@ -30,6 +30,7 @@ def lib():
         L.oracle_stats_f64.argtypes = [P, i64, i32, P, P]
         L.oracle_stats_f32.argtypes = [P, i64, i32, P, P]
         L.oracle_find_discords_f64.argtypes = [P, P, P, P, P, i64, f64, i32, i32]
+        L.oracle_find_discords_hint_f64.argtypes = [P, P, P, P, P, i64, f64, i32, i32]
         L.oracle_find_discords_f32.argtypes = [P, P, P, P, P, i64, C.c_float, i32]
         L.oracle_profile_f64.argtypes = [P, P, P, P, i64, i32, i32]
         L.oracle_topk.argtypes = [P, P, i64, f64, i32, i32, P, P, P]
@@ -53,8 +54,11 @@ def stats(T, m, fp32=False):
     return mu, sd
 
 
-def range_discords(T, m, r2, fp32=False, nthreads=0):
-    """find_discords_cpu at squared range r2; returns (cand int32[N], nnDist[N])."""
+def range_discords(T, m, r2, fp32=False, nthreads=0, hint=False):
+    """find_discords_cpu at squared range r2; returns (cand int32[N], nnDist[N]).
+    hint=True runs the same scan with each row first trying the column one past its
+    predecessor's pruning column (same cand and, where cand == 1, the same nnDist; the
+    nnDist of pruned rows is unspecified)."""
     T = np.ascontiguousarray(T, dtype=np.float32)
     N = len(T) - m + 1
     mu, sd = stats(T, m, fp32)
@@ -65,8 +69,8 @@ def range_discords(T, m, r2, fp32=False, nthreads=0):
                                        C.c_float(r2), m)
     else:
         nn = np.full(N, np.finfo(np.float64).max, np.float64)
-        lib().oracle_find_discords_f64(_p(T), _p(mu), _p(sd), _p(cand), _p(nn), N,
-                                       float(r2), m, nthreads)
+        fn = lib().oracle_find_discords_hint_f64 if hint else lib().oracle_find_discords_f64
+        fn(_p(T), _p(mu), _p(sd), _p(cand), _p(nn), N, float(r2), m, nthreads)
     return cand, nn
 
 

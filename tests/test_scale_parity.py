@@ -1,0 +1,153 @@
+"""Parity at the full sizes of the workloads the throughput figures are quoted on.
+
+c1 (65,536 points) is checked against the cached oracle result elsewhere.  Here:
+
+  c2  1,048,576-point ECG-like series, m = 64..128, top-3
+  c3  8,388,608-point random walk, 20 planted bursts, m = 128..256, top-3
+  c4  2,097,152-point AR(2) + sine mixture, m = 256..512 step 4, top-3
+  c5  1,024 uneven series (20k..500k points), m = 64..96, top-1 per series
+
+each run as the bench runs it (the autotuner's schedule, the learned form) and checked two
+ways:
+
+* Forms: the learned, batched and plain GPU forms return the identical discord set (c5: the
+  plain method looped over a seeded sample of 16 of the series, against those series'
+  records of the one batched call).
+* Certificates (the oracle, src/cpu_kernels/cpu_kernels.cpp:33-78 restated in double
+  precision): at the first, a middle and the last length (c5: of 8 seeded series) the oracle
+  runs at r2' = (1 - 1e-3) * d_k, d_k the engine's k-th distance.  The greedy top-k over a
+  range set is the same for every range below d_k, so if the oracle's greedy top-k at r2'
+  equals the engine's (indices bit-exact, distances within 1e-4 relative), the engine's
+  answer at that length is the exact one; an engine d_k wrong by more than 1e-3 would put
+  r2' above the true d_k and the oracle would return a different set.  The oracle scans
+  each row's diagonal successor column first (oracle_find_discords_hint_f64: the same set
+  and distances, in seconds instead of minutes at 8.4M points).
+"""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from paddi_amd import synth
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-4
+_CACHE = {}
+
+
+def _workload(name):
+    if name not in _CACHE:
+        _CACHE[name] = {"w": synth.WORKLOADS[name]()}
+    return _CACHE[name]
+
+
+def _search(ctx, name, mode, sel=None):
+    """The discords of workload `name` in `mode` (cached per module), with the tuned
+    schedule; `sel` restricts the call to those series (records renumbered back)."""
+    ent = _workload(name)
+    key = (mode, None if sel is None else tuple(sel))
+    if key not in ent:
+        w = ent["w"]
+        saved = ctx.get_tunables()
+        try:
+            ctx.apply_tuned(w["minL"])
+            series = w["series"] if sel is None else [w["series"][q] for q in sel]
+            t0 = time.perf_counter()
+            recs = ctx.discords_batch(series, w["minL"], w["maxL"], w["step"], w["k"],
+                                      mode=mode)
+            dt = time.perf_counter() - t0
+        finally:
+            ctx.set_tunables(**saved)
+        if sel is not None:
+            for r in recs:
+                r.series = sel[r.series]
+        print(f"{name} {mode}{'' if sel is None else f' ({len(sel)} series)'}: "
+              f"{len(recs)} discords in {dt:.2f} s", flush=True)
+        ent[key] = recs
+    return ent[key]
+
+
+def _key(recs):
+    return [(r.series, r.length, r.index) for r in recs]
+
+
+def _same(a, b):
+    assert _key(a) == _key(b)
+    for x, y in zip(a, b):
+        assert abs(x.nn_dist2 - y.nn_dist2) <= RTOL * y.nn_dist2, (x, y)
+
+
+def _lengths(w):
+    ms = list(range(w["minL"], w["maxL"] + 1, w["step"]))
+    return ms[0], ms[len(ms) // 2], ms[-1]
+
+
+def _c5_sample(count, seed):
+    w = _workload("c5")["w"]
+    rng = np.random.default_rng(seed)
+    return sorted(int(q) for q in rng.choice(len(w["series"]), count, replace=False))
+
+
+@pytest.mark.parametrize("name", ["c2", "c4"])
+def test_forms_identical_at_full_size(ctx, name):
+    w = _workload(name)["w"]
+    learned = _search(ctx, name, "learned")
+    nlen = (w["maxL"] - w["minL"]) // w["step"] + 1
+    assert len(learned) == nlen * w["k"]
+    _same(_search(ctx, name, "batched"), learned)
+    _same(_search(ctx, name, "plain"), learned)
+
+
+def test_c3_plain_equals_learned(ctx):
+    learned = _search(ctx, "c3", "learned")
+    assert len(learned) == 129 * 3
+    _same(_search(ctx, "c3", "plain"), learned)
+
+
+def test_c5_batched_equals_learned(ctx):
+    learned = _search(ctx, "c5", "learned")
+    assert len(learned) == 1024 * 33
+    _same(_search(ctx, "c5", "batched"), learned)
+
+
+def test_c5_plain_loop_on_sampled_series_equals_learned(ctx):
+    sel = _c5_sample(16, 55)
+    learned = [r for r in _search(ctx, "c5", "learned") if r.series in set(sel)]
+    _same(_search(ctx, "c5", "plain", sel), learned)
+
+
+def _certify(x, m, k, recs, tag):
+    assert len(recs) == k, (tag, recs)
+    r2 = (1.0 - 1e-3) * recs[-1].nn_dist2
+    t0 = time.perf_counter()
+    cand, nn = O.range_discords(x, m, r2, hint=True)
+    idx, d, nd = O.topk(cand, nn, r2, m, k)
+    dt = time.perf_counter() - t0
+    assert [r.index for r in recs] == [int(i) for i in idx], (tag, recs, idx, d)
+    rel = max(abs(r.nn_dist2 - v) / v for r, v in zip(recs, d))
+    print(f"{tag}: certified by the oracle at r2' = {r2:.6g} ({nd} range discords, "
+          f"{dt:.1f} s on the host), max rel err {rel:.2e}", flush=True)
+    assert rel <= RTOL, (tag, rel)
+
+
+CERT = [(n, j) for n in ("c2", "c3", "c4") for j in range(3)]
+
+
+@pytest.mark.parametrize("name,j", CERT, ids=[f"{n}-{('first', 'middle', 'last')[j]}"
+                                               for n, j in CERT])
+def test_certificate_single_series(ctx, name, j):
+    w = _workload(name)["w"]
+    m = _lengths(w)[j]
+    recs = [r for r in _search(ctx, name, "learned") if r.length == m]
+    _certify(w["series"][0], m, w["k"], recs, f"{name} m={m}")
+
+
+@pytest.mark.parametrize("j", range(3), ids=["first", "middle", "last"])
+def test_certificate_c5_sampled_series(ctx, j):
+    w = _workload("c5")["w"]
+    m = _lengths(w)[j]
+    recs = _search(ctx, "c5", "learned")
+    for q in _c5_sample(8, 77 + j):
+        mine = [r for r in recs if r.series == q and r.length == m]
+        _certify(w["series"][q], m, w["k"], mine, f"c5 series {q} (n={len(w['series'][q])}) m={m}")
