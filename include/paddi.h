@@ -36,7 +36,9 @@ enum {
     PADDI_ERR_DEVICE = 2,       /* HIP runtime error */
     PADDI_ERR_NOMEM = 3,        /* device allocation failed */
     PADDI_ERR_CAPACITY = 4,     /* output buffer too small; *count says how many */
-    PADDI_ERR_UNSUPPORTED = 5   /* request outside the engine's limits (e.g. m > 4096) */
+    PADDI_ERR_UNSUPPORTED = 5   /* request outside the engine's limits: m > 2048, a call of
+                                   2^31 rows or more (rows = windows at the shortest length,
+                                   padded to 64 per series, plus 128 per series) */
 };
 
 /* Output record: struct Discord {int ind; DATA_TYPE dist;} (src/PALMAD_types.hpp:7-11)
@@ -50,9 +52,14 @@ typedef struct {
 } paddi_discord;
 
 enum {
-    PADDI_MODE_PLAIN = 0,    /* per-length launches, host r-descent (main.cu:136-233) */
-    PADDI_MODE_BATCHED = 1,  /* all lengths in one persistent launch (hipGraph) */
-    PADDI_MODE_LEARNED = 2   /* batched + learned r proposal, plain descent as fallback */
+    PADDI_MODE_PLAIN = 0,    /* each series alone, per length the r-descent of main.cu:136-233 */
+    PADDI_MODE_BATCHED = 1,  /* every series of the call in each launch; per length one pass at
+                                an r taken from the previous length's discords (exact values at
+                                the new length), retried lower on failure; a pass's kernels run
+                                without a host round trip and may replay as a hipGraph
+                                (paddi_tunables.use_graph) */
+    PADDI_MODE_LEARNED = 2   /* batched, with the first length's r from the learned proposal;
+                                the descent remains the fallback */
 };
 
 typedef struct {
@@ -128,35 +135,45 @@ paddi_status paddi_compute_statistics(paddi_ctx *ctx, const float *T, int64_t n,
 paddi_status paddi_range_discords(paddi_ctx *ctx, const float *T, int64_t n, int32_t m, float r2,
                                   int32_t *is_discord, float *nn_dist2);
 
-/* One length of a search, for the reference's times table (src/IOdata.hpp:140-177): the
- * phases of one GPU (statistics, scans, exact refinement) and the candidate counts. */
+/* One (m, r) attempt of a search: a row of the reference's times table (src/IOdata.hpp:140-177),
+ * which src/main.cu:144,176,219 fills once per do_PALMAD_one_length call.  The phases of one
+ * GPU (statistics, scans, exact refinement) and the counts.  In the batched forms an attempt is
+ * one pass over the call's series. */
 typedef struct {
     int32_t m;
-    int32_t attempts;        /* passes at this length (r-descent steps) */
-    int64_t survivors;       /* rows left by the scans over those passes (candidates) */
-    int32_t discords;        /* discords accepted at this length */
-    int32_t reserved;
-    double ms_derive;        /* k_derive: window statistics */
+    int32_t attempt;         /* 1, 2, .. within the length */
+    float r2;                /* squared range (batched forms: the smallest over the pass's series) */
+    int32_t discords;        /* discords accepted by the attempt (summed over the series that
+                                finished the length with it) */
+    int64_t survivors;       /* rows left by the scans (the table's "# Candidates") */
+    double ms_derive;        /* k_derive: window statistics (on the length's first attempt) */
     double ms_scan;          /* k_scan_select + k_scan_refine */
-    double ms_exact;         /* k_exact: double-precision refinement */
-    double ms_total;         /* host wall time of the length */
-} paddi_length_stats;
+    double ms_exact;         /* k_exact: double-precision refinement (the first attempt also
+                                carries the length's range estimate) */
+    double ms_total;         /* host wall time since the previous attempt ended */
+} paddi_attempt_stats;
 
 /* Top-k discords for every length m = minL, minL+step, .., <= maxL.
  * Replaces the PALMAD length loop of src/main.cu:136-233 (top-1 there; for
  * k > 1 discords are ranked by (nn_dist2 desc, index asc) and accepted when
  * |i - a| >= m for every accepted a).  out[] receives up to top_k records per
  * length, lengths ascending, each length's records in rank order; *count is
- * the number written.  opts and stats may be NULL. */
-/* paddi_discords with a record per length (per_length[(maxL - minL) / step + 1]); the
- * CLI's times table.  No counterpart beyond the reference's timers (src/main.cu:136-233). */
-paddi_status paddi_discords_timed(paddi_ctx *ctx, const float *T, int64_t n, int32_t minL,
-                                  int32_t maxL, int32_t step, int32_t top_k,
-                                  const paddi_opts *opts, paddi_discord *out, int64_t cap,
-                                  int64_t *count, paddi_length_stats *per_length);
+ * the number written.  opts and stats may be NULL.  A wait for device work
+ * longer than PADDI_WAIT_LIMIT_S seconds (environment, read when the context
+ * is created; default 600) returns PADDI_ERR_DEVICE. */
 paddi_status paddi_discords(paddi_ctx *ctx, const float *T, int64_t n, int32_t minL, int32_t maxL,
                             int32_t step, int32_t top_k, const paddi_opts *opts,
                             paddi_discord *out, int64_t cap, int64_t *count, paddi_stats *stats);
+
+/* paddi_discords with a record per (m, r) attempt, the rows of the reference's times table
+ * (src/IOdata.hpp:140-177; no counterpart beyond the reference's timers): per_attempt receives
+ * up to cap_attempts records in the order the attempts ran, *n_attempts their number (when it
+ * exceeds cap_attempts, the later records are dropped and the call still succeeds). */
+paddi_status paddi_discords_timed(paddi_ctx *ctx, const float *T, int64_t n, int32_t minL,
+                                  int32_t maxL, int32_t step, int32_t top_k,
+                                  const paddi_opts *opts, paddi_discord *out, int64_t cap,
+                                  int64_t *count, paddi_attempt_stats *per_attempt,
+                                  int64_t cap_attempts, int64_t *n_attempts);
 
 /* Many independent series (uneven lengths) in one call; records carry
  * `series`.  Same per-series semantics as paddi_discords. */

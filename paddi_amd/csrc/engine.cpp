@@ -40,6 +40,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/paddi.h"
@@ -61,19 +62,6 @@ const char *status_str(paddi_status s)
     default: return "unknown status";
     }
 }
-
-struct HipError {
-    paddi_status st;
-    std::string what;
-};
-
-#define HIPC(expr)                                                                     \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            throw HipError{e_ == hipErrorOutOfMemory ? PADDI_ERR_NOMEM : PADDI_ERR_DEVICE, \
-                           std::string(#expr) + ": " + hipGetErrorString(e_)};         \
-    } while (0)
 
 void DevBuf::reserve(size_t nbytes)
 {
@@ -223,6 +211,8 @@ Engine::Engine(int device) : device_(device)
     HIPC(hipEventCreateWithFlags(&sync_ev_, hipEventDisableTiming));
     const char *dbg = std::getenv("PADDI_DEBUG");
     debug_ = dbg && dbg[0] == '1';
+    const char *lim = std::getenv("PADDI_WAIT_LIMIT_S");
+    if (lim && *lim) wait_limit_s_ = std::atof(lim);
 }
 
 Engine::~Engine()
@@ -295,7 +285,7 @@ void Engine::build_batch(Batch &B, const std::vector<const Series *> &ss, int mi
         B.roff[q] = ro;
         B.pboff[q] = po;
         xo += round_up(n + XPAD, 64);
-        ro += round_up(n - minL + 1, 64) + 128;
+        ro += series_rows(n, minL);
         po += (n + PB - 1) / PB + 1;
     }
     B.xtotal = xo;
@@ -339,14 +329,27 @@ void Engine::build_batch(Batch &B, const std::vector<const Series *> &ss, int mi
 
 // Wait for everything queued on stream_: the engine's round trips are short, so the host
 // spins on an event instead of sleeping in hipStreamSynchronize (whose wake-up costs tens of
-// microseconds per round trip).
+// microseconds per round trip); past 2 ms it yields the core between queries and past
+// 100 ms sleeps 100 us between them.  Work that has not completed after wait_limit_s_
+// (PADDI_WAIT_LIMIT_S, default 600 s) raises PADDI_ERR_DEVICE; the work itself is not
+// cancelled, so the context's later calls queue behind it.
 void Engine::wait()
 {
     HIPC(hipEventRecord(sync_ev_, stream_));
+    const auto t0 = std::chrono::steady_clock::now();
     for (;;) {
         const hipError_t e = hipEventQuery(sync_ev_);
         if (e == hipSuccess) return;
         if (e != hipErrorNotReady) HIPC(e);
+        const double el =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (el >= wait_limit_s_)
+            throw HipError(PADDI_ERR_DEVICE, "device work did not complete within " +
+                                                 std::to_string(wait_limit_s_) + " s");
+        if (el > 0.1)
+            std::this_thread::sleep_for(std::chrono::microseconds(100));
+        else if (el > 2e-3)
+            std::this_thread::yield();
     }
 }
 
@@ -896,28 +899,33 @@ static double now_ms()
         .count();
 }
 
-void Engine::len_begin(paddi_stats *st)
+void Engine::att_mark(paddi_stats *st)
 {
-    if (!per_len_ || !st) return;
+    if (!att_log_ || !st) return;
     collect_derive();
-    len_snap_ = *st;
-    len_t0_ = now_ms();
+    att_snap_ = *st;
+    att_t0_ = now_ms();
 }
 
-void Engine::len_end(int j, int m, int discords, paddi_stats *st)
+void Engine::att_end(int m, int attempt, double r2, int discords, paddi_stats *st)
 {
-    if (!per_len_ || !st) return;
+    if (!att_log_ || !st) return;
     collect_derive();
-    paddi_length_stats &r = per_len_[j];
+    paddi_attempt_stats r{};
     r.m = m;
-    r.attempts += (int32_t)(st->attempts - len_snap_.attempts);
-    r.survivors += st->survivors - len_snap_.survivors;
-    r.discords += discords;
-    r.ms_derive += st->kernel_ms[0] - len_snap_.kernel_ms[0];
-    r.ms_scan += (st->kernel_ms[1] - len_snap_.kernel_ms[1]) +
-                 (st->kernel_ms[2] - len_snap_.kernel_ms[2]);
-    r.ms_exact += st->kernel_ms[3] - len_snap_.kernel_ms[3];
-    r.ms_total += now_ms() - len_t0_;
+    r.attempt = attempt;
+    r.r2 = (float)r2;
+    r.discords = discords;
+    r.survivors = st->survivors - att_snap_.survivors;
+    r.ms_derive = st->kernel_ms[0] - att_snap_.kernel_ms[0];
+    r.ms_scan = (st->kernel_ms[1] - att_snap_.kernel_ms[1]) +
+                (st->kernel_ms[2] - att_snap_.kernel_ms[2]);
+    r.ms_exact = st->kernel_ms[3] - att_snap_.kernel_ms[3];
+    const double t = now_ms();
+    r.ms_total = t - att_t0_;
+    att_log_->push_back(r);
+    att_snap_ = *st;
+    att_t0_ = t;
 }
 
 // The plain form for series sid: src/main.cu:136-233 (guards as in the oracle).
@@ -933,7 +941,7 @@ void Engine::plain(Batch &B, int sid, int minL, int maxL, int step, int k, const
     wread_valid_ = false;
     for (int j = 0; j < nlen; j++) {
         const int m = minL + j * step;
-        len_begin(st);
+        att_mark(st);
         derive(B, m, st);
         double r, S = 0.0;
         if (j == 0) {
@@ -963,10 +971,10 @@ void Engine::plain(Batch &B, int sid, int minL, int maxL, int step, int k, const
             topk_all(B, m, tk, st);
             att++;
             const std::vector<Cand> &acc = tk[(size_t)sid].acc;
+            att_end(m, att, r2, (int)acc.size(), st);
             if ((int)acc.size() >= k || r == 0.0) {
                 emit(out, acc, m, sid, r2);
                 dk[(size_t)j] = acc.empty() ? 0.0 : acc.back().v;
-                len_end(j, m, (int)acc.size(), st);
                 break;
             }
             r = next_r_after_fail(j, r, S);
@@ -994,7 +1002,7 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
     for (int j = 0; j < nlen; j++) {
         const int m = minL + j * step;
         const int m_next = (ahead && j + 1 < nlen) ? m + step : 0;
-        len_begin(st);
+        att_mark(st);
         derive(B, m, st, m_next);
         std::vector<double> r2((size_t)ns, 0.0);
         std::vector<char> from_prev((size_t)ns, 0);
@@ -1086,16 +1094,19 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
             }
         std::vector<char> active((size_t)ns, 1);
         std::vector<int> att((size_t)ns, 0);
-        for (;;) {
+        for (int npass = 1;; npass++) {
             std::vector<double> r2p((size_t)ns, -1.0);
             bool any = false;
+            double rmin = INFINITY;
             for (int q = 0; q < ns; q++)
                 if (active[(size_t)q]) {
                     if (att[(size_t)q] >= max_att) r2[(size_t)q] = 0.0;
                     r2p[(size_t)q] = r2[(size_t)q];
+                    rmin = std::min(rmin, r2[(size_t)q]);
                     any = true;
                 }
             if (!any) break;
+            int finished = 0;
             pass(B, m, r2p, true, surv, st);
             for (int q = 0; q < ns; q++)
                 if (active[(size_t)q]) {
@@ -1118,6 +1129,7 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
                                      floor0[(size_t)q], r2[(size_t)q],
                                      acc.empty() ? 0.0 : acc.back().v, att[(size_t)q]);
                     emit(res[(size_t)q], acc, m, q, r2[(size_t)q]);
+                    finished += (int)acc.size();
                     prev[(size_t)q].clear();
                     prev_v[(size_t)q].clear();
                     for (const Cand &c : acc) {
@@ -1139,11 +1151,7 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
                     if (r2[(size_t)q] < 1e-9) r2[(size_t)q] = 0.0;
                 }
             }
-        }
-        if (per_len_) {
-            int nd = 0;
-            for (int q = 0; q < ns; q++) nd += (int)tk[(size_t)q].acc.size();
-            len_end(j, m, nd, st);
+            att_end(m, npass, rmin, finished, st);
         }
         // this length's witnesses become the next length's read side
         wread_valid_ = true;
@@ -1181,22 +1189,21 @@ void Engine::run(const std::vector<const Series *> &ss, int minL, int maxL, int 
 // ------------------------------------------------------------------------------------------
 // C ABI
 
-template <class F>
-static paddi_status guarded(paddi_ctx *ctx, F &&f)
-{
-    try {
-        return f();
-    } catch (const paddi::HipError &e) {
-        if (ctx) ctx->last_error = e.what;
-        return e.st;
-    } catch (const std::bad_alloc &) {
-        return PADDI_ERR_NOMEM;
-    }
-}
-
 static bool valid_lengths(int64_t n, int32_t minL, int32_t maxL, int32_t step, int32_t k)
 {
     return minL >= 4 && maxL >= minL && step >= 1 && k >= 1 && n >= (int64_t)maxL + 1;
+}
+
+// The row space of a call fits the engine's 32-bit row indices (MAX_ROWS).
+static bool rows_fit(const int64_t *n, int32_t ns, int32_t minL)
+{
+    int64_t rows = 0;
+    for (int32_t q = 0; q < ns; q++) {
+        if (n[q] >= paddi::MAX_ROWS) return false;
+        rows += paddi::series_rows(n[q], minL);
+        if (rows >= paddi::MAX_ROWS) return false;
+    }
+    return true;
 }
 
 extern "C" {
@@ -1265,6 +1272,7 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
 paddi_status paddi_series_upload(paddi_ctx *ctx, const float *T, int64_t n, paddi_series **out)
 {
     if (!ctx || !T || !out || n < 8) return PADDI_ERR_INVALID;
+    if (n >= paddi::MAX_ROWS) return PADDI_ERR_UNSUPPORTED;
     return guarded(ctx, [&] {
         paddi::Series *s = ctx->eng->upload(T, n);
         *out = new paddi_series{ctx, s};
@@ -1283,7 +1291,7 @@ paddi_status paddi_compute_statistics(paddi_ctx *ctx, const float *T, int64_t n,
                                       float *mean, float *std)
 {
     if (!ctx || !T || !mean || !std || m < 2 || n < m) return PADDI_ERR_INVALID;
-    if (m > paddi::PB) return PADDI_ERR_UNSUPPORTED;
+    if (m > paddi::PB || !rows_fit(&n, 1, m)) return PADDI_ERR_UNSUPPORTED;
     return guarded(ctx, [&] {
         paddi::Engine &E = *ctx->eng;
         std::unique_ptr<paddi::Series> s(E.upload(T, n));
@@ -1308,7 +1316,7 @@ paddi_status paddi_range_discords(paddi_ctx *ctx, const float *T, int64_t n, int
 {
     if (!ctx || !T || !is_discord || !nn_dist2 || m < 4 || n < (int64_t)m + 1 || !(r2 >= 0.f))
         return PADDI_ERR_INVALID;
-    if (m > paddi::MAXM) return PADDI_ERR_UNSUPPORTED;
+    if (m > paddi::MAXM || !rows_fit(&n, 1, m)) return PADDI_ERR_UNSUPPORTED;
     return guarded(ctx, [&] {
         paddi::Engine &E = *ctx->eng;
         std::unique_ptr<paddi::Series> s(E.upload(T, n));
@@ -1350,13 +1358,15 @@ paddi_status paddi_discords_series(paddi_ctx *ctx, paddi_series *const *series, 
 {
     if (!ctx || !series || nseries <= 0 || !count || (!out && cap > 0)) return PADDI_ERR_INVALID;
     std::vector<const paddi::Series *> ss;
+    std::vector<int64_t> ns;
     for (int32_t q = 0; q < nseries; q++) {
         if (!series[q] || series[q]->ctx != ctx ||
             !valid_lengths(series[q]->s->n, minL, maxL, step, top_k))
             return PADDI_ERR_INVALID;
         ss.push_back(series[q]->s);
+        ns.push_back(series[q]->s->n);
     }
-    if (maxL > paddi::MAXM) return PADDI_ERR_UNSUPPORTED;
+    if (maxL > paddi::MAXM || !rows_fit(ns.data(), nseries, minL)) return PADDI_ERR_UNSUPPORTED;
     paddi_opts o{};
     if (opts) o = *opts;
     if (o.mode < PADDI_MODE_PLAIN || o.mode > PADDI_MODE_LEARNED) return PADDI_ERR_INVALID;
@@ -1384,6 +1394,8 @@ paddi_status paddi_discords_batch(paddi_ctx *ctx, const float *const *T, const i
     if (!ctx || !T || !n || nseries <= 0 || !count) return PADDI_ERR_INVALID;
     for (int32_t q = 0; q < nseries; q++)
         if (!T[q] || !valid_lengths(n[q], minL, maxL, step, top_k)) return PADDI_ERR_INVALID;
+    // refused before any series is read or uploaded
+    if (maxL > paddi::MAXM || !rows_fit(n, nseries, minL)) return PADDI_ERR_UNSUPPORTED;
     std::vector<paddi_series *> ss;
     paddi_status st = PADDI_OK;
     for (int32_t q = 0; q < nseries && st == PADDI_OK; q++) {
@@ -1401,17 +1413,21 @@ paddi_status paddi_discords_batch(paddi_ctx *ctx, const float *const *T, const i
 paddi_status paddi_discords_timed(paddi_ctx *ctx, const float *T, int64_t n, int32_t minL,
                                   int32_t maxL, int32_t step, int32_t top_k,
                                   const paddi_opts *opts, paddi_discord *out, int64_t cap,
-                                  int64_t *count, paddi_length_stats *per_length)
+                                  int64_t *count, paddi_attempt_stats *per_attempt,
+                                  int64_t cap_attempts, int64_t *n_attempts)
 {
-    if (!ctx || !per_length || !valid_lengths(n, minL, maxL, step, top_k))
+    if (!ctx || !n_attempts || cap_attempts < 0 || (!per_attempt && cap_attempts > 0) ||
+        !valid_lengths(n, minL, maxL, step, top_k))
         return PADDI_ERR_INVALID;
-    const int nlen = (maxL - minL) / step + 1;
-    std::memset(per_length, 0, (size_t)nlen * sizeof(paddi_length_stats));
+    std::vector<paddi_attempt_stats> log;
     paddi_stats stats;
-    ctx->eng->per_len_ = per_length;
+    ctx->eng->att_log_ = &log;
     const paddi_status st = paddi_discords(ctx, T, n, minL, maxL, step, top_k, opts, out, cap,
                                            count, &stats);
-    ctx->eng->per_len_ = nullptr;
+    ctx->eng->att_log_ = nullptr;
+    *n_attempts = (int64_t)log.size();
+    const size_t w = std::min<size_t>(log.size(), (size_t)cap_attempts);
+    if (w) std::memcpy(per_attempt, log.data(), w * sizeof(paddi_attempt_stats));
     return st;
 }
 

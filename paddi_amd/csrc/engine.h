@@ -4,6 +4,7 @@
 
 #include <map>
 #include <memory>
+#include <stdexcept>
 #include <stdint.h>
 #include <string>
 #include <utility>
@@ -14,6 +15,22 @@
 #include "mlp.h"
 
 namespace paddi {
+
+// An error inside the engine: the status the C ABI returns for it and a message.  Every
+// failure the engine raises is one of these (or std::bad_alloc from the host allocator).
+struct HipError : std::runtime_error {
+    paddi_status st;
+    HipError(paddi_status s, const std::string &what) : std::runtime_error(what), st(s) {}
+};
+
+#define HIPC(expr)                                                                       \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::paddi::HipError(e_ == hipErrorOutOfMemory ? PADDI_ERR_NOMEM          \
+                                                              : PADDI_ERR_DEVICE,        \
+                                    std::string(#expr) + ": " + hipGetErrorString(e_));  \
+    } while (0)
 
 const char *status_str(paddi_status s);
 double delta_ed2(int m);
@@ -128,14 +145,16 @@ public:
     int prev_shift_ = 1;             // next r from previous discords at i, i-1 .. i-prev_shift
     bool bootstrap_ = true;          // learned first length: candidate lower bound on r
     bool debug_ = false;             // per-pass trace on stderr (PADDI_DEBUG=1)
-    paddi_length_stats *per_len_ = nullptr;  // per-length records of the running search
+    double wait_limit_s_ = 600.0;    // longest wait for queued device work (PADDI_WAIT_LIMIT_S)
+    std::vector<paddi_attempt_stats> *att_log_ = nullptr;  // per-attempt records, when kept
 
 private:
-    // per-length bookkeeping (per_len_): snapshot of the call's statistics at a length's start
-    void len_begin(paddi_stats *st);
-    void len_end(int j, int m, int discords, paddi_stats *st);
-    paddi_stats len_snap_{};
-    double len_t0_ = 0.0;
+    // per-attempt bookkeeping (att_log_): att_mark snapshots the call's statistics (at a
+    // length's start), att_end records what changed since as one attempt and snapshots again
+    void att_mark(paddi_stats *st);
+    void att_end(int m, int attempt, double r2, int discords, paddi_stats *st);
+    paddi_stats att_snap_{};
+    double att_t0_ = 0.0;
     void plain(Batch &B, int sid, int minL, int maxL, int step, int k, const paddi_opts &o,
                std::vector<paddi_discord> &out, paddi_stats *st);
     void fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opts &o,
@@ -211,3 +230,25 @@ struct paddi_series {
     paddi_ctx *ctx;
     paddi::Series *s;
 };
+
+// Body of a C entry point: its status, or the status of the error it raised (the message is
+// kept in ctx->last_error).  No exception leaves the library.
+template <class F>
+paddi_status guarded(paddi_ctx *ctx, F &&f)
+{
+    try {
+        return f();
+    } catch (const paddi::HipError &e) {
+        if (ctx) ctx->last_error = e.what();
+        return e.st;
+    } catch (const std::bad_alloc &) {
+        if (ctx) ctx->last_error = "host allocation failed";
+        return PADDI_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->last_error = e.what();
+        return PADDI_ERR_DEVICE;
+    } catch (...) {
+        if (ctx) ctx->last_error = "unknown error";
+        return PADDI_ERR_DEVICE;
+    }
+}

@@ -12,6 +12,12 @@ constexpr int EXACT_COLS = 4096; // columns per k_exact block (sub-tiles of 256 
 constexpr int EXACT_SLOTS = 64;  // result slots per candidate (blocks spread their minima)
 constexpr int64_t XPAD = 2048; // zero padding after each series (floats); the scans read
                                // up to n + 63
+// Rows of a call (every series' windows at the shortest length, block-padded) must stay below
+// 2^31: the candidate keys of k_block_best pack a global row in 32 bits, and the witness
+// tables, the scans' tile offsets and the pass counters hold rows and columns as int.
+constexpr int64_t MAX_ROWS = int64_t(1) << 31;
+// Row-space rows of a series of n points at shortest length minL (Engine::build_batch).
+inline int64_t series_rows(int64_t n, int minL) { return (n - minL + 1 + 63) / 64 * 64 + 128; }
 
 // One series of a batch, as the kernels see it.  The batch's table is uploaded once (k_derive,
 // k_exact and the feature kernels take N = n - m + 1 from n and m); a pass carries its own

@@ -20,11 +20,6 @@ namespace paddi {
 
 namespace {
 const char MAGIC[8] = {'P', 'D', 'R', 'M', 'L', 'P', '0', '2'};
-
-#define HIPL(expr)                                                                  \
-    do {                                                                            \
-        if ((expr) != hipSuccess) throw std::runtime_error(#expr);                  \
-    } while (0)
 }  // namespace
 
 const char *default_model_path()
@@ -93,7 +88,7 @@ void Engine::features(Batch &B, int m, int k, std::vector<float> &X)
     feat_.reserve((size_t)ns * 33 * 4);
     float *hist = feat_.as<float>();
     float *gstd = hist + (size_t)ns * 32;
-    HIPL(hipMemsetAsync(hist, 0, (size_t)ns * 33 * 4, stream_));
+    HIPC(hipMemsetAsync(hist, 0, (size_t)ns * 33 * 4, stream_));
     const int64_t waves = (int64_t)ns * NPAIR;
     hipLaunchKernelGGL(k_feat_pairs, dim3((unsigned)((waves * 64 + 255) / 256)), dim3(256), 0,
                        stream_, B.x.as<float>(), B.ser.as<SerDev>(), st64_.as<double2>(), ns, m,
@@ -101,8 +96,8 @@ void Engine::features(Batch &B, int m, int k, std::vector<float> &X)
     hipLaunchKernelGGL(k_feat_std, dim3((unsigned)((ns + 63) / 64)), dim3(64), 0, stream_,
                        B.ser.as<SerDev>(), B.P.as<double2>(), B.refs.as<double>(), ns, gstd);
     std::vector<float> h((size_t)ns * 33);
-    HIPL(hipMemcpyAsync(h.data(), hist, h.size() * 4, hipMemcpyDeviceToHost, stream_));
-    HIPL(hipStreamSynchronize(stream_));
+    HIPC(hipMemcpyAsync(h.data(), hist, h.size() * 4, hipMemcpyDeviceToHost, stream_));
+    HIPC(hipStreamSynchronize(stream_));
     X.assign((size_t)ns * NFEAT, 0.f);
     for (int q = 0; q < ns; q++) {
         float *r = X.data() + (size_t)q * NFEAT;
@@ -120,14 +115,14 @@ void Engine::mlp_forward(const float *prm, const float *fmu, const float *fsd, c
     mlpbuf_.reserve((size_t)(NPARAM + 2 * NFEAT + (size_t)Bn * (NFEAT + 1)) * 4);
     float *d = mlpbuf_.as<float>();
     float *dp = d, *dmu = dp + NPARAM, *dsd = dmu + NFEAT, *dX = dsd + NFEAT, *dY = dX + (size_t)Bn * NFEAT;
-    HIPL(hipMemcpyAsync(dp, prm, NPARAM * 4, hipMemcpyHostToDevice, stream_));
-    HIPL(hipMemcpyAsync(dmu, fmu, NFEAT * 4, hipMemcpyHostToDevice, stream_));
-    HIPL(hipMemcpyAsync(dsd, fsd, NFEAT * 4, hipMemcpyHostToDevice, stream_));
-    HIPL(hipMemcpyAsync(dX, X, (size_t)Bn * NFEAT * 4, hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(dp, prm, NPARAM * 4, hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(dmu, fmu, NFEAT * 4, hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(dsd, fsd, NFEAT * 4, hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(dX, X, (size_t)Bn * NFEAT * 4, hipMemcpyHostToDevice, stream_));
     hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)Bn), dim3(HID), 0, stream_, dX, Bn, dp, dmu,
                        dsd, (float *)nullptr, (float *)nullptr, dY);
-    HIPL(hipMemcpyAsync(Y, dY, (size_t)Bn * 4, hipMemcpyDeviceToHost, stream_));
-    HIPL(hipStreamSynchronize(stream_));
+    HIPC(hipMemcpyAsync(Y, dY, (size_t)Bn * 4, hipMemcpyDeviceToHost, stream_));
+    HIPC(hipStreamSynchronize(stream_));
 }
 
 // Train (Adam, full batch) or, with steps == 0, only return the gradient and loss.
@@ -145,12 +140,12 @@ void Engine::mlp_train(float *prm, const float *fmu, const float *fsd, const flo
     float *dmu = M2 + NPARAM, *dsd = dmu + NFEAT, *dX = dsd + NFEAT, *dT = dX + nb * NFEAT;
     float *H1 = dT + nb, *H2 = H1 + nb * HID, *D1 = H2 + nb * HID, *D2 = D1 + nb * HID;
     float *Y = D2 + nb * HID, *DY = Y + nb, *loss = DY + nb;
-    HIPL(hipMemcpyAsync(dp, prm, NPARAM * 4, hipMemcpyHostToDevice, stream_));
-    HIPL(hipMemsetAsync(M1, 0, 2 * NPARAM * 4, stream_));
-    HIPL(hipMemcpyAsync(dmu, fmu, NFEAT * 4, hipMemcpyHostToDevice, stream_));
-    HIPL(hipMemcpyAsync(dsd, fsd, NFEAT * 4, hipMemcpyHostToDevice, stream_));
-    HIPL(hipMemcpyAsync(dX, X, nb * NFEAT * 4, hipMemcpyHostToDevice, stream_));
-    HIPL(hipMemcpyAsync(dT, T, nb * 4, hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(dp, prm, NPARAM * 4, hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemsetAsync(M1, 0, 2 * NPARAM * 4, stream_));
+    HIPC(hipMemcpyAsync(dmu, fmu, NFEAT * 4, hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(dsd, fsd, NFEAT * 4, hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(dX, X, nb * NFEAT * 4, hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(dT, T, nb * 4, hipMemcpyHostToDevice, stream_));
     const int S = steps > 0 ? steps : 1;
     const float b1 = 0.9f, b2 = 0.999f;
     float p1 = 1.f, p2 = 1.f;
@@ -169,12 +164,12 @@ void Engine::mlp_train(float *prm, const float *fmu, const float *fsd, const flo
             hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, stream_, dp, G,
                                M1, M2, lr, b1, b2, 1e-8f, 1.f - p1, 1.f - p2, wd);
         }
-        if (losses) HIPL(hipMemcpyAsync(&lh[(size_t)it], loss, 4, hipMemcpyDeviceToHost, stream_));
+        if (losses) HIPC(hipMemcpyAsync(&lh[(size_t)it], loss, 4, hipMemcpyDeviceToHost, stream_));
     }
-    HIPL(hipGetLastError());
-    if (grad_out) HIPL(hipMemcpyAsync(grad_out, G, NPARAM * 4, hipMemcpyDeviceToHost, stream_));
-    if (steps > 0) HIPL(hipMemcpyAsync(prm, dp, NPARAM * 4, hipMemcpyDeviceToHost, stream_));
-    HIPL(hipStreamSynchronize(stream_));
+    HIPC(hipGetLastError());
+    if (grad_out) HIPC(hipMemcpyAsync(grad_out, G, NPARAM * 4, hipMemcpyDeviceToHost, stream_));
+    if (steps > 0) HIPC(hipMemcpyAsync(prm, dp, NPARAM * 4, hipMemcpyDeviceToHost, stream_));
+    HIPC(hipStreamSynchronize(stream_));
     if (losses) std::memcpy(losses, lh.data(), (size_t)S * 4);
 }
 
@@ -206,17 +201,6 @@ void Engine::set_model_path(const char *p)
 // ------------------------------------------------------------------------------------------
 // C ABI of the learned part (no counterpart in the reference).
 
-template <class F>
-static paddi_status guarded_l(paddi_ctx *ctx, F &&f)
-{
-    try {
-        return f();
-    } catch (const std::exception &e) {
-        if (ctx) ctx->last_error = e.what();
-        return PADDI_ERR_DEVICE;
-    }
-}
-
 extern "C" {
 
 paddi_status paddi_mlp_sizes(int32_t *nfeat, int32_t *hidden, int32_t *nparam)
@@ -234,12 +218,15 @@ paddi_status paddi_features(paddi_ctx *ctx, paddi_series *const *series, int32_t
     if (!ctx || !series || nseries <= 0 || !out || m < 4 || m > paddi::MAXM || top_k < 1)
         return PADDI_ERR_INVALID;
     std::vector<const paddi::Series *> ss;
+    int64_t rows = 0;
     for (int32_t q = 0; q < nseries; q++) {
         if (!series[q] || series[q]->ctx != ctx || series[q]->s->n < 3 * (int64_t)m + 8)
             return PADDI_ERR_INVALID;
         ss.push_back(series[q]->s);
+        rows += paddi::series_rows(series[q]->s->n, m);
     }
-    return guarded_l(ctx, [&] {
+    if (rows >= paddi::MAX_ROWS) return PADDI_ERR_UNSUPPORTED;
+    return guarded(ctx, [&] {
         paddi::Engine &E = *ctx->eng;
         paddi::Batch B;
         E.build_batch(B, ss, m);
@@ -256,7 +243,7 @@ paddi_status paddi_mlp_forward(paddi_ctx *ctx, const float *params, const float 
                                const float *fsd, const float *X, int32_t B, float *Y)
 {
     if (!ctx || !params || !fmu || !fsd || !X || !Y || B <= 0) return PADDI_ERR_INVALID;
-    return guarded_l(ctx, [&] {
+    return guarded(ctx, [&] {
         ctx->eng->mlp_forward(params, fmu, fsd, X, B, Y);
         return PADDI_OK;
     });
@@ -268,7 +255,7 @@ paddi_status paddi_mlp_gradient(paddi_ctx *ctx, const float *params, const float
 {
     if (!ctx || !params || !fmu || !fsd || !X || !T || !grad || !loss || B <= 0)
         return PADDI_ERR_INVALID;
-    return guarded_l(ctx, [&] {
+    return guarded(ctx, [&] {
         std::vector<float> p(params, params + paddi::NPARAM);
         ctx->eng->mlp_train(p.data(), fmu, fsd, X, T, B, 0, 0.f, 0.f, loss, grad);
         return PADDI_OK;
@@ -281,7 +268,7 @@ paddi_status paddi_mlp_train(paddi_ctx *ctx, float *params, const float *fmu, co
 {
     if (!ctx || !params || !fmu || !fsd || !X || !T || B <= 0 || steps <= 0 || !(lr > 0.f))
         return PADDI_ERR_INVALID;
-    return guarded_l(ctx, [&] {
+    return guarded(ctx, [&] {
         ctx->eng->mlp_train(params, fmu, fsd, X, T, B, steps, lr, weight_decay, losses, nullptr);
         return PADDI_OK;
     });

@@ -15,9 +15,11 @@
 //   discords.csv (src/IOdata.hpp:111-137): "m;top-1 discord idx;top-1 discord nnDist", then
 //     one row per length (top_k rows with --top-k, in rank order); nnDist is the squared
 //     z-normalised distance, as the reference writes it.
-//   times.csv (src/IOdata.hpp:140-177): the reference's 14 columns; one GPU, so the MPI
-//     and global-phase columns are 0.  Phase 1 = statistics + scans, phase 2 = the
-//     double-precision refinement, "# Candidates" = rows left by the scans.
+//   times.csv (src/IOdata.hpp:140-177): the reference's 14 columns, one row per (m, r)
+//     attempt as src/main.cu:144,176,219 add them; one GPU, so the MPI and global-phase
+//     columns are 0.  Phase 1 = statistics + scans, phase 2 = the double-precision
+//     refinement, "# Candidates" = rows left by the scans, "# Discords" = discords the
+//     attempt accepted.
 //   --range-profile (src/IOdata.hpp:90-108): "m;v_0;..;v_{N-1}" at squared range R2, v_i the
 //     NN distance of range discord i and -FLT_MAX for pruned rows (the reference's nnDist
 //     after refinement, src/PD3_kernels.cuh:375).
@@ -25,6 +27,7 @@
 #include <cfloat>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <fstream>
 #include <iostream>
@@ -103,13 +106,16 @@ int main(int argc, char **argv)
     if (st != PADDI_OK) return fail(nullptr, st, "paddi_create");
     const int nlen = (maxL - minL) / step + 1;
     std::vector<paddi_discord> out((size_t)nlen * (size_t)top_k);
-    std::vector<paddi_length_stats> pl((size_t)nlen);
-    int64_t count = 0;
+    // at most max_attempts (64) + 1 attempts per length
+    std::vector<paddi_attempt_stats> pa((size_t)nlen * 65);
+    int64_t count = 0, natt = 0;
     paddi_opts opts{};
     opts.mode = mode;
     st = paddi_discords_timed(ctx, x.data(), (int64_t)x.size(), minL, maxL, step, top_k, &opts,
-                              out.data(), (int64_t)out.size(), &count, pl.data());
+                              out.data(), (int64_t)out.size(), &count, pa.data(),
+                              (int64_t)pa.size(), &natt);
     if (st != PADDI_OK) return fail(ctx, st, "paddi_discords_timed");
+    pa.resize(std::min<size_t>(pa.size(), (size_t)natt));
 
     {
         std::ofstream f(disc_path, std::ios::app);
@@ -128,7 +134,7 @@ int main(int argc, char **argv)
             "max MatrixMultiply among GPUs (kernel), ms", "MPI exchange D, ms", "# Discords",
             "Total time PALMAD (rank = 0), ms"};
         for (int c = 0; c < 14; c++) f << cols[c] << (c < 13 ? ";" : "\n");
-        for (const paddi_length_stats &r : pl)
+        for (const paddi_attempt_stats &r : pa)
             f << r.m << ";" << r.ms_derive + r.ms_scan << ";" << r.ms_derive << ";" << r.ms_scan
               << ";" << r.ms_exact << ";" << r.ms_exact << ";" << r.survivors << ";0;0;0;0;0;"
               << r.discords << ";" << r.ms_total << "\n";

@@ -61,12 +61,26 @@ def test_cli_matches_library_and_formats(ctx, tmp_path, mode):
     assert [(int(a), int(b)) for a, b, _ in rows] == [(g.length, g.index) for g in got]
     for (_, _, d), g in zip(rows, got):
         assert float(d) == pytest.approx(g.nn_dist2, rel=1e-5)   # 6 significant digits
+    # one times-table row per (m, r) attempt, as src/main.cu:144,176,219 add them: the
+    # attempts the library reports for the same search, lengths ascending, and the last
+    # attempt of each length accepting its two discords
+    _, st = ctx.discords(xs, 64, 67, 1, 2, mode=mode, with_stats=True)
     t = [l.split(";") for l in (tmp_path / "t.csv").read_text().splitlines()]
     assert len(t[0]) == 14 and t[0][0] == "m" and t[0][-1].startswith("Total time")
-    assert [int(row[0]) for row in t[1:]] == [64, 65, 66, 67]
-    for row in t[1:]:
-        assert len(row) == 14
-        assert int(row[12]) == 2 and int(row[6]) >= 2 and float(row[13]) > 0
+    rows_t = t[1:]
+    assert len(rows_t) == st["attempts"]
+    ms = [int(row[0]) for row in rows_t]
+    assert ms == sorted(ms) and sorted(set(ms)) == [64, 65, 66, 67]
+    for q, row in enumerate(rows_t):
+        assert len(row) == 14 and float(row[13]) > 0
+        last = q + 1 == len(rows_t) or int(rows_t[q + 1][0]) != int(row[0])
+        if last:
+            assert int(row[12]) == 2 and int(row[6]) >= 2
+    if mode == "plain":
+        # the plain form runs the oracle driver's r-schedule: the same attempts per length
+        from oracle import oracle as O
+        ref = O.palmad(xs, 64, 67, 1, 2)
+        assert [ms.count(r["m"]) for r in ref] == [r["attempts"] for r in ref]
     p = (tmp_path / "p.csv").read_text().strip().split(";")
     flag, nn = ctx.range_discords(xs, 64, 60.0)
     assert int(p[0]) == 64 and len(p) == 1 + len(flag)

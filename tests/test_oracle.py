@@ -228,3 +228,55 @@ def test_mutual_neighbours_tie_to_lower_index():
     assert nn[700] == nn[2200]  # bitwise tie
     if d[0] == nn[700]:
         assert idx[0] == 700
+
+
+# --- the column-blocked scan and the certificate scan order ------------------------------
+
+def _scalar_profile(x, m):
+    """NN ED^2 of every row by the one-pair-at-a-time loop of cpu_kernels.cpp:47-75 in
+    double precision, in plain Python floats (IEEE double: each product rounded, then each
+    sum, ascending k), on the oracle's own window statistics."""
+    mu, sd = O.stats(x, m)
+    xs = [float(v) for v in np.asarray(x, np.float32)]
+    N = len(xs) - m + 1
+    out = []
+    for i in range(N):
+        best = float(np.finfo(np.float64).max)
+        for j in range(N):
+            if abs(i - 1 - j) < m:
+                continue
+            dot = 0.0
+            for k in range(m):
+                dot += xs[i + k] * xs[j + k]
+            den = float(m) * (float(sd[i]) * float(sd[j]))
+            ed = 2.0 * m if not den > 0.0 else \
+                2.0 * m * (1.0 - (dot - m * (float(mu[i]) * float(mu[j]))) / den)
+            best = min(best, ed)
+        out.append(best)
+    return np.array(out)
+
+
+def test_blocked_scan_is_bitwise_the_scalar_loop():
+    """The scan evaluates eight columns at a time; every pair value, and so every minimum,
+    must be bitwise the scalar loop's (N = 293 rows: blocks and a ragged tail)."""
+    x = synth.ar2_mix(300, 31)
+    m = 8
+    assert np.array_equal(O.profile(x, m), _scalar_profile(x, m))
+
+
+@pytest.mark.parametrize("kind", ["rw", "ecg", "ar2"])
+def test_hinted_scan_order_gives_the_same_range_set(kind):
+    """oracle_find_discords_hint_f64 changes only the order in which a row meets its columns:
+    the same flags, and for the rows flagged bitwise the same NN distances, as the
+    reference-order scan, at ranges from 0 to above every NN distance."""
+    x, m = {"rw": (synth.rw_planted(6000, 5, 2, 64, 12.0), 64),
+            "ecg": (synth.ecg_like(5000, 6, 1, 1), 80),
+            "ar2": (synth.ar2_mix(4000, 7), 40)}[kind]
+    nn_ref = O.profile(x, m)
+    for q in (0.0, 0.5, 0.9, 0.999, 1.0):
+        r2 = float(np.quantile(nn_ref, q)) if q > 0 else 0.0
+        c1, n1 = O.range_discords(x, m, r2)
+        c2, n2 = O.range_discords(x, m, r2, hint=True)
+        assert np.array_equal(c1, c2), (kind, q)
+        assert np.array_equal(n1[c1 == 1], n2[c2 == 1]), (kind, q)
+        assert np.array_equal(n1[c1 == 1], nn_ref[c1 == 1]), (kind, q)
