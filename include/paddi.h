@@ -135,6 +135,21 @@ paddi_status paddi_compute_statistics(paddi_ctx *ctx, const float *T, int64_t n,
 paddi_status paddi_range_discords(paddi_ctx *ctx, const float *T, int64_t n, int32_t m, float r2,
                                   int32_t *is_discord, float *nn_dist2);
 
+/* paddi_range_discords without any allocation: no hipMalloc, hipFree or pinned-host allocation
+ * during the call.  No counterpart in the reference; for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid allocation).  Same
+ * arguments, results and errors as paddi_range_discords; every device buffer of the call is a
+ * slice of a file-scope device array of PADDI_NOALLOC_SCRATCH_BYTES that each GPU holds from
+ * the library's first use on it until the process ends, and the pinned staging the call
+ * uses is allocated by paddi_create.  Series longer than PADDI_NOALLOC_MAX_POINTS return
+ * PADDI_ERR_UNSUPPORTED.  Every call on one GPU shares that GPU's array (whatever context it
+ * comes from) and leaves its contents there: calls that overlap on one GPU, from two
+ * contexts or two threads, race. */
+#define PADDI_NOALLOC_SCRATCH_BYTES (32u << 20)
+#define PADDI_NOALLOC_MAX_POINTS 300000
+paddi_status paddi_range_discords_noalloc(paddi_ctx *ctx, const float *T, int64_t n, int32_t m,
+                                          float r2, int32_t *is_discord, float *nn_dist2);
+
 /* One (m, r) attempt of a search: a row of the reference's times table (src/IOdata.hpp:140-177),
  * which src/main.cu:144,176,219 fills once per do_PALMAD_one_length call.  The phases of one
  * GPU (statistics, scans, exact refinement) and the counts.  In the batched forms an attempt is

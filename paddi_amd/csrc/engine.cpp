@@ -63,9 +63,20 @@ const char *status_str(paddi_status s)
     }
 }
 
+// Nonzero while a no-allocation call runs on this thread (ScratchScope): every buffer it needs
+// is bound beforehand, so a buffer that would have to grow is an error, not an allocation.
+static thread_local int t_noalloc = 0;
+
+static void refuse_growth(const char *what)
+{
+    throw HipError(PADDI_ERR_UNSUPPORTED,
+                   std::string("no-allocation call: ") + what + " does not fit its reserved space");
+}
+
 void DevBuf::reserve(size_t nbytes)
 {
     if (nbytes <= bytes) return;
+    if (borrowed || t_noalloc) refuse_growth("a device buffer");
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
@@ -74,10 +85,70 @@ void DevBuf::reserve(size_t nbytes)
 }
 void DevBuf::release()
 {
-    if (p) (void)hipFree(p);
+    if (p && !borrowed) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    borrowed = false;
 }
+
+// Binds device buffers to consecutive 256-byte aligned slices of the scratch array for one
+// no-allocation call and gives each back its own memory when the call ends.  Binding checks
+// the slices against SCRATCH_BYTES before any device work is queued.
+class ScratchScope {
+public:
+    explicit ScratchScope(Engine &E) : E_(E), base_((char *)E.scratch_)
+    {
+        if (!base_) throw HipError(PADDI_ERR_DEVICE, "no scratch array on this device");
+        t_noalloc++;
+        saved_graph_ = E.use_graph_;
+        E.use_graph_ = false;   // a graph would bake these slices and allocate to instantiate
+    }
+    ~ScratchScope()
+    {
+        for (size_t q = saved_.size(); q-- > 0;) {
+            Saved &v = saved_[q];
+            v.b->p = v.p;
+            v.b->bytes = v.bytes;
+            v.b->borrowed = v.borrowed;
+        }
+        E_.use_graph_ = saved_graph_;
+        t_noalloc--;
+    }
+    ScratchScope(const ScratchScope &) = delete;
+    ScratchScope &operator=(const ScratchScope &) = delete;
+    void bind(DevBuf &b, size_t bytes)
+    {
+        off_ = (off_ + 255) & ~(size_t)255;
+        if (off_ + bytes > SCRATCH_BYTES)
+            throw HipError(PADDI_ERR_UNSUPPORTED,
+                           "no-allocation call: the series needs more than the " +
+                               std::to_string(SCRATCH_BYTES >> 20) + " MiB scratch array");
+        place(b, base_ + off_, bytes);
+        off_ += bytes;
+    }
+    // b shares a's slice (buffers a call never uses together)
+    void share(DevBuf &b, const DevBuf &a) { place(b, a.p, a.bytes); }
+
+private:
+    struct Saved {
+        DevBuf *b;
+        void *p;
+        size_t bytes;
+        bool borrowed;
+    };
+    void place(DevBuf &b, void *p, size_t bytes)
+    {
+        saved_.push_back(Saved{&b, b.p, b.bytes, b.borrowed});
+        b.p = p;
+        b.bytes = bytes;
+        b.borrowed = true;
+    }
+    Engine &E_;
+    char *base_;
+    size_t off_ = 0;
+    bool saved_graph_ = false;
+    std::vector<Saved> saved_;
+};
 
 // Rounding of a scan bound's conversion to a squared distance d (the bound's last addition
 // and scaling by invn_i in the scan, then 2m (1 - c) in k_survivors, all in single
@@ -97,6 +168,16 @@ double delta_ed2(int m)
 }
 
 static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// Bytes of one exact launch block (Engine::exact) for nc candidates: EXACT_SLOTS result keys
+// per candidate (twice with a second length), the candidates' rows, and per group (at most
+// one per candidate, plus one closing entry per launch of one group size) its first block,
+// first candidate and series.
+static size_t exact_arg_bytes(size_t nc, bool two_lengths)
+{
+    const size_t ngx = nc + MAXU;
+    return nc * EXACT_SLOTS * 8 * (two_lengths ? 2 : 1) + nc * 8 + ngx * 16;
+}
 static inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
 // ------------------------------------------------------------------------------------------
@@ -209,6 +290,13 @@ Engine::Engine(int device) : device_(device)
                        hipHostMallocDefault));
     HIPC(hipEventCreateWithFlags(&ser_ev_, hipEventDisableTiming));
     HIPC(hipEventCreateWithFlags(&sync_ev_, hipEventDisableTiming));
+    // Staging the no-allocation call uses, sized here: a one-series pass's parameters and
+    // series table, and the arguments and results of one exact launch of
+    // EXACT_NOALLOC_CHUNK candidates (exact_arg_bytes).
+    ensure_pinned(4096, 4096);
+    pin_x_bytes_ = exact_arg_bytes(EXACT_NOALLOC_CHUNK, false);
+    HIPC(hipHostMalloc(&pin_x_, pin_x_bytes_, hipHostMallocDefault));
+    scratch_ = scratch_setup(device);
     const char *dbg = std::getenv("PADDI_DEBUG");
     debug_ = dbg && dbg[0] == '1';
     const char *lim = std::getenv("PADDI_WAIT_LIMIT_S");
@@ -238,6 +326,8 @@ void Engine::drop_graphs()
 // Pinned staging grows with the batch; a graph that copies from the old buffer is dropped.
 void Engine::ensure_pinned(size_t par_bytes, size_t ser_bytes)
 {
+    if ((par_bytes > pin_par_bytes_ || ser_bytes > pin_ser_bytes_) && t_noalloc)
+        refuse_growth("the pinned parameter staging");
     if (par_bytes > pin_par_bytes_) {
         HIPC(hipStreamSynchronize(stream_));
         drop_graphs();
@@ -268,7 +358,8 @@ Series *Engine::upload(const float *T, int64_t n)
     return s.release();
 }
 
-void Engine::build_batch(Batch &B, const std::vector<const Series *> &ss, int minL)
+void Engine::build_batch(Batch &B, const std::vector<const Series *> &ss, int minL,
+                         const float *const *host)
 {
     derive_m2_ = 0;  // statistics computed ahead belong to the previous row layout
     B.nser = (int)ss.size();
@@ -300,8 +391,9 @@ void Engine::build_batch(Batch &B, const std::vector<const Series *> &ss, int mi
     B.pser.reserve((size_t)po * sizeof(int));
     HIPC(hipMemsetAsync(B.x.p, 0, (size_t)xo * sizeof(float), stream_));
     for (size_t q = 0; q < ss.size(); q++)
-        HIPC(hipMemcpyAsync(B.x.as<float>() + B.xoff[q], ss[q]->x.p, (size_t)B.n[q] * 4,
-                            hipMemcpyDeviceToDevice, stream_));
+        HIPC(hipMemcpyAsync(B.x.as<float>() + B.xoff[q], host ? (const void *)host[q] : ss[q]->x.p,
+                            (size_t)B.n[q] * 4,
+                            host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, stream_));
     B.bser_host.assign((size_t)B.nblocks, -1);
     std::vector<int> pser((size_t)po, 0);
     for (size_t q = 0; q < ss.size(); q++) {
@@ -724,6 +816,19 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
     out.assign(req.size(), 0.0);
     if (out2) out2->assign(req.size(), DBL_MAX);
     if (req.empty()) return;
+    // a no-allocation call refines at most EXACT_NOALLOC_CHUNK candidates per launch block
+    if (t_noalloc && req.size() > EXACT_NOALLOC_CHUNK) {
+        for (size_t a = 0; a < req.size(); a += EXACT_NOALLOC_CHUNK) {
+            const size_t e = std::min(req.size(), a + EXACT_NOALLOC_CHUNK);
+            const std::vector<std::pair<int, int64_t>> sub(req.begin() + (std::ptrdiff_t)a,
+                                                           req.begin() + (std::ptrdiff_t)e);
+            std::vector<double> o, o2;
+            exact(B, m, sub, o, st, m2, out2 ? &o2 : nullptr);
+            std::copy(o.begin(), o.end(), out.begin() + (std::ptrdiff_t)a);
+            if (out2) std::copy(o2.begin(), o2.end(), out2->begin() + (std::ptrdiff_t)a);
+        }
+        return;
+    }
     // the second length needs its statistics from the last derive
     if (!out2 || m2 <= m || m2 > m + EXACT_EXT || m2 != derive_m2_) m2 = 0;
     // the series table on the device is derive(m)'s: every series at N = n - m + 1
@@ -788,6 +893,7 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
     const size_t o_cand = nkeys * 8, o_goff = o_cand + nc * 8, o_gfirst = o_goff + ngx * 8,
                  o_gser = o_gfirst + ngx * 4, xbytes = o_gser + ngx * 4;
     if (xbytes > pin_x_bytes_) {
+        if (t_noalloc) refuse_growth("the pinned exact-refinement staging");
         if (pin_x_) HIPC(hipHostFree(pin_x_));
         pin_x_ = nullptr;
         pin_x_bytes_ = 0;
@@ -1184,6 +1290,74 @@ void Engine::run(const std::vector<const Series *> &ss, int minL, int maxL, int 
     collect_derive();
 }
 
+void Engine::range(Batch &B, int m, float r2, int32_t *is_discord, float *nn_dist2)
+{
+    const int64_t N = B.n[0] - m + 1;
+    std::vector<std::vector<Cand>> surv;
+    pass(B, m, {(double)r2}, false, surv, nullptr);
+    std::fill(is_discord, is_discord + N, 0);
+    std::fill(nn_dist2, nn_dist2 + N, 3.4028235e38f);
+    std::vector<std::pair<int, int64_t>> amb;
+    for (const Cand &c : surv[0]) {
+        if (c.a >= 3.0e38f || (double)c.alo - conv_ed2(m, c.alo) > (double)r2) {
+            is_discord[c.i] = 1;
+            nn_dist2[c.i] = c.a;
+        } else if ((double)c.ahi + conv_ed2(m, c.ahi) > (double)r2) {
+            amb.emplace_back(0, c.i);
+        }
+    }
+    if (!amb.empty()) {
+        std::vector<double> v;
+        exact(B, m, amb, v, nullptr);
+        for (size_t q = 0; q < amb.size(); q++)
+            if (v[q] > (double)r2) {
+                is_discord[amb[q].second] = 1;
+                nn_dist2[amb[q].second] = (float)v[q];
+            }
+    }
+}
+
+// Every device buffer of the call is a slice of the scratch array, bound before any work is
+// queued (ScratchScope), the series is copied from the caller's memory straight into its
+// slice, and the pinned staging was sized when the context was created.
+void Engine::range_noalloc(const float *T, int64_t n, int m, float r2, int32_t *is_discord,
+                           float *nn_dist2)
+{
+    HIPC(hipSetDevice(device_));
+    Series s;   // describes the series (its n); the data come from T
+    s.n = n;
+    Batch B;    // outlives the scope, which gives B's buffers back on exit
+    ScratchScope S(*this);
+    const int64_t xo = round_up(n + XPAD, 64), npb = (n + PB - 1) / PB + 1;
+    const int64_t rows = series_rows(n, m), nblocks = rows / 64;
+    S.bind(B.x, (size_t)xo * 4);
+    S.bind(B.P, (size_t)npb * (PB + 1) * sizeof(double2));
+    S.bind(B.refs, (size_t)npb * sizeof(double));
+    S.bind(B.ser, sizeof(SerDev) * 4);
+    S.bind(B.bser, (size_t)nblocks * 4);
+    S.bind(B.pser, (size_t)npb * 4);
+    S.bind(flag_, 64);
+    S.bind(colv_, (size_t)rows * sizeof(float4));
+    S.bind(st64_, (size_t)rows * sizeof(double2));
+    S.bind(tmax_, (size_t)nblocks * sizeof(float2));
+    S.bind(rho_, (size_t)nblocks * sizeof(float));
+    S.bind(alive_, (size_t)nblocks * 8);
+    S.bind(best_, (size_t)rows * 4);
+    S.bind(besth_, (size_t)rows * 4);
+    S.bind(bestl_, (size_t)rows * 4);
+    S.bind(blist_, (size_t)nblocks * 4);
+    S.bind(pout_, PASS_OUT_HEAD + (size_t)rows * sizeof(Surv));
+    S.bind(wit_[0], (size_t)rows * 4);   // a pass without witnesses never touches them
+    S.share(wit_[1], wit_[0]);
+    S.bind(par_, sizeof(PassDev) + sizeof(SerDev));
+    S.bind(xarg_, exact_arg_bytes(EXACT_NOALLOC_CHUNK, false));
+    const float *const src[1] = {T};
+    build_batch(B, {&s}, m, src);
+    prefix(B);
+    derive(B, m, nullptr);
+    range(B, m, r2, is_discord, nn_dist2);
+}
+
 }  // namespace paddi
 
 // ------------------------------------------------------------------------------------------
@@ -1324,29 +1498,19 @@ paddi_status paddi_range_discords(paddi_ctx *ctx, const float *T, int64_t n, int
         E.build_batch(B, {s.get()}, m);
         E.prefix(B);
         E.derive(B, m, nullptr);
-        const int64_t N = n - m + 1;
-        std::vector<std::vector<paddi::Cand>> surv;
-        E.pass(B, m, {(double)r2}, false, surv, nullptr);
-        std::fill(is_discord, is_discord + N, 0);
-        std::fill(nn_dist2, nn_dist2 + N, 3.4028235e38f);
-        std::vector<std::pair<int, int64_t>> amb;
-        for (const paddi::Cand &c : surv[0]) {
-            if (c.a >= 3.0e38f || (double)c.alo - paddi::conv_ed2(m, c.alo) > (double)r2) {
-                is_discord[c.i] = 1;
-                nn_dist2[c.i] = c.a;
-            } else if ((double)c.ahi + paddi::conv_ed2(m, c.ahi) > (double)r2) {
-                amb.emplace_back(0, c.i);
-            }
-        }
-        if (!amb.empty()) {
-            std::vector<double> v;
-            E.exact(B, m, amb, v, nullptr);
-            for (size_t q = 0; q < amb.size(); q++)
-                if (v[q] > (double)r2) {
-                    is_discord[amb[q].second] = 1;
-                    nn_dist2[amb[q].second] = (float)v[q];
-                }
-        }
+        E.range(B, m, r2, is_discord, nn_dist2);
+        return PADDI_OK;
+    });
+}
+
+paddi_status paddi_range_discords_noalloc(paddi_ctx *ctx, const float *T, int64_t n, int32_t m,
+                                          float r2, int32_t *is_discord, float *nn_dist2)
+{
+    if (!ctx || !T || !is_discord || !nn_dist2 || m < 4 || n < (int64_t)m + 1 || !(r2 >= 0.f))
+        return PADDI_ERR_INVALID;
+    if (m > paddi::MAXM || n > PADDI_NOALLOC_MAX_POINTS) return PADDI_ERR_UNSUPPORTED;
+    return guarded(ctx, [&] {
+        ctx->eng->range_noalloc(T, n, m, r2, is_discord, nn_dist2);
         return PADDI_OK;
     });
 }

@@ -37,13 +37,25 @@ double delta_ed2(int m);
 double conv_ed2(int m, double d);
 
 constexpr int64_t SCAP = 8192;   // survivors returned with a pass's counters (more: a 2nd copy)
+
+// No-allocation range call (paddi_range_discords_noalloc): its device buffers are slices of a
+// file-scope device array of SCRATCH_BYTES (scratch.hip), its exact refinements run in
+// launches of at most EXACT_NOALLOC_CHUNK candidates (their arguments fit a slice and the
+// pinned staging sized when the context is created).
+constexpr size_t SCRATCH_BYTES = PADDI_NOALLOC_SCRATCH_BYTES;
+constexpr size_t EXACT_NOALLOC_CHUNK = 4096;
+// Address of this device's scratch array (hipGetSymbolAddress, taken once per device; the
+// device must be current).
+void *scratch_setup(int device);
 constexpr int64_t BOUND_WINDOW = 1024;  // columns behind the first-length candidates' bounds
 constexpr int BOUND_CANDIDATES = 16;    // candidates per series refined for the first r
 
-// Grow-only device buffer.
+// Grow-only device buffer.  A borrowed buffer is a slice of memory the engine does not own
+// (the scratch array, ScratchScope): it is never freed and never grows.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;
     void reserve(size_t nbytes);
     void release();
     DevBuf() = default;
@@ -55,6 +67,7 @@ struct DevBuf {
     {
         std::swap(p, o.p);
         std::swap(bytes, o.bytes);
+        std::swap(borrowed, o.borrowed);
     }
 };
 
@@ -109,7 +122,9 @@ public:
     Engine &operator=(const Engine &) = delete;
 
     Series *upload(const float *T, int64_t n);
-    void build_batch(Batch &B, const std::vector<const Series *> &ss, int minL);
+    // host: a host copy of each series to read instead of the device one (null: device)
+    void build_batch(Batch &B, const std::vector<const Series *> &ss, int minL,
+                     const float *const *host = nullptr);
     void prefix(Batch &B);
     // Window statistics at m (and at m2 > 0 for k_exact's second length).
     void derive(Batch &B, int m, paddi_stats *st, int m2 = 0);
@@ -133,6 +148,12 @@ public:
 
     void run(const std::vector<const Series *> &ss, int minL, int maxL, int step, int k,
              const paddi_opts &o, std::vector<paddi_discord> &out, paddi_stats *st);
+    // Range discords of a one-series batch at (m, r2) (derive(m) current): the flags and NN
+    // distances of paddi_range_discords.
+    void range(Batch &B, int m, float r2, int32_t *is_discord, float *nn_dist2);
+    // paddi_range_discords on the scratch array only: no device or pinned allocation.
+    void range_noalloc(const float *T, int64_t n, int m, float r2, int32_t *is_discord,
+                       float *nn_dist2);
     void copy_st64(double2 *dst, int64_t N);
 
     int64_t select_window_ = 8192;   // columns right of a block scanned by k_scan_select
@@ -217,6 +238,8 @@ private:
     void *pin_x_ = nullptr;        // its pinned staging
     size_t pin_x_bytes_ = 0;
     int witcur_ = 0;
+    void *scratch_ = nullptr;      // this device's scratch array (scratch_setup)
+    friend class ScratchScope;
 };
 
 }  // namespace paddi

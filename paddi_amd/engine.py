@@ -84,7 +84,8 @@ EXPORTS = ("paddi_get_tunables", "paddi_set_tunables", "paddi_status_str", "padd
            "paddi_compute_statistics", "paddi_range_discords", "paddi_discords",
            "paddi_discords_batch", "paddi_series_upload", "paddi_series_free",
            "paddi_discords_series", "paddi_has_mode", "paddi_mlp_sizes", "paddi_features",
-           "paddi_mlp_forward", "paddi_mlp_gradient", "paddi_mlp_train")
+           "paddi_mlp_forward", "paddi_mlp_gradient", "paddi_mlp_train",
+           "paddi_range_discords_noalloc")
 
 _lib = None
 
@@ -106,6 +107,7 @@ def load_library(path: str = LIB_PATH):
     L.paddi_destroy.restype = None
     L.paddi_compute_statistics.argtypes = [P, P, i64, i32, P, P]
     L.paddi_range_discords.argtypes = [P, P, i64, i32, C.c_float, P, P]
+    L.paddi_range_discords_noalloc.argtypes = [P, P, i64, i32, C.c_float, P, P]
     L.paddi_discords.argtypes = [P, P, i64, i32, i32, i32, i32, C.POINTER(Opts), P, i64,
                                  C.POINTER(i64), C.POINTER(Stats)]
     L.paddi_discords_batch.argtypes = [P, P, P, i32, i32, i32, i32, i32, C.POINTER(Opts), P,
@@ -247,6 +249,18 @@ class Context:
         nn = np.empty(max(N, 0), np.float32)
         self._check(self._L.paddi_range_discords(self._c, _ptr(T), len(T), m, C.c_float(r2),
                                                  _ptr(flag), _ptr(nn)), "paddi_range_discords")
+        return flag, nn
+
+    def range_discords_noalloc(self, T, m, r2):
+        """range_discords through the entry point that allocates nothing (its buffers are a
+        fixed device array; series of up to 300,000 points)."""
+        T = _f32(T)
+        N = len(T) - m + 1
+        flag = np.empty(max(N, 0), np.int32)
+        nn = np.empty(max(N, 0), np.float32)
+        self._check(self._L.paddi_range_discords_noalloc(self._c, _ptr(T), len(T), m,
+                                                         C.c_float(r2), _ptr(flag), _ptr(nn)),
+                    "paddi_range_discords_noalloc")
         return flag, nn
 
     # -- runner API --------------------------------------------------------------
