@@ -11,8 +11,9 @@ with the learned r proposal when built); `value` is the primary form's throughpu
 both must return the identical discord set.
 
 Printed: one JSON line with the metric, the roofline of the dominant kernel (algorithmic
-bytes per launch / hipEvent launch time, against 8 TB/s HBM3E) and the CPU oracle timed on
-a bounded sample on this host.
+bytes per launch / hipEvent launch time, against 8 TB/s HBM3E) and the CPU baseline: the
+oracle (the reference CPU path restated) on the workload's own series at six sampled
+lengths, each at its final range of the GPU run, on a declared sample of rows (cpu_baseline).
 """
 import argparse
 import json
@@ -46,7 +47,6 @@ def parse():
     p.add_argument("--primary", default="auto", help="plain|batched|learned|auto")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true", help="skip the plain form (profiling)")
-    p.add_argument("--cpu-points", type=int, default=131072)
     p.add_argument("--no-tuned", action="store_true", help="ignore the autotuner's table")
     p.add_argument("--no-gain", action="store_true",
                    help="skip the batched form without the learned proposal")
@@ -77,19 +77,76 @@ def run_form(ctx, handles, w, maxL, mode, steps, warmup):
     return recs, times, stats
 
 
-def cpu_baseline(x, w, npts):
-    """The oracle (C restatement of the reference CPU path, OpenMP over rows) on the first
-    `npts` points at the first length, with the main.cu r-schedule."""
+# CPU baseline sample per workload: (row-run stride, number of series sampled or None = all)
+CPU_SAMPLE = {"c1": (1, None), "c2": (1, None), "c3": (4, None), "c4": (2, None), "c5": (1, 8)}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def sample_lengths(w):
+    """The first three lengths and three more evenly spaced over the rest (the last one
+    included)."""
+    ms = list(range(w["minL"], w["maxL"] + 1, w["step"]))
+    if len(ms) <= 6:
+        return ms
+    rest = ms[3:]
+    pick = [rest[round((q + 1) * (len(rest) - 1) / 3)] for q in range(3)]
+    return ms[:3] + pick
+
+
+def cpu_baseline(w, recs, workload):
+    """The oracle (C restatement of find_discords_cpu, src/cpu_kernels/cpu_kernels.cpp:7-78,
+    FP64, reference scan order, OpenMP over rows) on the workload's own series: per sampled
+    length, the statistics of every row and the range pass at that length's final range of
+    the GPU run (the primary form's, read from its records), the pass on every stride-th run
+    of 256 rows against all columns and its time scaled by the sampled fraction of rows."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-    xs = x[:npts]
-    t0 = time.perf_counter()
-    O.palmad(xs, w["minL"], w["minL"], 1, w["k"], nthreads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": npts * 1 / dt, "unit": "points*lengths/s", "cores": threads,
-            "kind": "port", "seconds": dt,
-            "sample": f"first {npts} points of the series, m={w['minL']} only, top-{w['k']}, "
-                      f"plain r-descent, FP64 oracle, {threads} threads"}
+    stride, nser = CPU_SAMPLE[workload]
+    series = w["series"]
+    sel = list(range(len(series)))
+    if nser is not None and nser < len(series):
+        import numpy as np
+        sel = sorted(int(q) for q in np.random.default_rng(5).choice(len(series), nser,
+                                                                     replace=False))
+    final_r2 = {}
+    for r in recs:
+        final_r2[(r.series, r.length)] = r.range2
+    per, units, est = [], 0.0, 0.0
+    for m in sample_lengths(w):
+        for q in sel:
+            x = series[q]
+            r2 = final_r2.get((q, m), 0.0)
+            rows, cand, nn, ts, tscan = O.range_discords_sampled(x, m, r2, 256, stride, threads)
+            frac = len(rows) / (len(x) - m + 1)
+            e = ts + tscan / frac
+            units += len(x)
+            est += e
+            per.append({"series": q, "m": m, "r2": r2, "rows_sampled": int(len(rows)),
+                        "stats_s": round(ts, 3), "scan_s": round(tscan, 3),
+                        "estimated_s": round(e, 3)})
+            print(f"[bench] cpu baseline: series {q} m={m} r2={r2:.5g}: scan of "
+                  f"{len(rows)} rows {tscan:.2f} s -> {e:.2f} s for all rows", file=sys.stderr,
+                  flush=True)
+    ms = sample_lengths(w)
+    return {"value": units / est, "unit": "points*lengths/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(), "nproc": os.cpu_count(), "seconds_estimated": est,
+            "sample": f"{len(sel)} of {len(series)} series; lengths {ms}; final attempt only "
+                      f"(each length's final range of the GPU run; the reference's own "
+                      f"r-descent would also run its earlier attempts, and at a lower range); "
+                      f"statistics of all rows, range pass on every {stride}th run of 256 rows "
+                      f"against all columns, extrapolated to all rows; FP64, reference scan "
+                      f"order, {threads} threads",
+            "per_length": per}
 
 
 def main():
@@ -151,6 +208,20 @@ def main():
     t_step = sum(times) / len(times)
     value = units / t_step
     plain_value = units / (sum(plain_t) / len(plain_t))
+    # The first length: the plain form starts at the reference's r = 2 sqrt(m) and halves it
+    # (src/main.cu:38,155), the fast forms from the learned proposal or a probe descent; so
+    # the ratio is also given without it (each form run once more on the first length alone).
+    first = None
+    if not (a.no_plain and primary != "plain") and nlen > 1:
+        _, fp, _ = run_form(ctx, handles, w, w["minL"], "plain", 1, 0)
+        _, fq, _ = run_form(ctx, handles, w, w["minL"], primary, 1, 0)
+        rest_plain = sum(plain_t) / len(plain_t) - fp[0]
+        rest_primary = t_step - fq[0]
+        first = {"plain_ms": 1e3 * fp[0], "primary_ms": 1e3 * fq[0],
+                 "speedup_vs_plain_after_first_length":
+                     rest_plain / rest_primary if rest_primary > 0 else None}
+        print(f"[bench] first length alone: plain {fp[0]:.3f} s, {primary} {fq[0]:.3f} s",
+              file=sys.stderr, flush=True)
 
     # dominant kernel: the one with the most summed time across the timed steps
     names = ("derive", "select", "refine", "exact")
@@ -196,6 +267,7 @@ def main():
         "plain": {"value": plain_value, "ms_per_step": 1e3 * sum(plain_t) / len(plain_t),
                   "attempts": plain_st[-1]["attempts"]},
         "speedup_vs_plain": value / plain_value,
+        "first_length": first,
         "learned_gain": gain,
         "identical_discord_set": identical,
         "attempts": stats[-1]["attempts"],
@@ -211,7 +283,7 @@ def main():
                  "frac_of_peak": valu},
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(series[0], w, a.cpu_points)
+        out["cpu_baseline"] = cpu_baseline(w, recs, a.workload)
     print(json.dumps(out))
 
 

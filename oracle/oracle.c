@@ -182,6 +182,29 @@ void oracle_find_discords_f64(const float *T, const double *mean, const double *
 }
 
 /*
+ * oracle_find_discords_f64 on a sample of the rows: the rows of every stride-th run of
+ * `run` consecutive rows (runs 0, stride, 2 stride, ..), each against all columns in the
+ * reference's order; the other rows' cand/nnDist are left as they are.  Rows are independent
+ * (cpu_kernels.cpp:42-76), so the time of the sample times `stride` estimates the time of
+ * the whole range pass (the CPU baseline of long searches).
+ */
+void oracle_find_discords_runs_f64(const float *T, const double *mean, const double *std,
+                                   int32_t *cand, double *nnDist, int64_t N, double r,
+                                   int32_t m, int32_t nthreads, int64_t run, int64_t stride)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    const int64_t nruns = (N + run - 1) / run;
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t q = 0; q < nruns; q += stride) {
+        const int64_t e = (q + 1) * run < N ? (q + 1) * run : N;
+        for (int64_t i = q * run; i < e; i++)
+            if (scan_row(T, mean, std, i, 0, N, N, r, m, &nnDist[i]) >= 0) cand[i] = 0;
+    }
+}
+
+/*
  * The same range-discord set, for large series (the certificates of long
  * searches): cand[i] and, where cand[i] == 1, nnDist[i] equal
  * oracle_find_discords_f64's bit for bit, since neither depends on the order

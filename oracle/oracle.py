@@ -31,6 +31,7 @@ def lib():
         L.oracle_stats_f32.argtypes = [P, i64, i32, P, P]
         L.oracle_find_discords_f64.argtypes = [P, P, P, P, P, i64, f64, i32, i32]
         L.oracle_find_discords_hint_f64.argtypes = [P, P, P, P, P, i64, f64, i32, i32]
+        L.oracle_find_discords_runs_f64.argtypes = [P, P, P, P, P, i64, f64, i32, i32, i64, i64]
         L.oracle_find_discords_f32.argtypes = [P, P, P, P, P, i64, C.c_float, i32]
         L.oracle_profile_f64.argtypes = [P, P, P, P, i64, i32, i32]
         L.oracle_topk.argtypes = [P, P, i64, f64, i32, i32, P, P, P]
@@ -72,6 +73,26 @@ def range_discords(T, m, r2, fp32=False, nthreads=0, hint=False):
         fn = lib().oracle_find_discords_hint_f64 if hint else lib().oracle_find_discords_f64
         fn(_p(T), _p(mu), _p(sd), _p(cand), _p(nn), N, float(r2), m, nthreads)
     return cand, nn
+
+
+def range_discords_sampled(T, m, r2, run=256, stride=8, nthreads=0):
+    """find_discords_cpu at squared range r2 on every stride-th run of `run` rows (reference
+    scan order).  Returns (rows, cand, nnDist, stats_seconds, scan_seconds) for the sampled
+    rows; the statistics are computed for all rows, as the reference does per length."""
+    import time
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    N = len(T) - m + 1
+    t0 = time.perf_counter()
+    mu, sd = stats(T, m)
+    t1 = time.perf_counter()
+    cand = np.ones(N, np.int32)
+    nn = np.full(N, np.finfo(np.float64).max, np.float64)
+    lib().oracle_find_discords_runs_f64(_p(T), _p(mu), _p(sd), _p(cand), _p(nn), N, float(r2),
+                                        m, nthreads, run, stride)
+    t2 = time.perf_counter()
+    rows = np.arange(N)
+    rows = rows[(rows // run) % stride == 0]
+    return rows, cand[rows], nn[rows], t1 - t0, t2 - t1
 
 
 def profile(T, m, nthreads=0):

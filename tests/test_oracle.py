@@ -280,3 +280,18 @@ def test_hinted_scan_order_gives_the_same_range_set(kind):
         assert np.array_equal(c1, c2), (kind, q)
         assert np.array_equal(n1[c1 == 1], n2[c2 == 1]), (kind, q)
         assert np.array_equal(n1[c1 == 1], nn_ref[c1 == 1]), (kind, q)
+
+
+def test_sampled_rows_scan_is_the_full_scan_on_those_rows():
+    """The CPU baseline's row sample (every stride-th run of rows, reference scan order) gives
+    those rows exactly the full pass's flags and, where flagged, NN distances."""
+    x = synth.ar2_mix(20000, 3)
+    m = 50
+    nn_ref = O.profile(x, m)
+    for q in (0.5, 0.95):
+        r2 = float(np.quantile(nn_ref, q))
+        c, n = O.range_discords(x, m, r2)
+        rows, cs, ns, _, _ = O.range_discords_sampled(x, m, r2, run=256, stride=3)
+        assert len(rows) > 0 and np.all((rows // 256) % 3 == 0)
+        assert np.array_equal(cs, c[rows])
+        assert np.array_equal(ns[cs == 1], n[rows][cs == 1])
