@@ -191,6 +191,20 @@ static inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * 
 // candidate in exact order; it is accepted when its distance exceeds r2, and every
 // candidate overlapping it (|i - a| < m) is dropped, since all of them rank below it.
 
+// Exact distances within TIE_REL of each other rank as equal, and then by index.  k_exact's
+// value of a pair is not bitwise symmetric (it centres the candidate's side), while a mutual
+// nearest-neighbour pair shares one distance, which the reference's strict '>' ranks lower
+// index first (sets_processing.hpp:28); the evaluation errors are orders of magnitude below
+// TIE_REL, and TIE_REL far below the 1e-4 parity bar.
+constexpr double TIE_REL = 1e-9;
+static bool ranks_before(double va, int64_t ia, double vb, int64_t ib)
+{
+    const double tol = TIE_REL * std::max(std::fabs(va), std::fabs(vb));
+    if (va > vb + tol) return true;
+    if (vb > va + tol) return false;
+    return ia < ib;
+}
+
 void TopK::reset(std::vector<Cand> &&c, double r2_, int m_, int k_)
 {
     cands = std::move(c);
@@ -226,10 +240,12 @@ bool TopK::advance(std::vector<size_t> &need)
             }
         }
         if (T < 0) { done = true; break; }
+        // every candidate that can rank at or above T's lower bound (ties included)
         const double loT = lo(cands[(size_t)T]);
+        const double loTt = loT - 2.0 * TIE_REL * std::fabs(loT);
         std::vector<size_t> U;
         for (size_t q = 0; q < cands.size(); q++)
-            if (!gone[q] && hi(cands[q]) >= loT) U.push_back(q);
+            if (!gone[q] && hi(cands[q]) >= loTt) U.push_back(q);
         if (U.empty()) U.push_back((size_t)T);  // only when distances are not ordered (NaN)
         for (size_t q : U)
             if (!cands[q].exact) need.push_back(q);
@@ -265,10 +281,8 @@ bool TopK::advance(std::vector<size_t> &need)
             return true;
         }
         size_t B = U[0];
-        for (size_t q : U) {
-            const Cand &a = cands[q], &b = cands[B];
-            if (a.v > b.v || (a.v == b.v && a.i < b.i)) B = q;
-        }
+        for (size_t q : U)
+            if (ranks_before(cands[q].v, cands[q].i, cands[B].v, cands[B].i)) B = q;
         if (!(cands[B].v > r2)) { done = true; break; }
         acc.push_back(cands[B]);
         const int64_t bi = cands[B].i;
@@ -385,7 +399,7 @@ void Engine::build_batch(Batch &B, const std::vector<const Series *> &ss, int mi
     B.npb = po;
     B.x.reserve((size_t)xo * sizeof(float));
     B.P.reserve((size_t)po * (PB + 1) * sizeof(double2));
-    B.refs.reserve((size_t)po * sizeof(double));
+    B.refs.reserve((size_t)po * sizeof(double4));
     B.ser.reserve(ss.size() * sizeof(SerDev) * 4);
     B.bser.reserve((size_t)B.nblocks * sizeof(int));
     B.pser.reserve((size_t)po * sizeof(int));
@@ -463,7 +477,7 @@ void Engine::prefix(Batch &B)
     HIPC(hipMemsetAsync(flag_.p, 0, sizeof(int), stream_));
     hipLaunchKernelGGL(k_prefix, dim3((unsigned)B.npb), dim3(256), 0, stream_,
                        B.x.as<float>(), B.ser.as<SerDev>(), B.pser.as<int>(), B.P.as<double2>(),
-                       B.refs.as<double>(), flag_.as<int>());
+                       B.refs.as<double4>(), flag_.as<int>());
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(pin_, flag_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
     wait();
@@ -487,7 +501,7 @@ void Engine::derive(Batch &B, int m, paddi_stats *st, int m2)
     HIPC(hipEventRecord(ev_[4], stream_));
     hipLaunchKernelGGL(k_derive, dim3(grid1(B.rows, 256)), dim3(256), 0, stream_,
                        B.x.as<float>(), B.ser.as<SerDev>(), B.bser.as<int>(), B.P.as<double2>(),
-                       B.refs.as<double>(), B.rows, m, colv_.as<float4>(),
+                       B.refs.as<double4>(), B.rows, m, colv_.as<float4>(),
                        have ? nullptr : st64_.as<double2>(), tmax_.as<float2>(), m2,
                        st64n_.as<double2>());
     HIPC(hipGetLastError());
@@ -1332,7 +1346,7 @@ void Engine::range_noalloc(const float *T, int64_t n, int m, float r2, int32_t *
     const int64_t rows = series_rows(n, m), nblocks = rows / 64;
     S.bind(B.x, (size_t)xo * 4);
     S.bind(B.P, (size_t)npb * (PB + 1) * sizeof(double2));
-    S.bind(B.refs, (size_t)npb * sizeof(double));
+    S.bind(B.refs, (size_t)npb * sizeof(double4));
     S.bind(B.ser, sizeof(SerDev) * 4);
     S.bind(B.bser, (size_t)nblocks * 4);
     S.bind(B.pser, (size_t)npb * 4);

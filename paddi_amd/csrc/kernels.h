@@ -128,10 +128,10 @@ constexpr int EXACT_EXT = 4;  // largest m2 - m
 // Blocked prefix sums of every series; *bad becomes nonzero if a value is NaN or infinite.
 __global__ void k_prefix(const float *__restrict__ x, const SerDev *__restrict__ ser,
                          const int *__restrict__ pser, double2 *__restrict__ P,
-                         double *__restrict__ refs, int *__restrict__ bad);
+                         double4 *__restrict__ refs, int *__restrict__ bad);
 __global__ void k_derive(const float *__restrict__ x, const SerDev *__restrict__ ser,
                          const int *__restrict__ bser, const double2 *__restrict__ P,
-                         const double *__restrict__ refs, int64_t rows, int m,
+                         const double4 *__restrict__ refs, int64_t rows, int m,
                          float4 *__restrict__ colv, double2 *__restrict__ st64,
                          float2 *__restrict__ tmax, int m2, double2 *__restrict__ st64n);
 // The scan kernels: one 64-row block per workgroup (one wave); LDS from scan_lds_bytes(m).

@@ -71,11 +71,13 @@ __device__ __forceinline__ unsigned long long ord_key64(double d)
 // Prefix sums.  Prefix block b of series s covers x_s[b*PB, (b+1)*PB); c_b = x_s[b*PB].
 // P[(pboff_s + b)*(PB+1) + k] holds (sum, sum of squares) of (x - c_b) over
 // x_s[b*PB, b*PB+k), k = 0..PB (k = PB is the total).  Points at or beyond n_s count as c_b.
+// refs[pboff_s + b] = (c_b, total of squares Y_b, sqrt(PB Y_b), 0): the block's reference and
+// the magnitudes that bound the rounding of its sums (window_sums).
 
 __global__ __launch_bounds__(256) void k_prefix(const float *__restrict__ x,
                                                 const SerDev *__restrict__ ser,
                                                 const int *__restrict__ pser,
-                                                double2 *__restrict__ P, double *__restrict__ refs,
+                                                double2 *__restrict__ P, double4 *__restrict__ refs,
                                                 int *__restrict__ bad)
 {
     const int t = threadIdx.x;
@@ -118,45 +120,83 @@ __global__ __launch_bounds__(256) void k_prefix(const float *__restrict__ x,
 #pragma unroll
     for (int q = 0; q < PER; q++)
         Pb[t * PER + q] = make_double2(o1 + v1[q], o2 + v2[q]);
-    if (t == 255) Pb[PB] = make_double2(o1 + s1, o2 + s2);
-    if (t == 0) refs[gb] = c;
+    if (t == 255) {
+        const double y = o2 + s2;
+        Pb[PB] = make_double2(o1 + s1, y);
+        refs[gb] = make_double4(c, y, sqrt((double)PB * y), 0.0);
+    }
     if (nonfinite) atomicOr(bad, 1);
 }
 
-// (sum, sum of squares) of (x - c_b) over the window [i, i+m) of one series; m <= PB.
+// (sum, sum of squares) of (x - c_b) over the window [i, i+m) of one series; m <= PB.  *err
+// receives a bound on the rounding error of M2 = sum of squares - sum^2 / m formed from them:
+// each prefix value carries at most 64 roundings of the block's running totals (16 serial
+// adds per thread, six shuffle levels, the wave offsets), every partial sum of |x - c_b| is
+// at most sqrt(PB * total of squares), and the shift of a window crossing into block b+1
+// adds a few roundings of its terms.
 __device__ __forceinline__ double2 window_sums(const double2 *__restrict__ P,
-                                               const double *__restrict__ refs, int64_t i, int m,
-                                               double *cb)
+                                               const double4 *__restrict__ refs, int64_t i, int m,
+                                               double invm, double *cb, double *err)
 {
+    constexpr double G = 64.0 * 1.1102230246251565e-16;
     const int64_t b = i / PB;
     const int o = (int)(i - b * PB);
     const int e = o + m;
     const double2 *Pb = P + b * (PB + 1);
     const double2 lo = Pb[o];
-    *cb = refs[b];
+    const double4 rb = refs[b];
+    *cb = rb.x;
+    double2 s;
+    double Y = rb.y, X = rb.z;
     if (e <= PB) {
         const double2 hi = Pb[e];
-        return make_double2(hi.x - lo.x, hi.y - lo.y);
+        s = make_double2(hi.x - lo.x, hi.y - lo.y);
+    } else {
+        const double2 tot = Pb[PB];
+        const double2 nx = P[(b + 1) * (PB + 1) + (e - PB)];
+        const double4 rn = refs[b + 1];
+        const double D = rn.x - rb.x;
+        const double cnt = (double)(e - PB);
+        const double a1 = tot.x - lo.x, a2 = tot.y - lo.y;
+        s = make_double2(a1 + nx.x + cnt * D, a2 + nx.y + 2.0 * D * nx.x + cnt * D * D);
+        Y += rn.y + fabs(2.0 * D * nx.x) + cnt * D * D;
+        X += rn.z + fabs(cnt * D);
     }
-    const double2 tot = Pb[PB];
-    const double2 nx = P[(b + 1) * (PB + 1) + (e - PB)];
-    const double D = refs[b + 1] - refs[b];
-    const double cnt = (double)(e - PB);
-    const double a1 = tot.x - lo.x, a2 = tot.y - lo.y;
-    return make_double2(a1 + nx.x + cnt * D, a2 + nx.y + 2.0 * D * nx.x + cnt * D * D);
+    *err = G * (Y + 2.0 * fabs(s.x) * X * invm) + 4.0 * 1.1102230246251565e-16 * (fabs(s.y));
+    return s;
 }
 
+// Statistics of window [i, i+m): mean, 1/sqrt(M2) (0 for a window of zero spread) and M2.
+// From the prefix sums where their rounding is below 1e-8 of M2; otherwise -- near-flat
+// windows far from their block's reference point, where the sums cancel -- from the window's
+// values in two passes, centred on its first value (differences of two floats are exact in
+// double precision).
 __device__ __forceinline__ void window_stats(const double2 *__restrict__ P,
-                                             const double *__restrict__ refs, int64_t i, int m,
-                                             double *mu, double *invn, double *m2 = nullptr)
+                                             const double4 *__restrict__ refs,
+                                             const float *__restrict__ xs, int64_t i, int m,
+                                             double invm, double *mu, double *invn,
+                                             double *m2 = nullptr)
 {
-    double cb;
-    double2 s = window_sums(P, refs, i, m, &cb);
-    const double q = s.x * s.x / m;
-    double M2 = s.y - q;
-    *mu = cb + s.x / m;
-    // zero spread (up to rounding of the sums) = degenerate window
-    *invn = (M2 > 1e-13 * (s.y + q) && M2 > 0.0) ? 1.0 / sqrt(M2) : 0.0;
+    double cb, err;
+    const double2 s = window_sums(P, refs, i, m, invm, &cb, &err);
+    const double sm = s.x * invm;
+    double M2 = s.y - s.x * sm;
+    double mean = cb + sm;
+    if (!(M2 >= 1e8 * err)) {
+        const double x0 = (double)xs[i];
+        double d = 0.0;
+        for (int k = 0; k < m; k++) d += (double)xs[i + k] - x0;
+        d *= invm;
+        double v = 0.0;
+        for (int k = 0; k < m; k++) {
+            const double y = ((double)xs[i + k] - x0) - d;
+            v = fma(y, y, v);
+        }
+        M2 = v;
+        mean = x0 + d;
+    }
+    *mu = mean;
+    *invn = M2 > 0.0 ? 1.0 / sqrt(M2) : 0.0;
     if (m2) *m2 = M2 > 0.0 ? M2 : 0.0;
 }
 
@@ -215,7 +255,7 @@ __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
                                                 const SerDev *__restrict__ ser,
                                                 const int *__restrict__ bser,
                                                 const double2 *__restrict__ P,
-                                                const double *__restrict__ refs, int64_t rows,
+                                                const double4 *__restrict__ refs, int64_t rows,
                                                 int m, float4 *__restrict__ colv,
                                                 double2 *__restrict__ st64,
                                                 float2 *__restrict__ tmax, int m2,
@@ -232,13 +272,14 @@ __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
         if (i < S.n - m + 1) {
             const float *xs = x + S.xoff;
             const double2 *Ps = P + S.pboff * (PB + 1);
-            const double *rs = refs + S.pboff;
+            const double4 *rs = refs + S.pboff;
+            const double invm = 1.0 / m;
             double mu, invn, M2;
-            window_stats(Ps, rs, i, m, &mu, &invn, &M2);
+            window_stats(Ps, rs, xs, i, m, invm, &mu, &invn, &M2);
             double df = 0.0, dg = 0.0;
             if (i > 0) {
                 double mup, invp;
-                window_stats(Ps, rs, i - 1, m, &mup, &invp);
+                window_stats(Ps, rs, xs, i - 1, m, invm, &mup, &invp);
                 const double xin = (double)xs[i + m - 1], xout = (double)xs[i - 1];
                 df = 0.5 * (xin - xout);
                 dg = (xin - mu) + (xout - mup);
@@ -248,7 +289,7 @@ __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
             sd = sqrtf((float)M2);
             if (m2 && i < S.n - m2 + 1) {
                 double mu2, inv2;
-                window_stats(Ps, rs, i, m2, &mu2, &inv2);
+                window_stats(Ps, rs, xs, i, m2, 1.0 / m2, &mu2, &inv2);
                 sv2 = make_double2(mu2, inv2);
             }
         }
@@ -1061,11 +1102,16 @@ __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__
 // Exact (FP64) nearest-neighbour squared distance of candidate groups.  A group is up to
 // MAXU candidates of one series; block = (group, EXACT_COLS-column chunk of that series),
 // found by binary search in goff (prefix sums of chunk counts).  The pair value is the
-// reference's formula in double precision (src/cpu_kernels/cpu_kernels.cpp:55):
-// ED^2 = 2m (1 - (dot - m (mu_i mu_j)) (invn_i invn_j)), dot = sum_k x[i+k] x[j+k] by FMAs in
-// ascending k.  Every product in it is commutative, so the value is bitwise symmetric in
-// (i, j): mutual nearest neighbours tie exactly and the tie goes to the lower index.  (The
-// raw dot cancels against m mu_i mu_j; in double precision the loss is a few digits of 16.)
+// reference's z-normalised distance (src/cpu_kernels/cpu_kernels.cpp:55) in double
+// precision, ED^2 = 2m (1 - cov (invn_i invn_j)), with the covariance formed without the
+// cancellation of the raw form (dot - m mu_i mu_j): the candidate's values enter centred on
+// its mean, a_k = x[i+k] - mu_i (exact: x and mu_i are within a window's spread), the
+// column's offset by the chunk's first value c (exact: a difference of two floats), and since
+// sum_k a_k (x[j+k] - mu_j) is the centred product,
+//     cov = sum_k a_k (x[j+k] - c) - (mu_j - c) sum_k a_k,
+// whose rounding is relative to the chunk's local spread, not to the series' level.  The value
+// is not bitwise symmetric in (i, j); the top-k treats distances within TIE_REL of each other
+// as equal and ranks them by index, as the oracle's exact ties rank.
 // outkey[u][b mod EXACT_SLOTS] receives the atomic minimum of ord_key64(ED^2) of candidate u
 // over blocks b (the host takes the minimum over the slots).
 
@@ -1074,7 +1120,7 @@ __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__
 // their results.
 //
 // Thread t owns the QC = exact_qc(NU) adjacent columns c + QC t + q of a sub-tile of 256 QC
-// columns and keeps a window of QC + 4 doubles of x in a register ring: per 4 steps of k it
+// columns and keeps a window of QC + 4 doubles of x - c in a register ring: per 4 steps of k it
 // reads 4 new doubles (two 16-byte LDS reads; the staged window holds QC values per QC + 2
 // doubles, so the lanes' reads, QC + 2 doubles apart, fall in distinct banks) and the
 // candidates' next 4 values (broadcast), for 4 QC NU FP64 FMAs.  Value x[c + QC t + j] sits in
@@ -1082,12 +1128,10 @@ __global__ __launch_bounds__(256) void k_survivors(ScanPtrs Q, const PassDev *__
 // index is a constant and the window never moves between registers.  Per column the sum runs
 // in ascending k with one FMA per term; candidate rows are zero-padded to a multiple of 4,
 // which adds exact zeros.
-// The pair distance from its dot product (both rows' statistics at length m).
-__device__ __forceinline__ double pair_ed2(double dot, int m, double mui, double invi,
-                                           double muj, double invj)
+// The pair distance from its centred covariance (both rows' 1/sqrt(M2) at length m).
+__device__ __forceinline__ double pair_ed2(double cov, int m, double invi, double invj)
 {
-    if (invi > 0.0 && invj > 0.0)
-        return 2.0 * m * (1.0 - (dot - m * (mui * muj)) * (invi * invj));
+    if (invi > 0.0 && invj > 0.0) return 2.0 * m * (1.0 - cov * (invi * invj));
     return 2.0 * m;
 }
 
@@ -1189,7 +1233,9 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     const int m4 = exact_mk(m, NU);                     // a multiple of 4 (of 16 for NU = 3)
     double *xs = dsh;                                   // exact_xs_doubles(m, NU)
     double *A = dsh + exact_xs_doubles(m, NU);          // [NU][m4] candidate windows
-    __shared__ double ainv[MAXU], amu[MAXU], ainv2[MAXU], amu2[MAXU], aext[MAXU][EXACT_EXT];
+    __shared__ double ainv[MAXU], ainv2[MAXU], aext[MAXU][EXACT_EXT];
+    __shared__ double asa[MAXU], asb[MAXU];   // sum of a_k over k < m, and over k < m2
+    __shared__ double apart[MAXU][4];         // per-wave partial sums of a_k
     __shared__ int64_t cidx[MAXU];
     __shared__ unsigned long long wmin[MAXU][4];
     const int t = threadIdx.x;
@@ -1211,20 +1257,35 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     const int64_t N2 = DUAL ? S.n - m2 + 1 : 0;
     const double2 *st64n = E.st64n + S.roff;
     const int64_t cblk = (item - E.goff[grp]) * EXACT_COLS;
+    const double cref = (double)x[cblk];   // the columns' offset (cblk < N)
     for (int u = 0; u < NU; u++) {
         const int64_t i = E.cand[u0 + (u < nU ? u : 0)];
         const double2 st = st64[i];
         if (t == 0) {
             ainv[u] = st.y;
-            amu[u] = st.x;
             const double2 s2 = i < N2 ? st64n[i] : make_double2(0.0, 0.0);
-            amu2[u] = s2.x;
             ainv2[u] = s2.y;
             cidx[u] = i;
             for (int e = 0; e < EXACT_EXT; e++)
-                aext[u][e] = (e < ext && i < N2) ? (double)x[i + m + e] : 0.0;
+                aext[u][e] = (e < ext && i < N2) ? (double)x[i + m + e] - st.x : 0.0;
         }
-        for (int k = t; k < m4; k += 256) A[u * m4 + k] = k < m ? (double)x[i + k] : 0.0;
+        double part = 0.0;
+        for (int k = t; k < m4; k += 256) {
+            const double a = k < m ? (double)x[i + k] - st.x : 0.0;
+            A[u * m4 + k] = a;
+            part += a;
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
+        if ((t & 63) == 0) apart[u][t >> 6] = part;
+    }
+    __syncthreads();
+    if (t < NU) {
+        const double sa = (apart[t][0] + apart[t][1]) + (apart[t][2] + apart[t][3]);
+        double sb = sa;
+        for (int e = 0; e < EXACT_EXT; e++) sb += aext[t][e];
+        asa[t] = sa;
+        asb[t] = sb;
     }
     unsigned long long kmin[NU], kmin2[NU];
 #pragma unroll
@@ -1243,7 +1304,8 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
             for (int p = 0; p < 4; p++) v[p] = (k + 256 * p < avail) ? x[c + k + 256 * p] : 0.f;
 #pragma unroll
             for (int p = 0; p < 4; p++)
-                if (k + 256 * p < lim) xs[xs_at<QC>(k + 256 * p)] = (double)v[p];
+                if (k + 256 * p < lim)
+                    xs[xs_at<QC>(k + 256 * p)] = (k + 256 * p < avail) ? (double)v[p] - cref : 0.0;
         }
         __syncthreads();
         double acc[NU][QC];
@@ -1278,10 +1340,12 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
             if (k0 < m4) exact_step<NU, QC, 0>(xt, A, m4, k0, w, acc);
             if (UR == 3 && k0 + 4 < m4) exact_step<NU, QC, 4>(xt, A, m4, k0 + 4, w, acc);
         }
-        // Per column: its statistics, then every candidate's distance at m and, with m2, the
-        // dot product at m2 as the same FMA chain continued over k = m .. m2-1 (the
-        // zero-padded terms between add exact zeros), so each value is bitwise the one a
-        // launch at m2 computes.  Staged values reach column offset 256 QC + m4 + 3 >=
+        // Per column: its statistics, then every candidate's distance at m and, with m2, at m2:
+        // the FMA chain continued over k = m .. m2-1 with the candidate's values still centred
+        // on its mean at m (the zero-padded terms between add exact zeros), which gives the
+        // centred product at m2 after the same correction with the column's mean at m2 and
+        // the sum of those values over k < m2 (the product is the same for any centring of
+        // one side).  Staged values reach column offset 256 QC + m4 + 3 >=
         // 256 QC - 1 + m2 - 1.
 #pragma unroll
         for (int q = 0; q < QC; q++) {
@@ -1302,8 +1366,8 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
             for (int u = 0; u < NU; u++) {
                 const int64_t i = cidx[u];
                 const bool valid = (j < N) && ((j - i >= m - 1) || (i - j >= m + 1));
-                const double ed =
-                    valid ? pair_ed2(acc[u][q], m, amu[u], ainv[u], sj.x, sj.y) : DBL_MAX;
+                const double cov = fma(-(sj.x - cref), asa[u], acc[u][q]);
+                const double ed = valid ? pair_ed2(cov, m, ainv[u], sj.y) : DBL_MAX;
                 const unsigned long long key = ord_key64(ed);
                 kmin[u] = key < kmin[u] ? key : kmin[u];
                 if (DUAL) {
@@ -1313,8 +1377,8 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
                         if (e < ext) a2 = fma(aext[u][e], xe[e], a2);
                     const bool valid2 =
                         (i < N2) && (j < N2) && ((j - i >= m2 - 1) || (i - j >= m2 + 1));
-                    const double ed2 =
-                        valid2 ? pair_ed2(a2, m2, amu2[u], ainv2[u], sj2.x, sj2.y) : DBL_MAX;
+                    const double cov2 = fma(-(sj2.x - cref), asb[u], a2);
+                    const double ed2 = valid2 ? pair_ed2(cov2, m2, ainv2[u], sj2.y) : DBL_MAX;
                     const unsigned long long key2 = ord_key64(ed2);
                     kmin2[u] = key2 < kmin2[u] ? key2 : kmin2[u];
                 }

@@ -94,7 +94,7 @@ void Engine::features(Batch &B, int m, int k, std::vector<float> &X)
                        stream_, B.x.as<float>(), B.ser.as<SerDev>(), st64_.as<double2>(), ns, m,
                        FEAT_SEED, hist);
     hipLaunchKernelGGL(k_feat_std, dim3((unsigned)((ns + 63) / 64)), dim3(64), 0, stream_,
-                       B.ser.as<SerDev>(), B.P.as<double2>(), B.refs.as<double>(), ns, gstd);
+                       B.ser.as<SerDev>(), B.P.as<double2>(), B.refs.as<double4>(), ns, gstd);
     std::vector<float> h((size_t)ns * 33);
     HIPC(hipMemcpyAsync(h.data(), hist, h.size() * 4, hipMemcpyDeviceToHost, stream_));
     HIPC(hipStreamSynchronize(stream_));

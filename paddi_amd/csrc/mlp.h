@@ -17,7 +17,7 @@ __global__ void k_feat_pairs(const float *__restrict__ x, const SerDev *__restri
                              const double2 *__restrict__ st64, int nser, int m, uint64_t seed,
                              float *__restrict__ hist);
 __global__ void k_feat_std(const SerDev *__restrict__ ser, const double2 *__restrict__ P,
-                           const double *__restrict__ refs, int nser, float *__restrict__ gstd);
+                           const double4 *__restrict__ refs, int nser, float *__restrict__ gstd);
 __global__ void k_mlp_forward(const float *__restrict__ X, int B, const float *__restrict__ prm,
                               const float *__restrict__ fmu, const float *__restrict__ fsd,
                               float *__restrict__ H1, float *__restrict__ H2,

@@ -67,18 +67,18 @@ __global__ __launch_bounds__(256) void k_feat_pairs(const float *__restrict__ x,
 
 // Global standard deviation of each series from its prefix blocks (one thread per series).
 __global__ void k_feat_std(const SerDev *__restrict__ ser, const double2 *__restrict__ P,
-                           const double *__restrict__ refs, int nser, float *__restrict__ gstd)
+                           const double4 *__restrict__ refs, int nser, float *__restrict__ gstd)
 {
     const int sid = blockIdx.x * blockDim.x + threadIdx.x;
     if (sid >= nser) return;
     const SerDev S = ser[sid];
     const int64_t nb = (S.n + PB - 1) / PB;
     double s1 = 0.0, s2 = 0.0;
-    const double c0 = refs[S.pboff];
+    const double c0 = refs[S.pboff].x;
     for (int64_t b = 0; b < nb; b++) {
         const double2 t = P[(S.pboff + b) * (PB + 1) + PB];
         const double cnt = (double)((b + 1) * PB <= S.n ? PB : S.n - b * PB);
-        const double D = refs[S.pboff + b] - c0;  // shift block sums to the reference c0
+        const double D = refs[S.pboff + b].x - c0;  // shift block sums to the reference c0
         s1 += t.x + cnt * D;
         s2 += t.y + 2.0 * D * t.x + cnt * D * D;
     }
