@@ -245,6 +245,9 @@ def main():
                 traffic = v["bytes_per_dispatch"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     pairs = sum(s["pairs"] for s in stats) / len(stats)
+    # the share of each scan's lane-steps spent on rows still alive when their tile started
+    useful = {k: (sum(s["live_pairs"][k] for s in stats) /
+                  max(sum(s["scan_pairs"][k] for s in stats), 1)) for k in ("select", "refine")}
     scan_ms = (tot["select"] + tot["refine"]) / len(stats)
     valu = (pairs * OPS_PER_PAIR) / (scan_ms * 1e-3) / VALU_PEAK_OPS if scan_ms > 0 else 0.0
 
@@ -280,7 +283,7 @@ def main():
                      "bytes_per_launch": bytes_per_launch},
         "tunables": tunables,
         "valu": {"pairs_per_step": pairs, "ops_per_pair": OPS_PER_PAIR,
-                 "frac_of_peak": valu},
+                 "frac_of_peak": valu, "live_row_share": useful},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, recs, a.workload)

@@ -79,6 +79,9 @@ typedef struct {
     double seconds;          /* wall time of the call */
     double kernel_ms[4];     /* derive, select, refine, exact: summed kernel time (ms, hipEvent) */
     int64_t launches[4];     /* launches of the same four kernels */
+    int64_t live_pairs[2];   /* of the select and refine scans' pair evaluations, those on rows
+                                still alive when their 64-column tile started */
+    int64_t scan_pairs[2];   /* the select and refine scans' pair evaluations */
 } paddi_stats;
 
 typedef struct paddi_ctx paddi_ctx;
@@ -98,7 +101,7 @@ typedef struct {
                                 i-1 .. i-prev_shift (0..4) */
     int32_t bootstrap;       /* learned form, first length: r also bounded from below by the
                                 exact NN of the best-scored candidate rows (0/1) */
-    int32_t reserved;        /* 0 */
+    int32_t bound_candidates;/* those candidate rows per series (1..256; 0 = the default 32) */
 } paddi_tunables;
 
 const char *paddi_status_str(paddi_status s);

@@ -36,6 +36,7 @@ GRID = {
     "use_graph": [0, 1],
     "prev_shift": [0, 1, 2],
     "bootstrap": [0, 1],
+    "bound_candidates": [16, 32, 64],
 }
 
 

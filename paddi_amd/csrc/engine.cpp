@@ -693,7 +693,7 @@ void Engine::first_bound(Batch &B, int m, int k, const std::vector<char> &need,
 {
     lb.assign((size_t)B.nser, 0.0);
     std::vector<std::vector<int64_t>> cand;
-    bound_candidates(B, m, need, BOUND_CANDIDATES, cand, st);
+    bound_candidates(B, m, need, bound_cands_, cand, st);
     std::vector<std::pair<int, int64_t>> req;
     for (int q = 0; q < B.nser; q++)
         for (int64_t i : cand[(size_t)q]) req.emplace_back(q, i);
@@ -769,10 +769,12 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
     wait();
     collect_derive();
     const PassCtl C = *(const PassCtl *)pin_out_;
-    double pairs_sel = 0.0, pairs_ref = 0.0;
+    double pairs_sel = 0.0, pairs_ref = 0.0, live_sel = 0.0, live_ref = 0.0;
     for (int q = 0; q < PAIR_SLOTS; q++) {
         pairs_sel += (double)C.pairs[0][q];
         pairs_ref += (double)C.pairs[1][q];
+        live_sel += (double)C.live[0][q];
+        live_ref += (double)C.live[1][q];
     }
     const int ns = C.survivors;
     if (debug_) {
@@ -819,6 +821,10 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
         st->attempts += 1;
         st->survivors += ns;
         st->pairs += (int64_t)(pairs_sel + pairs_ref);
+        st->scan_pairs[0] += (int64_t)pairs_sel;
+        st->scan_pairs[1] += (int64_t)pairs_ref;
+        st->live_pairs[0] += (int64_t)live_sel;
+        st->live_pairs[1] += (int64_t)live_ref;
     }
 }
 
@@ -1432,7 +1438,7 @@ paddi_status paddi_get_tunables(paddi_ctx *ctx, paddi_tunables *out)
     const paddi::Engine &E = *ctx->eng;
     *out = paddi_tunables{E.select_window_, E.refine_chunk_, E.min_chunk_, E.refine_waves_,
                           E.witness_tiles_, E.spec_, E.use_graph_ ? 1 : 0, E.prev_shift_,
-                          E.bootstrap_ ? 1 : 0, 0};
+                          E.bootstrap_ ? 1 : 0, E.bound_cands_};
     return PADDI_OK;
 }
 
@@ -1442,7 +1448,8 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
     if (t->select_window < 64 || t->refine_chunk < 64 || t->min_chunk < 64 ||
         t->min_chunk > t->refine_chunk || t->refine_items < 1 || t->witness_tiles < 0 ||
         t->witness_tiles > 64 || t->speculate < 1 || t->speculate > 64 || t->prev_shift < 0 ||
-        t->prev_shift > 4 || t->bootstrap < 0 || t->bootstrap > 1)
+        t->prev_shift > 4 || t->bootstrap < 0 || t->bootstrap > 1 ||
+        t->bound_candidates < 0 || t->bound_candidates > 256)
         return PADDI_ERR_INVALID;
     paddi::Engine &E = *ctx->eng;
     E.select_window_ = (t->select_window + 63) / 64 * 64;
@@ -1454,6 +1461,7 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
     E.use_graph_ = t->use_graph != 0;
     E.prev_shift_ = t->prev_shift;
     E.bootstrap_ = t->bootstrap != 0;
+    E.bound_cands_ = t->bound_candidates > 0 ? t->bound_candidates : paddi::BOUND_CANDIDATES;
     return PADDI_OK;
 }
 

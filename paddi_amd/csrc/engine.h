@@ -48,7 +48,7 @@ constexpr size_t EXACT_NOALLOC_CHUNK = 4096;
 // device must be current).
 void *scratch_setup(int device);
 constexpr int64_t BOUND_WINDOW = 1024;  // columns behind the first-length candidates' bounds
-constexpr int BOUND_CANDIDATES = 16;    // candidates per series refined for the first r
+constexpr int BOUND_CANDIDATES = 32;    // default candidates per series refined for the first r
 
 // Grow-only device buffer.  A borrowed buffer is a slice of memory the engine does not own
 // (the scratch array, ScratchScope): it is never freed and never grows.
@@ -165,6 +165,7 @@ public:
     bool use_graph_ = false;         // per-length device pipeline as a hipGraph
     int prev_shift_ = 1;             // next r from previous discords at i, i-1 .. i-prev_shift
     bool bootstrap_ = true;          // learned first length: candidate lower bound on r
+    int bound_cands_ = BOUND_CANDIDATES;  // candidate rows per series behind that bound
     bool debug_ = false;             // per-pass trace on stderr (PADDI_DEBUG=1)
     double wait_limit_s_ = 600.0;    // longest wait for queued device work (PADDI_WAIT_LIMIT_S)
     std::vector<paddi_attempt_stats> *att_log_ = nullptr;  // per-attempt records, when kept

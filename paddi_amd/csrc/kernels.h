@@ -45,7 +45,7 @@ struct ScanArgs {
     int *besth, *bestl;       // global rows: ord_key of (max correlation +- its error bound)
     int *wit;                 // global rows: pruning tile recorded at this length (or null)
     const int *wit_read;      // global rows: tiles recorded at the previous length (or null)
-    unsigned long long *pairs;  // [0 .. PAIR_SLOTS) k_scan_select, [PAIR_SLOTS ..) k_scan_refine
+    unsigned long long *pairs;  // PassCtl::pairs then PassCtl::live (k_scan_select, k_scan_refine)
     int64_t WA;               // columns right of a block scanned by k_scan_select
     int64_t CH;               // columns per k_scan_refine work item (set per round on device)
     int64_t U0;               // first cyclic offset scanned: round_up(m + 64, 64)
@@ -103,6 +103,9 @@ struct PassCtl {
     int survivors;
     int pad[3];
     unsigned long long pairs[2][PAIR_SLOTS];  // [0] k_scan_select, [1] k_scan_refine
+    // the same pair evaluations restricted to rows still alive when their tile started (the
+    // useful share of the lane-steps), right after pairs
+    unsigned long long live[2][PAIR_SLOTS];
 };
 constexpr int REFINE_ROUNDS = 3;
 
@@ -151,7 +154,7 @@ struct Surv {
     float d2lo, d2hi;
 };
 // Pass results as one device block: the counters, then the survivors (one copy to the host).
-constexpr size_t PASS_OUT_HEAD = 1536;
+constexpr size_t PASS_OUT_HEAD = 2560;
 static_assert(sizeof(PassCtl) <= PASS_OUT_HEAD, "PassCtl fits the head of the pass results");
 // Per 64-row block, its row with the lowest maximum correlation found (the largest upper
 // bound on the NN distance): ((key ^ 0x80000000) << 32 | global row), ~0 for blocks without

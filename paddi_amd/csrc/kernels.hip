@@ -371,6 +371,7 @@ struct RowState {
     float mxl, mxh;// max over super-tiles of (their max of cov*invn_j -+ their bound)
     uint64_t alive;// wave-uniform alive mask of the 64 rows
     uint32_t tiles;// 64x64 tiles evaluated (pair counter)
+    uint32_t live; // rows alive at the start of those tiles, summed
     float qsum;    // sum of the staged, centred first row (L.qz)
 };
 
@@ -682,6 +683,7 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
                 continue;
             }
             R.tiles++;
+            R.live += (uint32_t)__builtin_popcountll(R.alive);
             const bool masked = (ct - s - 63 <= m && ct - s + 63 >= -m) || (ct + WAVE > N) ||
                                 (s + WAVE > N) || (ct + WAVE > c1);
             uint64_t rowhit = 0, colhit = 0;
@@ -780,6 +782,7 @@ __device__ __forceinline__ void load_rows(RowState &R, const View &V, int64_t s,
     R.mxl = -INFINITY;
     R.mxh = -INFINITY;
     R.tiles = 0;
+    R.live = 0;
 }
 
 // Row terms plus the block's centred first row in L.qz (the seeds' right operand).
@@ -823,11 +826,15 @@ __device__ __forceinline__ ScanLds carve_lds(int m)
     return L;
 }
 
+// pairs: this kernel's PassCtl::pairs row; its PassCtl::live row is 2 PAIR_SLOTS further.
 __device__ __forceinline__ void count_pairs(const RowState &R,
                                             unsigned long long *__restrict__ pairs)
 {
-    if (threadIdx.x == 0 && R.tiles)
-        atomicAdd(pairs + (blockIdx.x & (PAIR_SLOTS - 1)), (unsigned long long)R.tiles * 4096ull);
+    if (threadIdx.x == 0 && R.tiles) {
+        const int slot = blockIdx.x & (PAIR_SLOTS - 1);
+        atomicAdd(pairs + slot, (unsigned long long)R.tiles * 4096ull);
+        atomicAdd(pairs + 2 * PAIR_SLOTS + slot, (unsigned long long)R.live * 64ull);
+    }
 }
 
 // Rebase the batch onto global block gb's series.  Returns the series id (-1: padding).

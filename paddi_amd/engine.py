@@ -47,7 +47,8 @@ class Opts(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("attempts", C.c_int64), ("survivors", C.c_int64),
                 ("exact_refines", C.c_int64), ("pairs", C.c_int64), ("seconds", C.c_double),
-                ("kernel_ms", C.c_double * 4), ("launches", C.c_int64 * 4)]
+                ("kernel_ms", C.c_double * 4), ("launches", C.c_int64 * 4),
+                ("live_pairs", C.c_int64 * 2), ("scan_pairs", C.c_int64 * 2)]
 
     KERNELS = ("derive", "select", "refine", "exact")
 
@@ -55,7 +56,9 @@ class Stats(C.Structure):
         return dict(attempts=self.attempts, survivors=self.survivors,
                     exact_refines=self.exact_refines, pairs=self.pairs, seconds=self.seconds,
                     kernel_ms=dict(zip(self.KERNELS, list(self.kernel_ms))),
-                    launches=dict(zip(self.KERNELS, list(self.launches))))
+                    launches=dict(zip(self.KERNELS, list(self.launches))),
+                    scan_pairs=dict(select=self.scan_pairs[0], refine=self.scan_pairs[1]),
+                    live_pairs=dict(select=self.live_pairs[0], refine=self.live_pairs[1]))
 
 
 class Tunables(C.Structure):
@@ -63,9 +66,9 @@ class Tunables(C.Structure):
                 ("min_chunk", C.c_int64), ("refine_items", C.c_int64),
                 ("witness_tiles", C.c_int32), ("speculate", C.c_int32),
                 ("use_graph", C.c_int32), ("prev_shift", C.c_int32),
-                ("bootstrap", C.c_int32), ("reserved", C.c_int32)]
+                ("bootstrap", C.c_int32), ("bound_candidates", C.c_int32)]
     NAMES = ("select_window", "refine_chunk", "min_chunk", "refine_items", "witness_tiles",
-             "speculate", "use_graph", "prev_shift", "bootstrap")
+             "speculate", "use_graph", "prev_shift", "bootstrap", "bound_candidates")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.NAMES}
