@@ -534,7 +534,7 @@ void Engine::copy_st64(double2 *dst, int64_t N)
 // LDS class and replayed.  Eager launches are bracketed by events for the statistics;
 // a captured sequence has none (event nodes of a graph do not time the kernels between them).
 void Engine::enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t rows,
-                          int64_t nwords, int64_t scap, bool timed)
+                          int64_t nwords, int64_t scap, bool timed, const uint8_t *keep)
 {
     const PassDev *P = par_.as<PassDev>();
     PassCtl *ctl = pout_.as<PassCtl>();
@@ -545,6 +545,9 @@ void Engine::enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t ro
     HIPC(hipMemcpyAsync(par_.p, pin_par_, parb, hipMemcpyHostToDevice, stream_));
     hipLaunchKernelGGL(k_pass_init, dim3(grid1(std::max<int64_t>(rows, 1024), 256)), dim3(256), 0,
                        stream_, Q, P, rows, nwords, ctl);
+    if (keep)
+        hipLaunchKernelGGL(k_mask_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
+                           alive_.as<uint64_t>(), keep, nwords);
     if (timed) HIPC(hipEventRecord(ev_[0], stream_));
     launch_scan_select(Q, P, (unsigned)nwords, lds, stream_);
     if (timed) HIPC(hipEventRecord(ev_[1], stream_));
@@ -689,11 +692,13 @@ void Engine::bound_candidates(Batch &B, int m, const std::vector<char> &need, in
 // within m); the k-th accepted value is a lower bound of d_1 when k = 1 (any row's NN
 // distance is) and the usual estimate otherwise (a failed pass steps below it).
 void Engine::first_bound(Batch &B, int m, int k, const std::vector<char> &need,
-                         std::vector<double> &lb, paddi_stats *st)
+                         std::vector<double> &lb, paddi_stats *st,
+                         std::vector<std::vector<int64_t>> *cand_out)
 {
     lb.assign((size_t)B.nser, 0.0);
     std::vector<std::vector<int64_t>> cand;
     bound_candidates(B, m, need, bound_cands_, cand, st);
+    if (cand_out) *cand_out = cand;
     std::vector<std::pair<int, int64_t>> req;
     for (int q = 0; q < B.nser; q++)
         for (int64_t i : cand[(size_t)q]) req.emplace_back(q, i);
@@ -723,19 +728,73 @@ void Engine::first_bound(Batch &B, int m, int k, const std::vector<char> &need,
     }
 }
 
+// The first-length range of the learned form.  first_bound's candidates are each block's
+// best-scored row, usually a neighbour of the block's discord rather than the discord itself,
+// so their k-th greedy distance f falls short of d_k.  The candidate blocks are therefore
+// scanned in full at r2 = f (a pass restricted to those blocks), and the exact top-k of their
+// rows whose NN distance exceeds f gives f' >= f: d_k itself whenever the top-k lie in those
+// blocks (for k = 1 any row's NN distance is at most d_1, so f' <= d_1 always; for k > 1 a
+// range above d_k only makes the first pass fail and the descent step down).
+void Engine::bootstrap(Batch &B, int m, int k, const std::vector<char> &need,
+                       std::vector<double> &lb, std::vector<char> &found, paddi_stats *st)
+{
+    found.assign((size_t)B.nser, 0);
+    std::vector<std::vector<int64_t>> cand;
+    first_bound(B, m, k, need, lb, st, &cand);
+    std::vector<double> r2((size_t)B.nser, -1.0);
+    std::vector<uint8_t> keep((size_t)B.nblocks, 0);
+    bool any = false;
+    for (int q = 0; q < B.nser; q++) {
+        if (!need[(size_t)q] || !(lb[(size_t)q] > 0.0)) continue;
+        r2[(size_t)q] = lb[(size_t)q];
+        for (int64_t i : cand[(size_t)q]) keep[(size_t)((B.roff[(size_t)q] + i) >> 6)] = 1;
+        any = true;
+    }
+    if (!any) return;
+    std::vector<std::vector<Cand>> surv;
+    pass(B, m, r2, false, surv, st, &keep);
+    std::vector<TopK> tk((size_t)B.nser);
+    for (int q = 0; q < B.nser; q++) {
+        if (r2[(size_t)q] < 0.0) {
+            tk[(size_t)q].done = true;
+            continue;
+        }
+        tk[(size_t)q].reset(std::move(surv[(size_t)q]), r2[(size_t)q], m, k);
+        tk[(size_t)q].spec = spec_;
+    }
+    topk_all(B, m, tk, st);
+    for (int q = 0; q < B.nser; q++) {
+        const std::vector<Cand> &acc = tk[(size_t)q].acc;
+        if (r2[(size_t)q] >= 0.0 && (int)acc.size() >= k) {
+            found[(size_t)q] = 1;
+            if (acc.back().v * (1.0 - 1e-6) > lb[(size_t)q]) lb[(size_t)q] = acc.back().v * (1.0 - 1e-6);
+        }
+    }
+}
+
 // One (m, r2_s) pass: survivors per series.  Series with r2 < 0 are skipped (no rows).
 void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
-                  std::vector<std::vector<Cand>> &out, paddi_stats *st)
+                  std::vector<std::vector<Cand>> &out, paddi_stats *st,
+                  const std::vector<uint8_t> *keep)
 {
     const int64_t rows = B.rows, nwords = B.nblocks;
+    // a restricted pass runs eagerly (its block flags are not part of the captured graph)
+    const bool graph = use_graph_ && !keep;
+    const uint8_t *keep_dev = nullptr;
+    if (keep) {
+        keep_.reserve((size_t)nwords);
+        HIPC(hipMemcpyAsync(keep_.p, keep->data(), (size_t)nwords, hipMemcpyHostToDevice,
+                            stream_));
+        keep_dev = keep_.as<uint8_t>();
+    }
     // a graph is shared by the lengths of a 64-wide class, so it carries the class's LDS
-    const size_t lds = use_graph_ ? scan_lds_bytes((int)std::min<int64_t>(round_up(m, 64), MAXM))
+    const size_t lds = graph ? scan_lds_bytes((int)std::min<int64_t>(round_up(m, 64), MAXM))
                                   : scan_lds_bytes(m);
     const int64_t scap = std::min<int64_t>(SCAP, rows);
     size_t parb = 0;
     const ScanPtrs Q = fill_pass(B, m, r2, witness, select_window_, true, parb);
 
-    if (use_graph_) {
+    if (graph) {
         const std::vector<uintptr_t> sig = {
             (uintptr_t)par_.p,   (uintptr_t)pin_par_, (uintptr_t)pin_out_, (uintptr_t)pout_.p,
             (uintptr_t)alive_.p, (uintptr_t)blist_.p,
@@ -764,7 +823,7 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
         }
         HIPC(hipGraphLaunch(G.exec, stream_));
     } else {
-        enqueue_pass(Q, lds, parb, rows, nwords, scap, true);
+        enqueue_pass(Q, lds, parb, rows, nwords, scap, true, keep_dev);
     }
     wait();
     collect_derive();
@@ -779,7 +838,7 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
     const int ns = C.survivors;
     if (debug_) {
         float a = 0.f, b = 0.f;
-        if (!use_graph_) {
+        if (!graph) {
             HIPC(hipEventElapsedTime(&a, ev_[0], ev_[1]));
             HIPC(hipEventElapsedTime(&b, ev_[2], ev_[3]));
         }
@@ -809,7 +868,7 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
         }
     }
     if (st) {
-        if (!use_graph_) {  // a replayed graph is timed as a whole by the caller only
+        if (!graph) {  // a replayed graph is timed as a whole by the caller only
             float a = 0.f, b = 0.f;
             HIPC(hipEventElapsedTime(&a, ev_[0], ev_[1]));
             HIPC(hipEventElapsedTime(&b, ev_[2], ev_[3]));
@@ -818,8 +877,10 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
             st->kernel_ms[2] += b;
             st->launches[2] += REFINE_ROUNDS;
         }
-        st->attempts += 1;
-        st->survivors += ns;
+        if (!keep) {  // a candidate scan is not an (m, r) attempt of the search
+            st->attempts += 1;
+            st->survivors += ns;
+        }
         st->pairs += (int64_t)(pairs_sel + pairs_ref);
         st->scan_pairs[0] += (int64_t)pairs_sel;
         st->scan_pairs[1] += (int64_t)pairs_ref;
@@ -1198,7 +1259,7 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
         // first length (or no usable previous result): learned proposal, raised to the
         // candidates' lower bound when that is higher; else the probe descent from 2m
         std::vector<double> prop((size_t)ns, 0.0), floor2((size_t)ns, 0.0);
-        std::vector<char> need((size_t)ns, 0);
+        std::vector<char> need((size_t)ns, 0), found((size_t)ns, 0);
         bool need_prop = false;
         for (int q = 0; q < ns; q++) {
             need[(size_t)q] = !from_prev[(size_t)q];
@@ -1206,15 +1267,17 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
         }
         if (need_prop && learned) {
             propose_r2(B, m, k, prop);
-            if (bootstrap_) first_bound(B, m, k, need, floor2, st);
+            if (bootstrap_) bootstrap(B, m, k, need, floor2, found, st);
         }
         const std::vector<double> floor0 = floor2;  // for the trace
         for (int q = 0; q < ns; q++)
             if (!from_prev[(size_t)q]) {
                 double r = 2.0 * m;
                 if (learned) {
+                    // k discords found among the candidate blocks: their k-th distance; else
+                    // the proposal, raised to the candidates' bound
                     const double p = prop[(size_t)q], f = floor2[(size_t)q];
-                    r = p > 0.0 ? std::max(p, f) : (f > 0.0 ? f : r);
+                    r = found[(size_t)q] ? f : (p > 0.0 ? std::max(p, f) : (f > 0.0 ? f : r));
                 }
                 r2[(size_t)q] = r;
             }

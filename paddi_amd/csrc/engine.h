@@ -129,11 +129,18 @@ public:
     // Window statistics at m (and at m2 > 0 for k_exact's second length).
     void derive(Batch &B, int m, paddi_stats *st, int m2 = 0);
     // One (m, r2_s) pass over every series of the batch; survivors per series.
+    // keep: restrict the pass to these global blocks (a candidate scan, not an attempt).
     void pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
-              std::vector<std::vector<Cand>> &out, paddi_stats *st);
+              std::vector<std::vector<Cand>> &out, paddi_stats *st,
+              const std::vector<uint8_t> *keep = nullptr);
     // First-length lower bound of d_k per series in `need` (0 = none; see engine.cpp).
     void first_bound(Batch &B, int m, int k, const std::vector<char> &need,
-                     std::vector<double> &lb, paddi_stats *st);
+                     std::vector<double> &lb, paddi_stats *st,
+                     std::vector<std::vector<int64_t>> *cand_out = nullptr);
+    // first_bound, then the candidate blocks scanned in full at that bound (see engine.cpp)
+    // found[s]: k discords were found among series s's candidate blocks (lb[s] their k-th)
+    void bootstrap(Batch &B, int m, int k, const std::vector<char> &need,
+                   std::vector<double> &lb, std::vector<char> &found, paddi_stats *st);
     // First-length candidates of the series in `need` (see engine.cpp).
     void bound_candidates(Batch &B, int m, const std::vector<char> &need, int count,
                           std::vector<std::vector<int64_t>> &cand, paddi_stats *st);
@@ -199,7 +206,7 @@ private:
     // One pass's device work (parameter upload .. survivor download) on stream_, eagerly or
     // as a replay of the captured graph of its LDS class.
     void enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t rows,
-                      int64_t nwords, int64_t scap, bool timed);
+                      int64_t nwords, int64_t scap, bool timed, const uint8_t *keep = nullptr);
     void ensure_pinned(size_t par_bytes, size_t ser_bytes);
     ScanPtrs fill_pass(Batch &B, int m, const std::vector<double> &r2, bool witness, int64_t WA,
                        bool refine, size_t &parb);
@@ -236,6 +243,7 @@ private:
     int derive_m2_ = 0;            // that length (0: none)
     DevBuf xarg_;                  // exact-refine arguments and results (one block)
     DevBuf bkey_;                  // per-block candidate keys of bound_candidates
+    DevBuf keep_;                  // per-block flags of a restricted pass
     void *pin_x_ = nullptr;        // its pinned staging
     size_t pin_x_bytes_ = 0;
     int witcur_ = 0;

@@ -146,6 +146,9 @@ __global__ void k_pass_init(ScanPtrs Q, const PassDev *__restrict__ P, int64_t r
                             int64_t nwords, PassCtl *__restrict__ ctl);
 __global__ void k_list_blocks(const uint64_t *__restrict__ alive, int64_t nwords,
                               int *__restrict__ list, int *__restrict__ count);
+// Restrict a pass to the global blocks flagged in keep (the others' alive words cleared).
+__global__ void k_mask_blocks(uint64_t *__restrict__ alive, const uint8_t *__restrict__ keep,
+                              int64_t nwords);
 // A surviving row: global row, single-precision squared NN distance d2 and bounds
 // d2lo <= exact <= d2hi (up to the rounding of their conversion).
 struct Surv {

@@ -372,6 +372,9 @@ struct RowState {
     uint64_t alive;// wave-uniform alive mask of the 64 rows
     uint32_t tiles;// 64x64 tiles evaluated (pair counter)
     uint32_t live; // rows alive at the start of those tiles, summed
+    float wbest;   // largest cov*invn_j over the tiles evaluated so far, and its tile
+    int wtile;
+    uint64_t mine; // rows this workgroup pruned (their witness: wtile, at the end)
     float qsum;    // sum of the staged, centred first row (L.qz)
 };
 
@@ -695,6 +698,11 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
             } else {
                 tile_steps<false, COLS>(tile, R, s, ct, N, N, m, rowhit, colhit);
             }
+            if (R.mt > R.wbest) {
+                R.wbest = R.mt;
+                R.wtile = (int)ct;
+            }
+            R.mine |= rowhit & R.alive;
             if (V.wit) {
                 if (((rowhit & R.alive) >> t) & 1ull) V.wit[s + t] = (int)ct;
                 if (COLS && ((colhit >> t) & 1ull)) V.wit[ct + t] = (int)s;
@@ -783,6 +791,9 @@ __device__ __forceinline__ void load_rows(RowState &R, const View &V, int64_t s,
     R.mxh = -INFINITY;
     R.tiles = 0;
     R.live = 0;
+    R.wbest = -INFINITY;
+    R.wtile = -1;
+    R.mine = 0;
 }
 
 // Row terms plus the block's centred first row in L.qz (the seeds' right operand).
@@ -809,6 +820,14 @@ __device__ __forceinline__ void flush_best(const RowState &R, const View &V, int
         atomicMax(V.besth + s + t, ord_key(R.mxh * inv));
         atomicMax(V.bestl + s + t, ord_key(R.mxl * inv));
     }
+}
+
+// The witness of each row this workgroup pruned: the tile of its largest correlation among
+// the tiles the workgroup evaluated (at least as close a neighbour as the pruning tile's).
+__device__ __forceinline__ void record_witness(const RowState &R, const View &V, int64_t s)
+{
+    const int t = threadIdx.x;
+    if (V.wit && ((R.mine >> t) & 1ull) && R.wtile >= 0) V.wit[s + t] = R.wtile;
 }
 
 __device__ __forceinline__ ScanLds carve_lds(int m)
@@ -928,6 +947,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     }
     if (live) live = scan_cyclic<true>(L, R, V, s, A.U0, A.U0 + A.WA, A.U0);
     if (live) flush_best(R, V, s);
+    record_witness(R, V, s);
     count_pairs(R, A.pairs);
 }
 
@@ -947,6 +967,7 @@ __device__ __forceinline__ void refine_item(const ScanArgs &A, int64_t gb, int64
     load_block(L, R, V, s, thr_rho);
     const bool live = scan_cyclic<false>(L, R, V, s, u0, u1, A.U0);
     if (live) flush_best(R, V, s);
+    record_witness(R, V, s);
     count_pairs(R, A.pairs + PAIR_SLOTS);
 }
 
@@ -1056,6 +1077,14 @@ __global__ __launch_bounds__(256) void k_list_blocks(const uint64_t *__restrict_
         const int p = atomicAdd(count, 1);
         list[p] = (int)w;
     }
+}
+
+__global__ __launch_bounds__(256) void k_mask_blocks(uint64_t *__restrict__ alive,
+                                                     const uint8_t *__restrict__ keep,
+                                                     int64_t nwords)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nwords && !keep[w]) alive[w] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_block_best(ScanPtrs Q, const PassDev *__restrict__ P,

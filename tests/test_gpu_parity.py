@@ -284,18 +284,30 @@ def test_learned_proposal_saves_passes(ctx):
 
 
 def test_learned_proposal_too_high_falls_back(ctx, tmp_path):
-    """A model whose margin is hugely negative proposes r far above every discord distance:
-    the descent must fall back and still return the oracle's set."""
+    """A model whose margin is hugely negative proposes r far above every discord distance.
+    Without the candidate bootstrap the first pass runs at that proposal and the descent must
+    fall back; with it, the k discords found among the candidate blocks fix the first range
+    and the proposal is not needed.  Both return the oracle's set."""
     from paddi_amd import model as Mdl
     d = Mdl.load()
     bad = tmp_path / "bad.bin"
     Mdl.save(str(bad), d["fmu"], d["fsd"], d["params"], -3.0)
     x = synth.rw_planted(20000, 77, 2, 64, 12.0)
     ref = O.palmad(x, 64, 66, 1, 2)
-    got, st = ctx.discords(x, 64, 66, 1, 2, mode="learned", model_path=str(bad),
-                           with_stats=True)
-    assert_same_discords(got, expected(ref), "fallback")
-    assert st["attempts"] > 3
+    saved = ctx.get_tunables()
+    try:
+        ctx.set_tunables(bootstrap=0)
+        got, st = ctx.discords(x, 64, 66, 1, 2, mode="learned", model_path=str(bad),
+                               with_stats=True)
+        assert_same_discords(got, expected(ref), "fallback")
+        assert st["attempts"] > 3
+        ctx.set_tunables(bootstrap=1)
+        got, st = ctx.discords(x, 64, 66, 1, 2, mode="learned", model_path=str(bad),
+                               with_stats=True)
+        assert_same_discords(got, expected(ref), "bootstrap")
+        assert st["attempts"] == 3
+    finally:
+        ctx.set_tunables(**saved)
 
 
 def test_batched_uneven_series_match_oracle(ctx):

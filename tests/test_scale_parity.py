@@ -151,3 +151,27 @@ def test_certificate_c5_sampled_series(ctx, j):
     for q in _c5_sample(8, 77 + j):
         mine = [r for r in recs if r.series == q and r.length == m]
         _certify(w["series"][q], m, w["k"], mine, f"c5 series {q} (n={len(w['series'][q])}) m={m}")
+
+
+@pytest.mark.parametrize("name", ["c2", "c5"])
+def test_learned_first_length_evaluates_fewer_pairs(ctx, name):
+    """What the learned form gains where the first length weighs (c2: 65 lengths of one 1M-point
+    series; c5: a seeded sample of 128 of its uneven series): the candidate blocks' exact
+    top-k fixes the first range just below d_k, so the whole search evaluates fewer pairs than
+    the probe descent from 2m, with the identical discord set."""
+    w = _workload(name)["w"]
+    series = w["series"] if name == "c2" else [w["series"][q] for q in _c5_sample(128, 9)]
+    saved = ctx.get_tunables()
+    try:
+        ctx.apply_tuned(w["minL"])
+        out = {}
+        for mode in ("batched", "learned"):
+            recs, st = ctx.discords_batch(series, w["minL"], w["maxL"], w["step"], w["k"],
+                                          mode=mode, with_stats=True)
+            out[mode] = (recs, st)
+    finally:
+        ctx.set_tunables(**saved)
+    _same(out["learned"][0], out["batched"][0])
+    pb, pl = out["batched"][1]["pairs"], out["learned"][1]["pairs"]
+    print(f"{name}: pairs batched {pb:.4g}, learned {pl:.4g} ({pl / pb:.3f})", flush=True)
+    assert pl < pb
