@@ -78,7 +78,7 @@ def run_form(ctx, handles, w, maxL, mode, steps, warmup):
 
 
 # CPU baseline sample per workload: (row-run stride, number of series sampled or None = all)
-CPU_SAMPLE = {"c1": (1, None), "c2": (1, None), "c3": (4, None), "c4": (2, None), "c5": (1, 8)}
+CPU_SAMPLE = {"c1": (1, None), "c2": (1, None), "c3": (4, None), "c4": (8, None), "c5": (1, 8)}
 
 
 def cpu_model():
