@@ -16,7 +16,8 @@
  *   - a context may be used by one host thread at a time; separate contexts
  *     are independent;
  *   - row i and column j form a valid pair iff |i - 1 - j| >= m
- *     (src/cpu_kernels/cpu_kernels.cpp:57); a row is pruned by a valid pair
+ *     (src/cpu_kernels/cpu_kernels.cpp:57; paddi_set_exclusion selects the
+ *     reference GPU kernels' |i - j| >= m - 1 instead); a row is pruned by a valid pair
  *     with ED^2 <= r2 (cpu_kernels.cpp:59-72); a window with zero standard
  *     deviation has ED^2 = 2m to everything (src/PD3_kernels.cuh:63-64).
  */
@@ -116,6 +117,19 @@ paddi_status paddi_create(int32_t device, paddi_ctx **out);
 void paddi_destroy(paddi_ctx *ctx);
 paddi_status paddi_get_tunables(paddi_ctx *ctx, paddi_tunables *out);
 paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t);
+
+/* Exclusion rule of a context's later calls.  The reference has two: its CPU path pairs row
+ * i with column j iff |i - 1 - j| >= m, i.e. j - i >= m - 1 or i - j >= m + 1
+ * (src/cpu_kernels/cpu_kernels.cpp:57; the default here), its GPU kernels iff
+ * |i - j| >= m - 1 (src/PD3_kernels.cuh:70), which also pairs i - j = m - 1 and i - j = m.
+ * The top-k's non-overlap rule between accepted discords (|i - a| >= m) is the same under
+ * both.  Invalid rule: PADDI_ERR_INVALID. */
+enum {
+    PADDI_EXCL_CPU = 0,
+    PADDI_EXCL_SYMMETRIC = 1
+};
+paddi_status paddi_set_exclusion(paddi_ctx *ctx, int32_t rule);
+paddi_status paddi_get_exclusion(paddi_ctx *ctx, int32_t *rule);
 
 /* Sliding mean and standard deviation of the n-m+1 windows of T.
  * Replaces compute_statistics / compute_statistics_cpu

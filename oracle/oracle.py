@@ -150,19 +150,22 @@ def znorm_windows(T, m):
     return z, sd[:, 0]
 
 
-def brute_profile(T, m):
+def brute_profile(T, m, symmetric=False):
     """NN squared z-ED of every row by direct vector differences, with the
-    exclusion j - i >= m-1 or i - j >= m+1.  Degenerate (sd == 0) pairs are 2m."""
+    exclusion j - i >= m-1 or i - j >= m+1 (src/cpu_kernels/cpu_kernels.cpp:57), or with
+    symmetric=True the reference GPU kernels' |i - j| >= m-1 (src/PD3_kernels.cuh:70).
+    Degenerate (sd == 0) pairs are 2m."""
     z, sd = znorm_windows(T, m)
     N = z.shape[0]
     nn = np.full(N, np.inf)
     arg = np.full(N, -1, np.int64)
+    left = m - 1 if symmetric else m + 1
     for i in range(N):
         d = ((z - z[i]) ** 2).sum(axis=1)
         deg = (sd == 0) | (sd[i] == 0)
         d[deg] = 2.0 * m
         j = np.arange(N)
-        ok = (j - i >= m - 1) | (i - j >= m + 1)
+        ok = (j - i >= m - 1) | (i - j >= left)
         d = np.where(ok, d, np.inf)
         a = int(np.argmin(d))
         nn[i], arg[i] = d[a], a

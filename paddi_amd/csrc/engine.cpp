@@ -617,6 +617,7 @@ ScanPtrs Engine::fill_pass(Batch &B, int m, const std::vector<double> &r2, bool 
     A.WA = WA;
     A.U0 = round_up(m + 64, 64);
     A.m = m;
+    A.ml = excl_sym_ ? m - 1 : m + 1;
     A.witness_max = witness_tiles_;
     A.wsel = witness ? witcur_ : -1;
     A.rsel = (witness && wread_valid_) ? (witcur_ ^ 1) : -1;
@@ -1006,6 +1007,7 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
         E.ngroups = L.ngroups;
         E.m = m;
         E.m2 = m2;
+        E.sym = excl_sym_ ? 1 : 0;
         launch_exact(E, L.nu, (unsigned)L.blocks, exact_lds_bytes(m, L.nu), stream_);
         HIPC(hipGetLastError());
     }
@@ -1502,6 +1504,21 @@ paddi_status paddi_get_tunables(paddi_ctx *ctx, paddi_tunables *out)
     *out = paddi_tunables{E.select_window_, E.refine_chunk_, E.min_chunk_, E.refine_waves_,
                           E.witness_tiles_, E.spec_, E.use_graph_ ? 1 : 0, E.prev_shift_,
                           E.bootstrap_ ? 1 : 0, E.bound_cands_};
+    return PADDI_OK;
+}
+
+paddi_status paddi_set_exclusion(paddi_ctx *ctx, int32_t rule)
+{
+    if (!ctx || (rule != PADDI_EXCL_CPU && rule != PADDI_EXCL_SYMMETRIC))
+        return PADDI_ERR_INVALID;
+    ctx->eng->excl_sym_ = rule == PADDI_EXCL_SYMMETRIC;
+    return PADDI_OK;
+}
+
+paddi_status paddi_get_exclusion(paddi_ctx *ctx, int32_t *rule)
+{
+    if (!ctx || !rule) return PADDI_ERR_INVALID;
+    *rule = ctx->eng->excl_sym_ ? PADDI_EXCL_SYMMETRIC : PADDI_EXCL_CPU;
     return PADDI_OK;
 }
 

@@ -173,6 +173,7 @@ public:
     int prev_shift_ = 1;             // next r from previous discords at i, i-1 .. i-prev_shift
     bool bootstrap_ = true;          // learned first length: candidate lower bound on r
     int bound_cands_ = BOUND_CANDIDATES;  // candidate rows per series behind that bound
+    bool excl_sym_ = false;          // exclusion rule: |i - j| >= m - 1 instead of |i - 1 - j| >= m
     bool debug_ = false;             // per-pass trace on stderr (PADDI_DEBUG=1)
     double wait_limit_s_ = 600.0;    // longest wait for queued device work (PADDI_WAIT_LIMIT_S)
     std::vector<paddi_attempt_stats> *att_log_ = nullptr;  // per-attempt records, when kept

@@ -50,6 +50,7 @@ struct ScanArgs {
     int64_t CH;               // columns per k_scan_refine work item (set per round on device)
     int64_t U0;               // first cyclic offset scanned: round_up(m + 64, 64)
     int m;
+    int ml;                   // ScanScal::ml
     int witness_max;          // witness ranges scanned first
     float dk;                 // delta(m) / (2m): error bound per unit of block and tile scale
 };
@@ -81,6 +82,11 @@ struct ScanPtrs {
 struct ScanScal {
     int64_t WA, U0;
     int m;
+    // Exclusion: row i and column j form a pair iff j - i >= m - 1 or i - j >= ml; ml = m + 1
+    // is the reference CPU rule |i - 1 - j| >= m (src/cpu_kernels/cpu_kernels.cpp:57), ml = m - 1
+    // its GPU kernels' |i - j| >= m - 1 (src/PD3_kernels.cuh:70).  The scans' masked tiles cover
+    // offsets within m + 63 either side of the diagonal, which holds both bands.
+    int ml;
     int witness_max;
     int wsel, rsel;           // witness buffer written / read (0, 1; -1 = none)
     int some_off;             // 1 when the pass switches some series off
@@ -125,6 +131,7 @@ struct ExactArgs {
     int ngroups;
     int m;
     int m2;                   // 0: one length
+    int sym;                  // exclusion rule: 0 the CPU rule, 1 the symmetric one (ScanScal::ml)
 };
 constexpr int EXACT_EXT = 4;  // largest m2 - m
 

@@ -388,6 +388,7 @@ struct View {
     int *wit;
     int64_t N;
     int m;
+    int ml;       // row i and column j pair iff j - i >= m - 1 or i - j >= ml (ScanScal::ml)
     float dk;
 };
 
@@ -397,8 +398,8 @@ struct View {
 // far columns it scans belong to blocks that are almost all dead by then).
 template <bool MASKED, bool COLS>
 __device__ __forceinline__ void tile_steps(const float4 *tile, RowState &R, int64_t s, int64_t c,
-                                           int64_t N, int64_t climit, int m, uint64_t &rowhit,
-                                           uint64_t &colhit)
+                                           int64_t N, int64_t climit, int m, int ml,
+                                           uint64_t &rowhit, uint64_t &colhit)
 {
     const int t = threadIdx.x;
     float cov = R.cov, mx = R.mt;
@@ -425,12 +426,12 @@ __device__ __forceinline__ void tile_steps(const float4 *tile, RowState &R, int6
         } else {
             const int d = dbase + j;
             const bool colin = j < colmax;
-            const bool rv = colin && ((d >= m - 1) || (d <= -(m + 1)));
+            const bool rv = colin && ((d >= m - 1) || (d <= -ml));
             mx = rv ? fmaxf(mx, cj) : mx;
             const uint64_t hr = __builtin_amdgcn_ballot_w64(hit && rv);
             rowhit |= hr;
             if (COLS) {
-                const bool cv = colin && rowin && ((d >= m + 1) || (d <= -(m - 1)));
+                const bool cv = colin && rowin && ((d >= ml) || (d <= -(m - 1)));
                 const uint64_t hc = __builtin_amdgcn_ballot_w64(hit && cv);
                 colhit |= (hc != 0) ? (1ull << j) : 0ull;
             }
@@ -692,11 +693,11 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
             uint64_t rowhit = 0, colhit = 0;
             if (masked) {
                 const int64_t lim = c1 < N ? c1 : N;
-                tile_steps<true, COLS>(tile, R, s, ct, N, lim, m, rowhit, colhit);
+                tile_steps<true, COLS>(tile, R, s, ct, N, lim, m, V.ml, rowhit, colhit);
             } else if (COLS) {
                 tile_steps_bc(tile, R, rowhit, colhit);
             } else {
-                tile_steps<false, COLS>(tile, R, s, ct, N, N, m, rowhit, colhit);
+                tile_steps<false, COLS>(tile, R, s, ct, N, N, m, V.ml, rowhit, colhit);
             }
             if (R.mt > R.wbest) {
                 R.wbest = R.mt;
@@ -874,6 +875,7 @@ __device__ __forceinline__ int make_view(const ScanArgs &A, int64_t gb, View &V,
     V.wit = A.wit ? A.wit + S.roff : nullptr;
     V.N = S.N;
     V.m = A.m;
+    V.ml = A.ml;
     s = gb * WAVE - S.roff;
     thr = S.thr;
     return sid;
@@ -902,6 +904,7 @@ __device__ __forceinline__ ScanArgs load_args(const ScanPtrs &Q, const PassDev *
     A.CH = 0;
     A.U0 = S.U0;
     A.m = S.m;
+    A.ml = S.ml;
     A.witness_max = S.witness_max;
     A.dk = S.dk;
     return A;
@@ -1291,6 +1294,8 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     const int m2 = DUAL ? E.m2 : 0;
     const int ext = DUAL ? m2 - m : 0;
     const int64_t N2 = DUAL ? S.n - m2 + 1 : 0;
+    // pairs iff j - i >= m - 1 or i - j >= ml (ExactArgs::sym)
+    const int ml = E.sym ? m - 1 : m + 1, ml2 = E.sym ? m2 - 1 : m2 + 1;
     const double2 *st64n = E.st64n + S.roff;
     const int64_t cblk = (item - E.goff[grp]) * EXACT_COLS;
     const double cref = (double)x[cblk];   // the columns' offset (cblk < N)
@@ -1401,7 +1406,7 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
 #pragma unroll
             for (int u = 0; u < NU; u++) {
                 const int64_t i = cidx[u];
-                const bool valid = (j < N) && ((j - i >= m - 1) || (i - j >= m + 1));
+                const bool valid = (j < N) && ((j - i >= m - 1) || (i - j >= ml));
                 const double cov = fma(-(sj.x - cref), asa[u], acc[u][q]);
                 const double ed = valid ? pair_ed2(cov, m, ainv[u], sj.y) : DBL_MAX;
                 const unsigned long long key = ord_key64(ed);
@@ -1412,7 +1417,7 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
                     for (int e = 0; e < EXACT_EXT; e++)
                         if (e < ext) a2 = fma(aext[u][e], xe[e], a2);
                     const bool valid2 =
-                        (i < N2) && (j < N2) && ((j - i >= m2 - 1) || (i - j >= m2 + 1));
+                        (i < N2) && (j < N2) && ((j - i >= m2 - 1) || (i - j >= ml2));
                     const double cov2 = fma(-(sj2.x - cref), asb[u], a2);
                     const double ed2 = valid2 ? pair_ed2(cov2, m2, ainv2[u], sj2.y) : DBL_MAX;
                     const unsigned long long key2 = ord_key64(ed2);

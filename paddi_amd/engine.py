@@ -88,7 +88,7 @@ EXPORTS = ("paddi_get_tunables", "paddi_set_tunables", "paddi_status_str", "padd
            "paddi_discords_batch", "paddi_series_upload", "paddi_series_free",
            "paddi_discords_series", "paddi_has_mode", "paddi_mlp_sizes", "paddi_features",
            "paddi_mlp_forward", "paddi_mlp_gradient", "paddi_mlp_train",
-           "paddi_range_discords_noalloc")
+           "paddi_range_discords_noalloc", "paddi_set_exclusion", "paddi_get_exclusion")
 
 _lib = None
 
@@ -124,6 +124,8 @@ def load_library(path: str = LIB_PATH):
     L.paddi_has_mode.restype = i32
     L.paddi_get_tunables.argtypes = [P, C.POINTER(Tunables)]
     L.paddi_set_tunables.argtypes = [P, C.POINTER(Tunables)]
+    L.paddi_set_exclusion.argtypes = [P, i32]
+    L.paddi_get_exclusion.argtypes = [P, C.POINTER(i32)]
     L.paddi_mlp_sizes.argtypes = [P, P, P]
     L.paddi_features.argtypes = [P, P, i32, i32, i32, P]
     L.paddi_mlp_forward.argtypes = [P, P, P, P, P, i32, P]
@@ -222,6 +224,21 @@ class Context:
                 raise KeyError(k)
             setattr(t, k, int(v))
         self._check(self._L.paddi_set_tunables(self._c, C.byref(t)), "paddi_set_tunables")
+
+    # -- exclusion rule ----------------------------------------------------------------
+
+    EXCLUSION = {"cpu": 0, "symmetric": 1}
+
+    def set_exclusion(self, rule):
+        """"cpu": pair iff |i-1-j| >= m (src/cpu_kernels/cpu_kernels.cpp:57, the default);
+        "symmetric": iff |i-j| >= m-1 (src/PD3_kernels.cuh:70)."""
+        code = self.EXCLUSION[rule] if isinstance(rule, str) else int(rule)
+        self._check(self._L.paddi_set_exclusion(self._c, code), "paddi_set_exclusion")
+
+    def get_exclusion(self):
+        v = C.c_int32()
+        self._check(self._L.paddi_get_exclusion(self._c, C.byref(v)), "paddi_get_exclusion")
+        return {c: n for n, c in self.EXCLUSION.items()}[v.value]
 
     def apply_tuned(self, minL, path=TUNED_PATH):
         """Apply the autotuner's entry for this length range, if the table has one."""

@@ -134,6 +134,26 @@ def test_planted_mistake_is_caught():
     assert nn_ref[i] > 0.1 and nn[i] == pytest.approx(nn_ref[i], rel=1e-8)
     # and the earlier row sees the copy at +(m-1) under both rules
     assert nn[a] < 1e-8
+    # the brute force's symmetric rule (the engine's alternative) finds the planted pair
+    nn_sym, arg_sym = O.brute_profile(x, m, symmetric=True)
+    assert nn_sym[i] < 1e-8 and arg_sym[i] == a
+
+
+def test_symmetric_rule_differs_only_at_two_offsets():
+    """|i-j| >= m-1 (src/PD3_kernels.cuh:70) pairs everything |i-1-j| >= m does plus
+    i-j = m-1 and i-j = m: its profile is never larger, and where it is smaller the new
+    nearest neighbour sits at one of those two offsets."""
+    rng = np.random.default_rng(17)
+    x = np.cumsum(rng.standard_normal(500)).astype(np.float32)
+    for m in (8, 16):
+        nn, _ = O.brute_profile(x, m)
+        nn_sym, arg_sym = O.brute_profile(x, m, symmetric=True)
+        assert np.all(nn_sym <= nn)
+        lower = nn_sym < nn
+        assert lower.any()
+        rows = np.nonzero(lower)[0]
+        assert set((rows - arg_sym[rows]).tolist()) <= {m - 1, m}
+        np.testing.assert_array_equal(nn_sym[~lower], nn[~lower])
 
 
 # --- top-k (sets_processing.hpp:11-46) and r-independence ----------------------------
