@@ -283,7 +283,9 @@ def main():
                      "bytes_per_launch": bytes_per_launch},
         "tunables": tunables,
         "valu": {"pairs_per_step": pairs, "ops_per_pair": OPS_PER_PAIR,
-                 "frac_of_peak": valu, "live_row_share": useful},
+                 "frac_of_peak": valu, "live_row_share": useful,
+                 "scan_pairs_per_step": {k: sum(s["scan_pairs"][k] for s in stats) / len(stats)
+                                         for k in ("select", "refine")}},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, recs, a.workload)

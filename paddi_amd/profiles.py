@@ -4,6 +4,7 @@
     python3 -m paddi_amd.profiles stats CSV OUT.md CMD  # kernel-trace stats -> markdown table
     python3 -m paddi_amd.profiles traffic OUT.json KERNEL FETCH.csv WRITE.csv COPYF.csv COPYW.csv CMD
     python3 -m paddi_amd.profiles util OUT.md TRAFFIC.json BUSY.csv CMD
+    python3 -m paddi_amd.profiles counters OUT.md CMD PMC.csv [PMC.csv ...]
 
 `util` combines the traffic summary with a `rocprofv3 --pmc VALUBusy SALUBusy MfmaUtil` run
 (the derived metrics, per dispatch) into per-kernel HBM GB/s against the 8 TB/s peak and
@@ -132,6 +133,37 @@ def util(out, traffic_json, busy_csv, cmd):
     print("wrote", out)
 
 
+def counters(out, cmd, *csvs):
+    """Per-kernel means per dispatch of the SQ_ counters in one or more `rocprofv3 --pmc`
+    runs, and each instruction count per wave (SQ_INSTS_* / SQ_WAVES of the same run)."""
+    table = defaultdict(dict)
+    for path in csvs:
+        per = defaultdict(lambda: defaultdict(list))
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                name = r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+                per[name][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        for k, cs in per.items():
+            waves = _mean(cs.get("SQ_WAVES", []))
+            for c, v in cs.items():
+                table[k][c] = (_mean(v), len(v))
+                if c.startswith("SQ_INSTS_") and waves > 0:
+                    table[k][c + " / wave"] = (_mean(v) / waves, len(v))
+    cols = sorted({c for v in table.values() for c in v})
+    with open(out, "w") as f:
+        f.write("# SQ counters per dispatch (rocprofv3 --pmc)\n\nCommands:\n\n```\n"
+                f"{cmd}\n```\n\nMean per dispatch; `/ wave` divides by SQ_WAVES of the same "
+                "run.\n\n")
+        for k in sorted(table, key=lambda k: -table[k].get("SQ_WAVES", (0, 0))[0]):
+            f.write(f"## {k}\n\n| counter | mean per dispatch | dispatches |\n|---|---:|---:|\n")
+            for c in cols:
+                if c in table[k]:
+                    v, n = table[k][c]
+                    f.write(f"| {c} | {v:.4g} | {n} |\n")
+            f.write("\n")
+    print("wrote", out)
+
+
 def stats(path, out, cmd):
     rows = list(csv.DictReader(open(path)))
     with open(out, "w") as f:
@@ -156,5 +188,7 @@ if __name__ == "__main__":
         traffic(*sys.argv[2:9])
     elif what == "util":
         util(*sys.argv[2:6])
+    elif what == "counters":
+        counters(*sys.argv[2:])
     else:
         raise SystemExit(__doc__)
