@@ -146,9 +146,10 @@ paddi_status paddi_compute_statistics(paddi_ctx *ctx, const float *T, int64_t n,
  * single-GPU do_PD3 + refine pipeline (src/PD3.cuh:15-61):
  * is_discord[i] = 1 iff every valid pair of row i has ED^2 > r2, and then
  * nn_dist2[i] is row i's nearest-neighbour ED^2 (FLT_MAX when row i has no
- * valid pair): the single-precision scans' value where their error bound keeps it
- * above r2, the double-precision value for rows within that bound of r2.
- * nn_dist2 of rows with is_discord[i] = 0 is unspecified. */
+ * valid pair): the double-precision value for rows within the single-precision scans'
+ * error bound of r2 and for distances below m/5 (up to 65536 such rows per call), the
+ * scans' single-precision value elsewhere (its error is a few 1e-6 of 2m, below 1e-4
+ * relative above m/5).  nn_dist2 of rows with is_discord[i] = 0 is unspecified. */
 paddi_status paddi_range_discords(paddi_ctx *ctx, const float *T, int64_t n, int32_t m, float r2,
                                   int32_t *is_discord, float *nn_dist2);
 

@@ -95,6 +95,8 @@ def main():
                 if t < best_t * 0.98:
                     best_t, best, changed = t, cand, True
                     print(f"[tune] {name}={v}: {t * 1e3:.1f} ms (new best)", flush=True)
+                else:
+                    print(f"[tune] {name}={v}: {t * 1e3:.1f} ms", flush=True)
         if not changed:
             break
     table = {}

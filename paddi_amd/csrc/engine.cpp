@@ -499,7 +499,7 @@ void Engine::derive(Batch &B, int m, paddi_stats *st, int m2)
     derive_m2_ = m2;
     rho_.reserve((size_t)B.nblocks * sizeof(float));
     HIPC(hipEventRecord(ev_[4], stream_));
-    hipLaunchKernelGGL(k_derive, dim3(grid1(B.rows, 256)), dim3(256), 0, stream_,
+    hipLaunchKernelGGL(k_derive, dim3(grid1(B.rows, 256 * DERIVE_RPT)), dim3(256), 0, stream_,
                        B.x.as<float>(), B.ser.as<SerDev>(), B.bser.as<int>(), B.P.as<double2>(),
                        B.refs.as<double4>(), B.rows, m, colv_.as<float4>(),
                        have ? nullptr : st64_.as<double2>(), tmax_.as<float2>(), m2,
@@ -791,7 +791,9 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
     // a graph is shared by the lengths of a 64-wide class, so it carries the class's LDS
     const size_t lds = graph ? scan_lds_bytes((int)std::min<int64_t>(round_up(m, 64), MAXM))
                                   : scan_lds_bytes(m);
-    const int64_t scap = std::min<int64_t>(SCAP, rows);
+    // The survivors come back with the counters in one download, sized from the previous
+    // pass's count (a graph keeps the largest size it was captured with).
+    const int64_t scap = std::min<int64_t>(graph ? SCAP : scap_next_, rows);
     size_t parb = 0;
     const ScanPtrs Q = fill_pass(B, m, r2, witness, select_window_, true, parb);
 
@@ -837,6 +839,7 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
         live_ref += (double)C.live[1][q];
     }
     const int ns = C.survivors;
+    scap_next_ = std::min<int64_t>(SCAP, std::max<int64_t>(SCAP_MIN, 2 * (int64_t)ns + 64));
     if (debug_) {
         float a = 0.f, b = 0.f;
         if (!graph) {
@@ -1382,22 +1385,29 @@ void Engine::range(Batch &B, int m, float r2, int32_t *is_discord, float *nn_dis
     pass(B, m, {(double)r2}, false, surv, nullptr);
     std::fill(is_discord, is_discord + N, 0);
     std::fill(nn_dist2, nn_dist2 + N, 3.4028235e38f);
-    std::vector<std::pair<int, int64_t>> amb;
+    // Rows within the scans' error bound of r2 are decided in double precision.  So are the
+    // small distances of range discords: the single-precision value's error is a few 1e-6 of
+    // 2m whatever the distance, more than 1e-4 relative below about m/5 (up to
+    // RANGE_EXACT_CAP such rows per call; beyond that they keep the single-precision value).
+    std::vector<std::pair<int, int64_t>> amb, low;
     for (const Cand &c : surv[0]) {
         if (c.a >= 3.0e38f || (double)c.alo - conv_ed2(m, c.alo) > (double)r2) {
             is_discord[c.i] = 1;
             nn_dist2[c.i] = c.a;
+            if ((double)c.a < 0.2 * m) low.emplace_back(0, c.i);
         } else if ((double)c.ahi + conv_ed2(m, c.ahi) > (double)r2) {
             amb.emplace_back(0, c.i);
         }
     }
+    const size_t namb = amb.size();
+    if (low.size() <= RANGE_EXACT_CAP) amb.insert(amb.end(), low.begin(), low.end());
     if (!amb.empty()) {
         std::vector<double> v;
         exact(B, m, amb, v, nullptr);
         for (size_t q = 0; q < amb.size(); q++)
-            if (v[q] > (double)r2) {
+            if (q >= namb || v[q] > (double)r2) {
                 is_discord[amb[q].second] = 1;
-                nn_dist2[amb[q].second] = (float)v[q];
+                nn_dist2[amb[q].second] = v[q] >= 3.0e38 ? 3.4028235e38f : (float)v[q];
             }
     }
 }

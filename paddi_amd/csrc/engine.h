@@ -36,7 +36,14 @@ const char *status_str(paddi_status s);
 double delta_ed2(int m);
 double conv_ed2(int m, double d);
 
-constexpr int64_t SCAP = 8192;   // survivors returned with a pass's counters (more: a 2nd copy)
+// Survivors returned with a pass's counters (more: a second copy).  The download is sized
+// from the previous pass's count, between SCAP_MIN and SCAP: on MI355X a device-to-pinned
+// copy of the 200 KB an 8192-survivor head needs started about 50 us later than one of a few
+// KB, every pass.
+constexpr int64_t SCAP = 2048;
+// Range discords whose small distance is refined in double precision, per call (see range()).
+constexpr size_t RANGE_EXACT_CAP = 65536;
+constexpr int64_t SCAP_MIN = 256;
 
 // No-allocation range call (paddi_range_discords_noalloc): its device buffers are slices of a
 // file-scope device array of SCRATCH_BYTES (scratch.hip), its exact refinements run in
@@ -173,6 +180,7 @@ public:
     int prev_shift_ = 1;             // next r from previous discords at i, i-1 .. i-prev_shift
     bool bootstrap_ = true;          // learned first length: candidate lower bound on r
     int bound_cands_ = BOUND_CANDIDATES;  // candidate rows per series behind that bound
+    int64_t scap_next_ = SCAP;       // survivors downloaded with the next pass's counters
     bool excl_sym_ = false;          // exclusion rule: |i - j| >= m - 1 instead of |i - 1 - j| >= m
     bool debug_ = false;             // per-pass trace on stderr (PADDI_DEBUG=1)
     double wait_limit_s_ = 600.0;    // longest wait for queued device work (PADDI_WAIT_LIMIT_S)

@@ -135,6 +135,7 @@ struct ExactArgs {
 };
 constexpr int EXACT_EXT = 4;  // largest m2 - m
 
+constexpr int DERIVE_RPT = 4;  // rows per k_derive thread (blocks of 256 DERIVE_RPT rows)
 // Blocked prefix sums of every series; *bad becomes nonzero if a value is NaN or infinite.
 __global__ void k_prefix(const float *__restrict__ x, const SerDev *__restrict__ ser,
                          const int *__restrict__ pser, double2 *__restrict__ P,

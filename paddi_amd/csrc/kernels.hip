@@ -250,7 +250,10 @@ __device__ __forceinline__ float wave_prefix_max(float x)
 // Per-length terms for every global row (padding rows get zeros), and per 64-row block
 // (one wave; rows is a multiple of 64) the largest sqrt(M2) and invn of its rows; with
 // m2 > 0 also (mu, invn) at length m2 into st64n (k_exact's second length).  st64 = null
-// skips (mu, invn) at m (already there from the previous length's st64n).
+// skips (mu, invn) at m (already there from the previous length's st64n).  A thread takes
+// DERIVE_RPT rows 256 apart (independent loads in flight); the recurrence term dg needs the
+// previous window's mean, mu_{i-1} = mu_i + (x[i-1] - x[i+m-1]) / m, whose double-precision
+// rounding vanishes in the single-precision dg.
 __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
                                                 const SerDev *__restrict__ ser,
                                                 const int *__restrict__ bser,
@@ -261,46 +264,48 @@ __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
                                                 float2 *__restrict__ tmax, int m2,
                                                 double2 *__restrict__ st64n)
 {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    float4 cv = make_float4(0.f, 0.f, 0.f, 0.f);
-    double2 sv = make_double2(0.0, 0.0), sv2 = make_double2(0.0, 0.0);
-    float sd = 0.f;
-    const int sid = g < rows ? bser[g >> 6] : -1;
-    if (sid >= 0) {
-        const SerDev S = ser[sid];
-        const int64_t i = g - S.roff;
-        if (i < S.n - m + 1) {
-            const float *xs = x + S.xoff;
-            const double2 *Ps = P + S.pboff * (PB + 1);
-            const double4 *rs = refs + S.pboff;
-            const double invm = 1.0 / m;
-            double mu, invn, M2;
-            window_stats(Ps, rs, xs, i, m, invm, &mu, &invn, &M2);
-            double df = 0.0, dg = 0.0;
-            if (i > 0) {
-                double mup, invp;
-                window_stats(Ps, rs, xs, i - 1, m, invm, &mup, &invp);
-                const double xin = (double)xs[i + m - 1], xout = (double)xs[i - 1];
-                df = 0.5 * (xin - xout);
-                dg = (xin - mu) + (xout - mup);
-            }
-            cv = make_float4((float)df, (float)dg, (float)invn, (float)mu);
-            sv = make_double2(mu, invn);
-            sd = sqrtf((float)M2);
-            if (m2 && i < S.n - m2 + 1) {
-                double mu2, inv2;
-                window_stats(Ps, rs, xs, i, m2, 1.0 / m2, &mu2, &inv2);
-                sv2 = make_double2(mu2, inv2);
+    const double invm = 1.0 / m, invm2 = m2 ? 1.0 / m2 : 0.0;
+#pragma unroll
+    for (int q = 0; q < DERIVE_RPT; q++) {
+        const int64_t g = ((int64_t)blockIdx.x * DERIVE_RPT + q) * blockDim.x + threadIdx.x;
+        float4 cv = make_float4(0.f, 0.f, 0.f, 0.f);
+        double2 sv = make_double2(0.0, 0.0), sv2 = make_double2(0.0, 0.0);
+        float sd = 0.f;
+        const int sid = g < rows ? bser[g >> 6] : -1;
+        if (sid >= 0) {
+            const SerDev S = ser[sid];
+            const int64_t i = g - S.roff;
+            if (i < S.n - m + 1) {
+                const float *xs = x + S.xoff;
+                const double2 *Ps = P + S.pboff * (PB + 1);
+                const double4 *rs = refs + S.pboff;
+                double mu, invn, M2;
+                window_stats(Ps, rs, xs, i, m, invm, &mu, &invn, &M2);
+                double df = 0.0, dg = 0.0;
+                if (i > 0) {
+                    const double xin = (double)xs[i + m - 1], xout = (double)xs[i - 1];
+                    const double mup = mu + (xout - xin) * invm;
+                    df = 0.5 * (xin - xout);
+                    dg = (xin - mu) + (xout - mup);
+                }
+                cv = make_float4((float)df, (float)dg, (float)invn, (float)mu);
+                sv = make_double2(mu, invn);
+                sd = sqrtf((float)M2);
+                if (m2 && i < S.n - m2 + 1) {
+                    double mu2, inv2;
+                    window_stats(Ps, rs, xs, i, m2, invm2, &mu2, &inv2);
+                    sv2 = make_double2(mu2, inv2);
+                }
             }
         }
+        if (g < rows) {
+            colv[g] = cv;
+            if (st64) st64[g] = sv;
+            if (m2) st64n[g] = sv2;
+        }
+        const float a = wave_max(sd), b = wave_max(cv.z);
+        if ((threadIdx.x & 63) == 0 && g < rows) tmax[g >> 6] = make_float2(a, b);
     }
-    if (g < rows) {
-        colv[g] = cv;
-        if (st64) st64[g] = sv;
-        if (m2) st64n[g] = sv2;
-    }
-    const float a = wave_max(sd), b = wave_max(cv.z);
-    if ((threadIdx.x & 63) == 0 && g < rows) tmax[g >> 6] = make_float2(a, b);
 }
 
 
@@ -343,6 +348,20 @@ __device__ __forceinline__ float corr_lanes(const ScanLds &L, const float *v,
     return acc - (mu_lane - ref) * vsum;
 }
 
+// Sum over the wave, uniform: quad butterflies and the row mirrors give every lane its row's
+// sum, row_bcast:15 and :31 carry the rows into lane 63.  DPP needs no lane-address registers
+// (a shuffle's do, and the scan kernels spilled them).
+__device__ __forceinline__ float wave_sum(float v)
+{
+    v += dpp_z<0xB1, 0xF, 0xF>(v);   // quad_perm [1,0,3,2]
+    v += dpp_z<0x4E, 0xF, 0xF>(v);   // quad_perm [2,3,0,1]
+    v += dpp_z<0x141, 0xF, 0xF>(v);  // row_half_mirror
+    v += dpp_z<0x140, 0xF, 0xF>(v);  // row_mirror
+    v += dpp_z<0x142, 0xA, 0xF>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_z<0x143, 0xC, 0xF>(v);  // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // Stage dst[k] = x[p+k] - mu_p and return sum dst (wave-reduced, uniform).
 __device__ __forceinline__ float stage_centred(float *dst, const float *__restrict__ x,
                                                int64_t p, int m, float mu_p)
@@ -354,8 +373,7 @@ __device__ __forceinline__ float stage_centred(float *dst, const float *__restri
         dst[k] = v;
         s += v;
     }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    s = wave_sum(s);
     __syncthreads();
     return s;
 }
@@ -712,7 +730,9 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
                 __hip_atomic_fetch_and(V.alive + (ct >> 6), ~colhit, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
             // Rows pruned elsewhere, as read back one tile ago, and rows pruned here now.
-            const uint64_t pend = __shfl(pend_lane0, 0, 64);
+            const uint64_t pend =
+                ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(pend_lane0 >> 32), 0) << 32) |
+                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pend_lane0, 0);
             const uint64_t newdead = rowhit & R.alive;
             R.alive &= pend & ~rowhit;
             if (t == 0) {
@@ -936,7 +956,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             const uint64_t has = __builtin_amdgcn_ballot_w64(w >= 0 && w < lim && !(w & 63));
             if (!has) break;
             const int lane = __builtin_ctzll(has);
-            const int wt = __shfl(w, lane, 64);
+            const int wt = __builtin_amdgcn_readlane(w, lane);
             int we = wt + WAVE;
             if (w == wt) w = -1;
             for (int ext = 0; ext < 3; ext++) {
