@@ -335,6 +335,7 @@ __device__ __forceinline__ float corr_lanes(const ScanLds &L, const float *v,
     __syncthreads();
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
     int k = 0;
+#pragma unroll 2
     for (; k + 4 <= m; k += 4) {
         const float4 qv = *reinterpret_cast<const float4 *>(v + k);
         a0 = fmaf(qv.x, L.xs[t + k], a0);
@@ -390,6 +391,7 @@ struct RowState {
     uint64_t alive;// wave-uniform alive mask of the 64 rows
     uint32_t tiles;// 64x64 tiles evaluated (pair counter)
     uint32_t live; // rows alive at the start of those tiles, summed
+    uint32_t dtiles; // 64-column tiles of a single live row evaluated directly (refine_item)
     float wbest;   // largest cov*invn_j over the tiles evaluated so far, and its tile
     int wtile;
     uint64_t mine; // rows this workgroup pruned (their witness: wtile, at the end)
@@ -574,12 +576,12 @@ __device__ __forceinline__ int ring_at(int f) { return f + (f >> 4); }
 // D[a][j] = sum_k q[k] (x[c+16a+j+k] - x[c+16a]).  Lane l supplies A[l%16][l/16 + 4i] and
 // B[l/16 + 4i][l%16] and receives D[4(l/16)+r][l%16] in register r.  Column n = 16a + j then
 // has seed D[a][j] - (mu_{c+n} - x[c+16a]) * sum(q), applied per tile.  r0 = c mod W.
-__device__ __forceinline__ void super_seeds(const ScanLds &L, int r0)
+__device__ __forceinline__ void super_seeds(const ScanLds &L, int r0, const float *qz)
 {
     typedef float f4 __attribute__((ext_vector_type(4)));
     const int l = threadIdx.x, i = l & 15, kk = l >> 4;
     const float *ap = L.ring + ring_at(r0 + 16 * i) + kk;
-    const float *bp = L.qz + 16 + kk - i;
+    const float *bp = qz + 16 + kk - i;
     const float refi = L.ring[ring_at(r0 + 16 * i)];
     f4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
     int k0 = 0;
@@ -672,7 +674,7 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
     for (int64_t c = cs; c < c1; c += 256) {
         const int r0 = (int)(c % W);
         __syncthreads();  // ring writes of the previous super-tile are visible
-        super_seeds(L, r0);
+        super_seeds(L, r0, L.qz);
         // The rows' thresholds raised by the super-tile's error bound on cov*invn_j: the
         // lane's row scale times the largest column factor of its tiles (tiles start at
         // multiples of 64, so tile T is block (c + 64 T) / 64's columns; positive floats
@@ -767,15 +769,160 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
     return true;
 }
 
+// Columns [c0, c1) for a block with one or two live rows (lanes D.tr[0..nr)), without the
+// recurrence: per super-tile a live row's covariances with the 256 columns are the seed
+// product itself (super_seeds with the row's centred window: L.qz, and for a second row
+// the tile buffer, which this path does not use), and lane t takes column c + 64 T + t of
+// tile T.  The values are what a chain start holds, so the pruning threshold and the bounds
+// are the recurrence's (kb of the row's lane covers its scale).  No lane evaluates a dead
+// row: per row and 256 columns one product of K4 / 4 matrix-core steps and about 40 vector
+// instructions, against some 1200 for the recurrence's four tiles.  Returns false once every
+// row is pruned (here or by another work item).
+struct DirectRows {
+    int nr;
+    int tr[2];
+    float qsum[2];
+};
+
+__device__ bool scan_range_direct(const ScanLds &L, RowState &R, const View &V, int64_t s,
+                                  const DirectRows &D, int64_t c0, int64_t c1)
+{
+    const int t = threadIdx.x;
+    const int m = V.m;
+    const int64_t N = V.N;
+    const int W = L.W, K4 = L.K4;
+    const uint64_t word = (uint64_t)(s >> 6);
+    const int64_t lim = c1 < N ? c1 : N;
+    const float *qz[2] = {L.qz, reinterpret_cast<const float *>(L.tile)};
+    float thr0[2], kb[2], mx[2], mxl[2], mxh[2], wbest[2];
+    int wtile[2];
+    bool on[2];
+    uint64_t alive = 0;
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int tu = u < D.nr ? D.tr[u] : D.tr[0];
+        thr0[u] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(R.thr0), tu));
+        kb[u] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(R.kb), tu));
+        mx[u] = mxl[u] = mxh[u] = wbest[u] = -INFINITY;
+        wtile[u] = -1;
+        on[u] = u < D.nr;
+        if (on[u]) alive |= 1ull << tu;
+    }
+    ring_fill(L, V.x, c0, 256 + K4);
+    float nx[4];
+    if (c0 + 256 < c1) {
+#pragma unroll
+        for (int p = 0; p < 4; p++) nx[p] = V.x[c0 + 256 + K4 + p * WAVE + t];
+    }
+    float4 cvs[4];
+#pragma unroll
+    for (int T = 0; T < 4; T++) cvs[T] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int64_t c = c0; c < c1 && alive; c += 256) {
+        const int r0 = (int)(c % W);
+        uint32_t rb = 0;
+        for (int T = 0; T < 4 && c + 64 * T < c1; T++) {
+            const uint32_t b = __float_as_uint(V.rho[(c >> 6) + T]);
+            rb = rb > b ? rb : b;
+            cvs[T] = V.colv[c + 64 * T + t];
+        }
+        uint64_t newdead = 0;
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            if (!on[u]) continue;
+            __syncthreads();  // ring writes and the previous product's readers are done
+            super_seeds(L, r0, qz[u]);
+            __syncthreads();  // seeds visible
+            const int64_t i = s + D.tr[u];
+            const float inc = kb[u] * __uint_as_float(rb);
+            const float thr = thr0[u] + inc;
+            float lmax = -INFINITY;
+            int ltile = -1;
+            bool hit = false;
+#pragma unroll
+            for (int T = 0; T < 4; T++) {
+                const int64_t ct = c + 64 * T;
+                const float refa = L.ring[ring_at(r0 + 64 * T + 16 * (t >> 4))];
+                const float seed = L.seed[64 * T + t] - (cvs[T].w - refa) * D.qsum[u];
+                const int64_t j = ct + t;
+                const bool valid =
+                    ct < c1 && j < lim && ((j - i >= m - 1) || (i - j >= V.ml));
+                const float cj = valid ? seed * cvs[T].z : -INFINITY;
+                hit |= cj >= thr;
+                if (cj > lmax) {
+                    lmax = cj;
+                    ltile = (int)ct;
+                }
+                R.dtiles += ct < c1;
+            }
+            const float mt = wave_max(lmax);
+            mx[u] = fmaxf(mx[u], mt);
+            mxl[u] = fmaxf(mxl[u], mt - inc);
+            mxh[u] = fmaxf(mxh[u], mt + inc);
+            if (mt > wbest[u]) {
+                wbest[u] = mt;
+                const uint64_t at = __builtin_amdgcn_ballot_w64(lmax == mt);
+                wtile[u] = __builtin_amdgcn_readlane(ltile, at ? __builtin_ctzll(at) : 0);
+            }
+            if (__builtin_amdgcn_ballot_w64(hit)) newdead |= 1ull << D.tr[u];
+        }
+        // rows pruned here, then those pruned by other work items
+        int lo = 0, hi = 0;
+        if (t == 0) {
+            const uint64_t a =
+                newdead ? __hip_atomic_fetch_and(V.alive + word, ~newdead, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)
+                        : __hip_atomic_load(V.alive + word, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+            lo = (int)(uint32_t)a;
+            hi = (int)(a >> 32);
+        }
+        const uint64_t now = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(hi, 0) << 32) |
+                             (uint32_t)__builtin_amdgcn_readlane(lo, 0);
+        R.mine |= newdead & alive;
+        alive &= now & ~newdead;
+#pragma unroll
+        for (int u = 0; u < 2; u++) on[u] = on[u] && ((alive >> D.tr[u]) & 1ull);
+        // the next window's new columns, as in scan_range
+        if (alive && c + 256 < c1) {
+            __syncthreads();  // the last product's readers of the ring are done
+            int rn = (int)((c + 256 + K4) % W);
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                int pos = rn + p * WAVE + t;
+                pos = pos >= W ? pos - W : pos;
+                L.ring[ring_at(pos)] = nx[p];
+                L.ring[ring_at(pos + W)] = nx[p];
+            }
+            if (c + 512 < c1) {
+#pragma unroll
+                for (int p = 0; p < 4; p++) nx[p] = V.x[c + 512 + K4 + p * WAVE + t];
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+        if (u < D.nr && t == D.tr[u]) {
+            R.mx = fmaxf(R.mx, mx[u]);
+            R.mxl = fmaxf(R.mxl, mxl[u]);
+            R.mxh = fmaxf(R.mxh, mxh[u]);
+            if (wbest[u] > R.wbest) {
+                R.wbest = wbest[u];
+                R.wtile = wtile[u];
+            }
+        }
+    R.alive = alive;
+    return alive != 0;
+}
+
 // Columns at cyclic offsets [u0, u1) from the block start s: column (s + u) mod N, with the
 // offsets clamped to [U0, N + U0) -- every column exactly once.  Scanning starts at
 // U0 = round_up(m + 64, 64), past the exclusion band right of the block, so the early tiles
 // that kill most blocks take the unmasked path; the band comes last, wrapped.  u0 is a
 // multiple of 64: an unwrapped piece starts aligned, a wrapped one at the aligned-down
 // column (re-scanning at most 63 columns is harmless: same values, same decisions).
-template <bool COLS>
+template <bool COLS, bool DIRECT = false>
 __device__ bool scan_cyclic(const ScanLds &L, RowState &R, const View &V, int64_t s, int64_t u0,
-                            int64_t u1, int64_t U0)
+                            int64_t u1, int64_t U0, const DirectRows *D = nullptr)
 {
     const int64_t N = V.N;
     u0 = u0 > U0 ? u0 : U0;
@@ -789,6 +936,11 @@ __device__ bool scan_cyclic(const ScanLds &L, RowState &R, const View &V, int64_
     }
     const int64_t e = c + (u1 - u0);
     if (wrapped) c &= ~(int64_t)63;
+    if (DIRECT) {
+        bool live = scan_range_direct(L, R, V, s, *D, c, e < N ? e : N);
+        if (live && e > N) live = scan_range_direct(L, R, V, s, *D, 0, e - N);
+        return live;
+    }
     bool live = scan_range<COLS>(L, R, V, s, c, e < N ? e : N);
     if (live && e > N) live = scan_range<COLS>(L, R, V, s, 0, e - N);
     return live;
@@ -812,6 +964,7 @@ __device__ __forceinline__ void load_rows(RowState &R, const View &V, int64_t s,
     R.mxh = -INFINITY;
     R.tiles = 0;
     R.live = 0;
+    R.dtiles = 0;
     R.wbest = -INFINITY;
     R.wtile = -1;
     R.mine = 0;
@@ -870,10 +1023,12 @@ __device__ __forceinline__ ScanLds carve_lds(int m)
 __device__ __forceinline__ void count_pairs(const RowState &R,
                                             unsigned long long *__restrict__ pairs)
 {
-    if (threadIdx.x == 0 && R.tiles) {
+    if (threadIdx.x == 0 && (R.tiles || R.dtiles)) {
         const int slot = blockIdx.x & (PAIR_SLOTS - 1);
-        atomicAdd(pairs + slot, (unsigned long long)R.tiles * 4096ull);
-        atomicAdd(pairs + 2 * PAIR_SLOTS + slot, (unsigned long long)R.live * 64ull);
+        atomicAdd(pairs + slot,
+                  (unsigned long long)R.tiles * 4096ull + (unsigned long long)R.dtiles * 64ull);
+        atomicAdd(pairs + 2 * PAIR_SLOTS + slot,
+                  (unsigned long long)R.live * 64ull + (unsigned long long)R.dtiles * 64ull);
     }
 }
 
@@ -925,6 +1080,8 @@ __device__ __forceinline__ ScanArgs load_args(const ScanPtrs &Q, const PassDev *
     A.U0 = S.U0;
     A.m = S.m;
     A.ml = S.ml;
+    A.dm1 = S.dm1;
+    A.dm2 = S.dm2;
     A.witness_max = S.witness_max;
     A.dk = S.dk;
     return A;
@@ -988,7 +1145,29 @@ __device__ __forceinline__ void refine_item(const ScanArgs &A, int64_t gb, int64
     if (R.alive == 0) return;
     ScanLds L = carve_lds(V.m);
     load_block(L, R, V, s, thr_rho);
-    const bool live = scan_cyclic<false>(L, R, V, s, u0, u1, A.U0);
+    bool live;
+    const int nlive = __builtin_popcountll(R.alive);
+    if ((nlive == 1 && V.m <= A.dm1) || (nlive == 2 && V.m <= A.dm2)) {
+        // one or two live rows: their covariances straight from the seed product
+        // (scan_range_direct); the second row's window goes to the unused tile buffer
+        DirectRows D;
+        D.nr = nlive;
+        const uint64_t a = R.alive;
+        D.tr[0] = __builtin_ctzll(a);
+        D.tr[1] = nlive == 2 ? 63 - __builtin_clzll(a) : D.tr[0];
+        float *q2 = reinterpret_cast<float *>(L.tile);
+        if (nlive == 2) {
+            for (int k = threadIdx.x; k < 16; k += WAVE) q2[k] = 0.f;
+            for (int k = 16 + V.m + threadIdx.x; k < L.K4 + 20; k += WAVE) q2[k] = 0.f;
+        }
+        D.qsum[0] = stage_centred(L.qz + 16, V.x, s + D.tr[0], V.m, V.colv[s + D.tr[0]].w);
+        D.qsum[1] = nlive == 2 ? stage_centred(q2 + 16, V.x, s + D.tr[1], V.m,
+                                               V.colv[s + D.tr[1]].w)
+                               : 0.f;
+        live = scan_cyclic<false, true>(L, R, V, s, u0, u1, A.U0, &D);
+    } else {
+        live = scan_cyclic<false>(L, R, V, s, u0, u1, A.U0);
+    }
     if (live) flush_best(R, V, s);
     record_witness(R, V, s);
     count_pairs(R, A.pairs + PAIR_SLOTS);

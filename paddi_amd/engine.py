@@ -66,9 +66,11 @@ class Tunables(C.Structure):
                 ("min_chunk", C.c_int64), ("refine_items", C.c_int64),
                 ("witness_tiles", C.c_int32), ("speculate", C.c_int32),
                 ("use_graph", C.c_int32), ("prev_shift", C.c_int32),
-                ("bootstrap", C.c_int32), ("bound_candidates", C.c_int32)]
+                ("bootstrap", C.c_int32), ("bound_candidates", C.c_int32),
+                ("direct_max_m", C.c_int32), ("direct2_max_m", C.c_int32)]
     NAMES = ("select_window", "refine_chunk", "min_chunk", "refine_items", "witness_tiles",
-             "speculate", "use_graph", "prev_shift", "bootstrap", "bound_candidates")
+             "speculate", "use_graph", "prev_shift", "bootstrap", "bound_candidates",
+             "direct_max_m", "direct2_max_m")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.NAMES}
