@@ -103,9 +103,9 @@ typedef struct {
     int32_t bootstrap;       /* learned form, first length: r also bounded from below by the
                                 exact NN of the best-scored candidate rows (0/1) */
     int32_t bound_candidates;/* those candidate rows per series (1..256; 0 = the default 32) */
-    int32_t direct_max_m;    /* longest m at which a refine work item whose block has one live
-                                row computes it on the matrix cores alone (0 = never) */
-    int32_t direct2_max_m;   /* the same for blocks with two live rows */
+    int32_t direct_budget;   /* a refine work item whose block has L live rows computes them on
+                                the matrix cores alone when L (m + 15) <= direct_budget (0 = never) */
+    int32_t direct_rows;     /* and L <= direct_rows (0..4) */
 } paddi_tunables;
 
 const char *paddi_status_str(paddi_status s);

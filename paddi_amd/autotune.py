@@ -37,8 +37,8 @@ GRID = {
     "prev_shift": [0, 1, 2],
     "bootstrap": [0, 1],
     "bound_candidates": [16, 32, 64],
-    "direct_max_m": [0, 256, 384, 512, 768],
-    "direct2_max_m": [0, 128, 192, 256, 320],
+    "direct_budget": [0, 400, 550, 700, 850, 1000],
+    "direct_rows": [1, 2, 3, 4],
 }
 
 

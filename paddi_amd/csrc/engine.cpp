@@ -618,8 +618,8 @@ ScanPtrs Engine::fill_pass(Batch &B, int m, const std::vector<double> &r2, bool 
     A.U0 = round_up(m + 64, 64);
     A.m = m;
     A.ml = excl_sym_ ? m - 1 : m + 1;
-    A.dm1 = direct_m1_;
-    A.dm2 = std::min(direct_m2_, DIRECT2_LIMIT_M);
+    A.dbudget = direct_budget_;
+    A.drows = direct_rows_;
     A.witness_max = witness_tiles_;
     A.wsel = witness ? witcur_ : -1;
     A.rsel = (witness && wread_valid_) ? (witcur_ ^ 1) : -1;
@@ -1515,7 +1515,8 @@ paddi_status paddi_get_tunables(paddi_ctx *ctx, paddi_tunables *out)
     const paddi::Engine &E = *ctx->eng;
     *out = paddi_tunables{E.select_window_, E.refine_chunk_, E.min_chunk_, E.refine_waves_,
                           E.witness_tiles_, E.spec_, E.use_graph_ ? 1 : 0, E.prev_shift_,
-                          E.bootstrap_ ? 1 : 0, E.bound_cands_, E.direct_m1_, E.direct_m2_};
+                          E.bootstrap_ ? 1 : 0, E.bound_cands_, E.direct_budget_,
+                          E.direct_rows_};
     return PADDI_OK;
 }
 
@@ -1541,8 +1542,8 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
         t->min_chunk > t->refine_chunk || t->refine_items < 1 || t->witness_tiles < 0 ||
         t->witness_tiles > 64 || t->speculate < 1 || t->speculate > 64 || t->prev_shift < 0 ||
         t->prev_shift > 4 || t->bootstrap < 0 || t->bootstrap > 1 ||
-        t->bound_candidates < 0 || t->bound_candidates > 256 || t->direct_max_m < 0 ||
-        t->direct2_max_m < 0 || t->direct2_max_m > paddi::DIRECT2_LIMIT_M)
+        t->bound_candidates < 0 || t->bound_candidates > 256 || t->direct_budget < 0 ||
+        t->direct_rows < 0 || t->direct_rows > paddi::DIRECT_ROWS)
         return PADDI_ERR_INVALID;
     paddi::Engine &E = *ctx->eng;
     E.select_window_ = (t->select_window + 63) / 64 * 64;
@@ -1555,8 +1556,8 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
     E.prev_shift_ = t->prev_shift;
     E.bootstrap_ = t->bootstrap != 0;
     E.bound_cands_ = t->bound_candidates > 0 ? t->bound_candidates : paddi::BOUND_CANDIDATES;
-    E.direct_m1_ = t->direct_max_m;
-    E.direct_m2_ = t->direct2_max_m;
+    E.direct_budget_ = t->direct_budget;
+    E.direct_rows_ = t->direct_rows;
     return PADDI_OK;
 }
 

@@ -181,8 +181,8 @@ public:
     bool bootstrap_ = true;          // learned first length: candidate lower bound on r
     int bound_cands_ = BOUND_CANDIDATES;  // candidate rows per series behind that bound
     int64_t scap_next_ = SCAP;       // survivors downloaded with the next pass's counters
-    int direct_m1_ = DIRECT_MAX_M;   // refine: one live row on the matrix cores up to this m
-    int direct_m2_ = DIRECT2_MAX_M;  // the same for two live rows
+    int direct_budget_ = DIRECT_BUDGET;  // refine: L live rows on the matrix cores while
+    int direct_rows_ = DIRECT_ROWS;      // L <= direct_rows_ and L (m + 15) <= the budget
     bool excl_sym_ = false;          // exclusion rule: |i - j| >= m - 1 instead of |i - 1 - j| >= m
     bool debug_ = false;             // per-pass trace on stderr (PADDI_DEBUG=1)
     double wait_limit_s_ = 600.0;    // longest wait for queued device work (PADDI_WAIT_LIMIT_S)

@@ -51,7 +51,7 @@ struct ScanArgs {
     int64_t U0;               // first cyclic offset scanned: round_up(m + 64, 64)
     int m;
     int ml;                   // ScanScal::ml
-    int dm1, dm2;             // ScanScal::dm1, dm2
+    int dbudget, drows;       // ScanScal::dbudget, drows
     int witness_max;          // witness ranges scanned first
     float dk;                 // delta(m) / (2m): error bound per unit of block and tile scale
 };
@@ -88,9 +88,9 @@ struct ScanScal {
     // its GPU kernels' |i - j| >= m - 1 (src/PD3_kernels.cuh:70).  The scans' masked tiles cover
     // offsets within m + 63 either side of the diagonal, which holds both bands.
     int ml;
-    // A refine work item whose block has one (two) live rows computes them on the matrix cores
-    // alone at m <= dm1 (dm2); see scan_range_direct.
-    int dm1, dm2;
+    // A refine work item whose block has L <= drows live rows computes them on the matrix
+    // cores alone when L (m + 15) <= dbudget (see scan_range_direct).
+    int dbudget, drows;
     int witness_max;
     int wsel, rsel;           // witness buffer written / read (0, 1; -1 = none)
     int some_off;             // 1 when the pass switches some series off
@@ -118,13 +118,13 @@ struct PassCtl {
     unsigned long long live[2][PAIR_SLOTS];
 };
 constexpr int REFINE_ROUNDS = 3;
-// Default longest m at which k_scan_refine evaluates a block with one live row on the matrix
-// cores alone (scan_range_direct): its product costs K4 / 4 steps per 256 columns, the
-// recurrence's four tiles about 1200 vector instructions whatever m.  Two rows take two
-// products.  The second row's window is staged in the tile buffer: m <= DIRECT2_LIMIT_M.
-constexpr int DIRECT_MAX_M = 768;
-constexpr int DIRECT2_MAX_M = 320;
-constexpr int DIRECT2_LIMIT_M = 400;
+// k_scan_refine evaluates a block with L live rows on the matrix cores alone
+// (scan_range_direct) when L <= DIRECT_ROWS and L (m + 15) <= the budget: each row costs one
+// product of K4 / 4 ~ (m + 15) / 4 matrix-core steps per 256 columns, the recurrence's four
+// tiles about 1200 vector instructions whatever m.  Rows after the first stage their windows
+// in the tile buffer (512 floats), which bounds L at the longer lengths.
+constexpr int DIRECT_ROWS = 4;
+constexpr int DIRECT_BUDGET = 700;
 
 struct ExactArgs {
     const float *x;

@@ -769,19 +769,20 @@ __device__ bool scan_range(const ScanLds &L, RowState &R, const View &V, int64_t
     return true;
 }
 
-// Columns [c0, c1) for a block with one or two live rows (lanes D.tr[0..nr)), without the
-// recurrence: per super-tile a live row's covariances with the 256 columns are the seed
-// product itself (super_seeds with the row's centred window: L.qz, and for a second row
-// the tile buffer, which this path does not use), and lane t takes column c + 64 T + t of
-// tile T.  The values are what a chain start holds, so the pruning threshold and the bounds
+// Columns [c0, c1) for a block with up to DIRECT_ROWS live rows (lanes D.tr[0..nr)), without
+// the recurrence: per super-tile a live row's covariances with the 256 columns are the seed
+// product itself (super_seeds with the row's centred window: L.qz for the first, the tile
+// buffer, which this path does not use, for the others), and lane t takes column
+// c + 64 T + t of tile T.  The values are what a chain start holds, so the pruning threshold and the bounds
 // are the recurrence's (kb of the row's lane covers its scale).  No lane evaluates a dead
 // row: per row and 256 columns one product of K4 / 4 matrix-core steps and about 40 vector
 // instructions, against some 1200 for the recurrence's four tiles.  Returns false once every
 // row is pruned (here or by another work item).
 struct DirectRows {
     int nr;
-    int tr[2];
-    float qsum[2];
+    int tr[DIRECT_ROWS];
+    float qsum[DIRECT_ROWS];
+    const float *qz[DIRECT_ROWS];
 };
 
 __device__ bool scan_range_direct(const ScanLds &L, RowState &R, const View &V, int64_t s,
@@ -793,13 +794,13 @@ __device__ bool scan_range_direct(const ScanLds &L, RowState &R, const View &V, 
     const int W = L.W, K4 = L.K4;
     const uint64_t word = (uint64_t)(s >> 6);
     const int64_t lim = c1 < N ? c1 : N;
-    const float *qz[2] = {L.qz, reinterpret_cast<const float *>(L.tile)};
-    float thr0[2], kb[2], mx[2], mxl[2], mxh[2], wbest[2];
-    int wtile[2];
-    bool on[2];
+    float thr0[DIRECT_ROWS], kb[DIRECT_ROWS], mx[DIRECT_ROWS], mxl[DIRECT_ROWS],
+        mxh[DIRECT_ROWS], wbest[DIRECT_ROWS];
+    int wtile[DIRECT_ROWS];
+    bool on[DIRECT_ROWS];
     uint64_t alive = 0;
 #pragma unroll
-    for (int u = 0; u < 2; u++) {
+    for (int u = 0; u < DIRECT_ROWS; u++) {
         const int tu = u < D.nr ? D.tr[u] : D.tr[0];
         thr0[u] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(R.thr0), tu));
         kb[u] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(R.kb), tu));
@@ -827,10 +828,10 @@ __device__ bool scan_range_direct(const ScanLds &L, RowState &R, const View &V, 
         }
         uint64_t newdead = 0;
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
+        for (int u = 0; u < DIRECT_ROWS; u++) {
             if (!on[u]) continue;
             __syncthreads();  // ring writes and the previous product's readers are done
-            super_seeds(L, r0, qz[u]);
+            super_seeds(L, r0, D.qz[u]);
             __syncthreads();  // seeds visible
             const int64_t i = s + D.tr[u];
             const float inc = kb[u] * __uint_as_float(rb);
@@ -881,7 +882,7 @@ __device__ bool scan_range_direct(const ScanLds &L, RowState &R, const View &V, 
         R.mine |= newdead & alive;
         alive &= now & ~newdead;
 #pragma unroll
-        for (int u = 0; u < 2; u++) on[u] = on[u] && ((alive >> D.tr[u]) & 1ull);
+        for (int u = 0; u < DIRECT_ROWS; u++) on[u] = on[u] && ((alive >> D.tr[u]) & 1ull);
         // the next window's new columns, as in scan_range
         if (alive && c + 256 < c1) {
             __syncthreads();  // the last product's readers of the ring are done
@@ -900,7 +901,7 @@ __device__ bool scan_range_direct(const ScanLds &L, RowState &R, const View &V, 
         }
     }
 #pragma unroll
-    for (int u = 0; u < 2; u++)
+    for (int u = 0; u < DIRECT_ROWS; u++)
         if (u < D.nr && t == D.tr[u]) {
             R.mx = fmaxf(R.mx, mx[u]);
             R.mxl = fmaxf(R.mxl, mxl[u]);
@@ -1080,8 +1081,8 @@ __device__ __forceinline__ ScanArgs load_args(const ScanPtrs &Q, const PassDev *
     A.U0 = S.U0;
     A.m = S.m;
     A.ml = S.ml;
-    A.dm1 = S.dm1;
-    A.dm2 = S.dm2;
+    A.dbudget = S.dbudget;
+    A.drows = S.drows;
     A.witness_max = S.witness_max;
     A.dk = S.dk;
     return A;
@@ -1147,23 +1148,29 @@ __device__ __forceinline__ void refine_item(const ScanArgs &A, int64_t gb, int64
     load_block(L, R, V, s, thr_rho);
     bool live;
     const int nlive = __builtin_popcountll(R.alive);
-    if ((nlive == 1 && V.m <= A.dm1) || (nlive == 2 && V.m <= A.dm2)) {
-        // one or two live rows: their covariances straight from the seed product
-        // (scan_range_direct); the second row's window goes to the unused tile buffer
+    const int qs = (L.K4 + 20 + 3) & ~3;  // floats of one staged row window
+    if (nlive <= A.drows && nlive * (V.m + 15) <= A.dbudget && (nlive - 1) * qs <= 2 * WAVE * 4) {
+        // a few live rows: their covariances straight from the seed product
+        // (scan_range_direct); the rows after the first stage their windows in the tile buffer
         DirectRows D;
         D.nr = nlive;
-        const uint64_t a = R.alive;
-        D.tr[0] = __builtin_ctzll(a);
-        D.tr[1] = nlive == 2 ? 63 - __builtin_clzll(a) : D.tr[0];
-        float *q2 = reinterpret_cast<float *>(L.tile);
-        if (nlive == 2) {
-            for (int k = threadIdx.x; k < 16; k += WAVE) q2[k] = 0.f;
-            for (int k = 16 + V.m + threadIdx.x; k < L.K4 + 20; k += WAVE) q2[k] = 0.f;
+        uint64_t a = R.alive;
+        float *tb = reinterpret_cast<float *>(L.tile);
+#pragma unroll
+        for (int u = 0; u < DIRECT_ROWS; u++) {
+            D.tr[u] = a ? __builtin_ctzll(a) : D.tr[0];
+            a &= a - 1;
+            float *q = u == 0 ? L.qz : tb + (u - 1) * qs;
+            D.qz[u] = q;
+            D.qsum[u] = 0.f;
+            if (u < nlive) {
+                if (u > 0) {
+                    for (int k = threadIdx.x; k < 16; k += WAVE) q[k] = 0.f;
+                    for (int k = 16 + V.m + threadIdx.x; k < L.K4 + 20; k += WAVE) q[k] = 0.f;
+                }
+                D.qsum[u] = stage_centred(q + 16, V.x, s + D.tr[u], V.m, V.colv[s + D.tr[u]].w);
+            }
         }
-        D.qsum[0] = stage_centred(L.qz + 16, V.x, s + D.tr[0], V.m, V.colv[s + D.tr[0]].w);
-        D.qsum[1] = nlive == 2 ? stage_centred(q2 + 16, V.x, s + D.tr[1], V.m,
-                                               V.colv[s + D.tr[1]].w)
-                               : 0.f;
         live = scan_cyclic<false, true>(L, R, V, s, u0, u1, A.U0, &D);
     } else {
         live = scan_cyclic<false>(L, R, V, s, u0, u1, A.U0);

@@ -27,8 +27,8 @@ SCHEDULES = [
     dict(use_graph=1, witness_tiles=0, refine_items=4096),
     dict(prev_shift=0, witness_tiles=48),
     dict(prev_shift=4, speculate=2, bootstrap=0),
-    dict(direct_max_m=0, direct2_max_m=0),
-    dict(direct_max_m=2048, direct2_max_m=400, refine_items=4096),
+    dict(direct_budget=0),
+    dict(direct_budget=1 << 20, direct_rows=4, refine_items=4096),
 ]
 
 
@@ -47,8 +47,8 @@ def test_tunables_roundtrip_and_validation(ctx, restore):
     assert u["select_window"] == 4032 and u["witness_tiles"] == 2   # rounded up to 64
     for bad in (dict(select_window=0), dict(min_chunk=1 << 20, refine_chunk=1 << 16),
                 dict(witness_tiles=-1), dict(witness_tiles=65), dict(speculate=0),
-                dict(prev_shift=5), dict(bootstrap=2), dict(direct_max_m=-1),
-                dict(direct2_max_m=401)):
+                dict(prev_shift=5), dict(bootstrap=2), dict(direct_budget=-1),
+                dict(direct_rows=5)):
         with pytest.raises(PaddiError):
             ctx.set_tunables(**bad)
         assert ctx.get_tunables() == u
