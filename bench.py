@@ -223,17 +223,30 @@ def main():
         print(f"[bench] first length alone: plain {fp[0]:.3f} s, {primary} {fq[0]:.3f} s",
               file=sys.stderr, flush=True)
 
-    # dominant kernel: the one with the most summed time across the timed steps
+    # Per-kernel times come from hipEvents around eager launches.  With the length loop on
+    # the device the timed steps replay graphs, which carry no per-kernel events, so one more
+    # step of the same form runs with the host length loop (the same kernels on the same
+    # lengths) and gives the kernel figures.
+    kstats, kernel_timing = stats, "the timed steps"
+    if primary != "plain" and tunables.get("device_lengths", 0) > 0:
+        ctx.set_tunables(device_lengths=0)
+        _, kt, kstats = run_form(ctx, handles, w, maxL, primary, 1, 0)
+        ctx.set_tunables(device_lengths=tunables["device_lengths"])
+        kernel_timing = (f"one extra step with the host length loop ({1e3 * kt[0]:.1f} ms): "
+                         "graph replays carry no per-kernel events")
+        print(f"[bench] {primary} with the host length loop (kernel times): {kt[0]:.3f} s",
+              file=sys.stderr, flush=True)
+    # dominant kernel: the one with the most summed time across those steps
     names = ("derive", "select", "refine", "exact")
-    tot = {k: sum(s["kernel_ms"][k] for s in stats) for k in names}
-    nl = {k: sum(s["launches"][k] for s in stats) for k in names}
+    tot = {k: sum(s["kernel_ms"][k] for s in kstats) for k in names}
+    nl = {k: sum(s["launches"][k] for s in kstats) for k in names}
     dom = max(names, key=lambda k: tot[k])
     avg_ms = tot[dom] / max(nl[dom], 1)
     kname = {"select": "paddi::k_scan_select", "refine": "paddi::k_scan_refine",
              "derive": "paddi::k_derive", "exact": "paddi::k_exact"}[dom]
     # algorithmic bytes: 16 B per (point x length) for each pass over a length, shared by the
     # kernel's launches of that pass (k_scan_refine runs as three round launches per pass)
-    passes = sum(s["attempts"] for s in stats)
+    passes = sum(s["attempts"] for s in kstats)
     bytes_per_launch = 16.0 * points * passes / max(nl[dom], 1)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{a.workload}.json")
@@ -244,11 +257,11 @@ def main():
             if k.startswith(kname):
                 traffic = v["bytes_per_dispatch"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    pairs = sum(s["pairs"] for s in stats) / len(stats)
+    pairs = sum(s["pairs"] for s in kstats) / len(kstats)
     # the share of each scan's lane-steps spent on rows still alive when their tile started
-    useful = {k: (sum(s["live_pairs"][k] for s in stats) /
-                  max(sum(s["scan_pairs"][k] for s in stats), 1)) for k in ("select", "refine")}
-    scan_ms = (tot["select"] + tot["refine"]) / len(stats)
+    useful = {k: (sum(s["live_pairs"][k] for s in kstats) /
+                  max(sum(s["scan_pairs"][k] for s in kstats), 1)) for k in ("select", "refine")}
+    scan_ms = (tot["select"] + tot["refine"]) / len(kstats)
     valu = (pairs * OPS_PER_PAIR) / (scan_ms * 1e-3) / VALU_PEAK_OPS if scan_ms > 0 else 0.0
 
     out = {
@@ -276,7 +289,8 @@ def main():
         "attempts": stats[-1]["attempts"],
         "survivors": stats[-1]["survivors"],
         "exact_refines": stats[-1]["exact_refines"],
-        "kernel_ms_per_step": {k: tot[k] / len(stats) for k in names},
+        "kernel_ms_per_step": {k: tot[k] / len(kstats) for k in names},
+        "kernel_timing": kernel_timing,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "avg_launch_ms": avg_ms, "launches": nl[dom],
@@ -284,7 +298,7 @@ def main():
         "tunables": tunables,
         "valu": {"pairs_per_step": pairs, "ops_per_pair": OPS_PER_PAIR,
                  "frac_of_peak": valu, "live_row_share": useful,
-                 "scan_pairs_per_step": {k: sum(s["scan_pairs"][k] for s in stats) / len(stats)
+                 "scan_pairs_per_step": {k: sum(s["scan_pairs"][k] for s in kstats) / len(kstats)
                                          for k in ("select", "refine")}},
     }
     if not a.no_cpu:

@@ -106,6 +106,12 @@ typedef struct {
     int32_t direct_budget;   /* a refine work item whose block has L live rows computes them on
                                 the matrix cores alone when L (m + 15) <= direct_budget (0 = never) */
     int32_t direct_rows;     /* and L <= direct_rows (0..4) */
+    int32_t device_lengths;  /* batched and learned forms: run the lengths after the first on the
+                                device, each one captured graph (range from the previous length's
+                                discords, pass, exact top-k), the host synchronising every
+                                device_lengths lengths (0 = every length on the host) */
+    int32_t device_rounds;   /* exact rounds a length's graph holds (1..4); a length needing more
+                                is finished by the host */
 } paddi_tunables;
 
 const char *paddi_status_str(paddi_status s);

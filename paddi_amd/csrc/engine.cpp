@@ -150,23 +150,6 @@ private:
     std::vector<Saved> saved_;
 };
 
-// Rounding of a scan bound's conversion to a squared distance d (the bound's last addition
-// and scaling by invn_i in the scan, then 2m (1 - c) in k_survivors, all in single
-// precision): at most 10m + 2|d| units of 2^-24 (|c| <= 1 contributes the 10m), with a
-// factor 4 of room.
-double conv_ed2(int m, double d) { return 4.0 * (10.0 * m + 2.0 * std::fabs(d)) * 6.0e-8; }
-
-double delta_ed2(int m)
-{
-    // Bound on |single - exact| squared distance where the windows a cell's chain crossed
-    // are of the scale of the cell's own: correlation error of a seed (m-term FP32 dot
-    // product on locally centred data) plus up to 63 recurrence steps, with a wide safety
-    // factor over the largest error measured on the test workloads.  The scans scale it by
-    // the ratio of the largest scale crossed to the cell's (k_pass_init).
-    const double rho_err = 4.0e-5 + 2.0e-7 * m;
-    return 2.0 * m * rho_err;
-}
-
 static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // Bytes of one exact launch block (Engine::exact) for nc candidates: EXACT_SLOTS result keys
@@ -191,12 +174,6 @@ static inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * 
 // candidate in exact order; it is accepted when its distance exceeds r2, and every
 // candidate overlapping it (|i - a| < m) is dropped, since all of them rank below it.
 
-// Exact distances within TIE_REL of each other rank as equal, and then by index.  k_exact's
-// value of a pair is not bitwise symmetric (it centres the candidate's side), while a mutual
-// nearest-neighbour pair shares one distance, which the reference's strict '>' ranks lower
-// index first (sets_processing.hpp:28); the evaluation errors are orders of magnitude below
-// TIE_REL, and TIE_REL far below the 1e-4 parity bar.
-constexpr double TIE_REL = 1e-9;
 static bool ranks_before(double va, int64_t ia, double vb, int64_t ib)
 {
     const double tol = TIE_REL * std::max(std::fabs(va), std::fabs(vb));
@@ -325,7 +302,7 @@ Engine::~Engine()
     for (auto &e : ev_) (void)hipEventDestroy(e);
     (void)hipEventDestroy(ser_ev_);
     (void)hipEventDestroy(sync_ev_);
-    for (void *p : {pin_, pin_par_, pin_out_, pin_ser_, pin_x_})
+    for (void *p : {pin_, pin_par_, pin_out_, pin_ser_, pin_x_, pin_ch_})
         if (p) (void)hipHostFree(p);
     (void)hipStreamDestroy(stream_);
 }
@@ -335,6 +312,9 @@ void Engine::drop_graphs()
     for (auto &g : graphs_)
         if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
     graphs_.clear();
+    for (auto &g : chain_graphs_)
+        if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+    chain_graphs_.clear();
 }
 
 // Pinned staging grows with the batch; a graph that copies from the old buffer is dropped.
@@ -503,7 +483,7 @@ void Engine::derive(Batch &B, int m, paddi_stats *st, int m2)
                        B.x.as<float>(), B.ser.as<SerDev>(), B.bser.as<int>(), B.P.as<double2>(),
                        B.refs.as<double4>(), B.rows, m, colv_.as<float4>(),
                        have ? nullptr : st64_.as<double2>(), tmax_.as<float2>(), m2,
-                       st64n_.as<double2>());
+                       st64n_.as<double2>(), nullptr);
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(ev_[5], stream_));
     derive_st_ = st;  // timed at the next synchronisation (collect_derive)
@@ -534,7 +514,8 @@ void Engine::copy_st64(double2 *dst, int64_t N)
 // LDS class and replayed.  Eager launches are bracketed by events for the statistics;
 // a captured sequence has none (event nodes of a graph do not time the kernels between them).
 void Engine::enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t rows,
-                          int64_t nwords, int64_t scap, bool timed, const uint8_t *keep)
+                          int64_t nwords, int64_t scap, bool timed, const uint8_t *keep,
+                          bool host_io)
 {
     const PassDev *P = par_.as<PassDev>();
     PassCtl *ctl = pout_.as<PassCtl>();
@@ -542,7 +523,7 @@ void Engine::enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t ro
     // k_scan_refine needs more workgroups than listed blocks (see there)
     const unsigned grid =
         (unsigned)std::max<int64_t>(refine_waves_ + refine_waves_ / 2, nwords + 64);
-    HIPC(hipMemcpyAsync(par_.p, pin_par_, parb, hipMemcpyHostToDevice, stream_));
+    if (host_io) HIPC(hipMemcpyAsync(par_.p, pin_par_, parb, hipMemcpyHostToDevice, stream_));
     hipLaunchKernelGGL(k_pass_init, dim3(grid1(std::max<int64_t>(rows, 1024), 256)), dim3(256), 0,
                        stream_, Q, P, rows, nwords, ctl);
     if (keep)
@@ -564,8 +545,9 @@ void Engine::enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t ro
     if (timed) HIPC(hipEventRecord(ev_[3], stream_));
     hipLaunchKernelGGL(k_survivors, dim3(grid1(rows, 256)), dim3(256), 0, stream_, Q, P, rows,
                        ctl, surv, rows);
-    HIPC(hipMemcpyAsync(pin_out_, pout_.p, PASS_OUT_HEAD + (size_t)scap * sizeof(Surv),
-                        hipMemcpyDeviceToHost, stream_));
+    if (host_io)
+        HIPC(hipMemcpyAsync(pin_out_, pout_.p, PASS_OUT_HEAD + (size_t)scap * sizeof(Surv),
+                            hipMemcpyDeviceToHost, stream_));
     HIPC(hipGetLastError());
 }
 
@@ -1193,7 +1175,17 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
     std::vector<std::vector<Cand>> surv;
     std::vector<TopK> tk((size_t)ns);
     wread_valid_ = false;
+    // lengths on the device (from the second, up to the last but one, which has a next
+    // length for its DUAL refinements); twice without progress turns it off for the call
+    bool on_device = device_lengths_ > 0 && ahead && !att_log_ && k <= CHAIN_KMAX;
+    int stalls = 0;
     for (int j = 0; j < nlen; j++) {
+        if (on_device && j > 0 && j + 1 < nlen) {
+            const int js = chain(B, minL, step, k, j, nlen - 1, prev, prev_v, res, st);
+            stalls = js == j ? stalls + 1 : 0;
+            if (stalls >= 2) on_device = false;
+            j = js;
+        }
         const int m = minL + j * step;
         const int m_next = (ahead && j + 1 < nlen) ? m + step : 0;
         att_mark(st);
@@ -1356,6 +1348,267 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
     for (int q = 0; q < ns; q++) out.insert(out.end(), res[(size_t)q].begin(), res[(size_t)q].end());
 }
 
+// ------------------------------------------------------------------------------------------
+// The length loop on the device (ChainArgs in kernels.h).  On entry every series holds the k
+// discords of length j0 - 1 in prev and their exact distances at length j0 in prev_v, and the
+// last derive was derive(m_{j0-1}, m_{j0}) (its second buffer, st64n_, holds m_{j0}'s
+// statistics).  Length j reads the statistics buffer S[(j - j0) & 1] (S0 = st64n_, S1 = st64_
+// at entry) and k_derive writes the next length's into the other, so two graphs per length
+// class cover the loop.  On return the host state is that of the host path about to run the
+// returned length: witness parity, statistics buffers, prev / prev_v.
+int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend,
+                  std::vector<std::vector<int64_t>> &prev, std::vector<std::vector<double>> &prev_v,
+                  std::vector<std::vector<paddi_discord>> &res, paddi_stats *st)
+{
+    const int ns = B.nser;
+    const int m0 = minL + j0 * step;
+    if (jend <= j0 || k < 1 || k > CHAIN_KMAX || derive_m2_ != m0 || step > EXACT_EXT ||
+        !wread_valid_ || st64n_.bytes < (size_t)B.rows * sizeof(double2))
+        return j0;
+    for (int q = 0; q < ns; q++)
+        if ((int)prev[(size_t)q].size() != k || prev_v[(size_t)q].size() != (size_t)k) return j0;
+    const int64_t rows = B.rows, nwords = B.nblocks;
+    const int mlast = minL + (jend - 1) * step;
+    const int cls_max = (int)std::min<int64_t>(round_up(mlast, 64), MAXM);
+    const int nu = std::max(1, std::min({spec_, CHAIN_NU, exact_max_u(cls_max)}));
+    const int rounds = std::max(1, std::min(device_rounds_, CHAIN_ROUNDS));
+    const int64_t scap = std::min<int64_t>(rows, int64_t(1) << 20);
+    const int nl = jend - j0;
+
+    // the pass's buffers (as fill_pass), the loop's own
+    alive_.reserve((size_t)nwords * 8);
+    best_.reserve((size_t)rows * 4);
+    besth_.reserve((size_t)rows * 4);
+    bestl_.reserve((size_t)rows * 4);
+    blist_.reserve((size_t)nwords * 4);
+    pout_.reserve(PASS_OUT_HEAD + (size_t)rows * sizeof(Surv));
+    wit_[0].reserve((size_t)rows * 4);
+    wit_[1].reserve((size_t)rows * 4);
+    par_.reserve(sizeof(PassDev) + (size_t)ns * sizeof(SerDev));
+    colv_.reserve((size_t)rows * sizeof(float4));
+    tmax_.reserve((size_t)nwords * sizeof(float2));
+    rho_.reserve((size_t)nwords * sizeof(float));
+    ch_ctl_.reserve(sizeof(ChainCtl));
+    ch_ser_.reserve((size_t)ns * sizeof(ChainSer));
+    ch_res_.reserve((size_t)nl * ns * sizeof(ChainRes));
+    ch_xv_.reserve((size_t)scap * 8);
+    ch_xv2_.reserve((size_t)scap * 8);
+    ch_cand_.reserve((size_t)ns * CHAIN_NU * 8);
+    ch_cnt_.reserve((size_t)ns * 4);
+    ch_key_.reserve((size_t)ns * CHAIN_NU * EXACT_SLOTS * 8);
+    ch_key2_.reserve((size_t)ns * CHAIN_NU * EXACT_SLOTS * 8);
+    ch_goff_.reserve((size_t)(ns + 1) * 8);
+    // pinned: control block (the graphs' download target) | series state | results / goff
+    const size_t o_ser = 256, o_res = o_ser + round_up((int64_t)ns * sizeof(ChainSer), 256);
+    const size_t pbytes =
+        o_res + std::max((size_t)nl * ns * sizeof(ChainRes), (size_t)(ns + 1) * 8);
+    if (pbytes > pin_ch_bytes_) {
+        if (t_noalloc) refuse_growth("the pinned device-loop staging");
+        HIPC(hipStreamSynchronize(stream_));
+        drop_graphs();
+        if (pin_ch_) HIPC(hipHostFree(pin_ch_));
+        pin_ch_ = nullptr;
+        pin_ch_bytes_ = 0;
+        HIPC(hipHostMalloc(&pin_ch_, pbytes, hipHostMallocDefault));
+        pin_ch_bytes_ = pbytes;
+    }
+    char *hp = (char *)pin_ch_;
+    ChainCtl *hc = (ChainCtl *)hp;
+    ChainSer *hs = (ChainSer *)(hp + o_ser);
+    ChainRes *hr = (ChainRes *)(hp + o_res);
+    int64_t *hg = (int64_t *)(hp + o_res);
+
+    ChainCtl *dctl = ch_ctl_.as<ChainCtl>();
+    ChainArgs C{};
+    C.ctl = dctl;
+    C.cs = ch_ser_.as<ChainSer>();
+    C.res = ch_res_.as<ChainRes>();
+    C.ser = B.ser.as<SerDev>();
+    C.pser = (SerDev *)((char *)par_.p + sizeof(PassDev));
+    C.par = par_.as<PassDev>();
+    C.pctl = pout_.as<PassCtl>();
+    C.surv = (const Surv *)((char *)pout_.p + PASS_OUT_HEAD);
+    C.xv = ch_xv_.as<double>();
+    C.xv2 = ch_xv2_.as<double>();
+    C.cand = ch_cand_.as<int64_t>();
+    C.cnt = ch_cnt_.as<int>();
+    C.outkey = ch_key_.as<unsigned long long>();
+    C.outkey2 = ch_key2_.as<unsigned long long>();
+    C.scap = scap;
+    C.WA = select_window_;
+    C.max_chunk = refine_chunk_;
+    C.min_chunk = min_chunk_;
+    C.items = refine_waves_;
+    C.witness_max = witness_tiles_;
+    C.dbudget = direct_budget_;
+    C.drows = direct_rows_;
+    C.sym = excl_sym_ ? 1 : 0;
+    C.nser = ns;
+    C.k = k;
+    C.nu = nu;
+    C.spec = spec_;
+    C.rounds = rounds;
+    C.minL = minL;
+    C.step = step;
+    ScanPtrs Q{};
+    Q.x = B.x.as<float>();
+    Q.ser = C.pser;
+    Q.bser = B.bser.as<int>();
+    Q.colv = colv_.as<float4>();
+    Q.tmax = tmax_.as<float2>();
+    Q.rho = rho_.as<float>();
+    Q.alive = alive_.as<uint64_t>();
+    Q.best = best_.as<int>();
+    Q.besth = besth_.as<int>();
+    Q.bestl = bestl_.as<int>();
+    Q.wit0 = wit_[0].as<int>();
+    Q.wit1 = wit_[1].as<int>();
+    Q.pairs = &pout_.as<PassCtl>()->pairs[0][0];
+    // k_exact_chain's blocks: series q's column chunks at the loop's first (longest) windows
+    hg[0] = 0;
+    for (int q = 0; q < ns; q++) {
+        const int64_t N = std::max<int64_t>(0, B.n[(size_t)q] - m0 + 1);
+        hg[q + 1] = hg[q] + (N + EXACT_COLS - 1) / EXACT_COLS;
+    }
+    const int64_t xblocks = hg[ns];
+    void *const S[2] = {st64n_.p, st64_.p};
+
+    // graphs: one per (statistics buffer read, length class)
+    auto graph_for = [&](int par, int cls) -> hipGraphExec_t {
+        const size_t lds_scan = scan_lds_bytes(cls), lds_ex = exact_lds_bytes(cls, nu);
+        ExactArgs E{};
+        E.x = B.x.as<float>();
+        E.ser = B.ser.as<SerDev>();
+        E.st64 = (const double2 *)S[par];
+        E.st64n = (const double2 *)S[par ^ 1];
+        E.cand = C.cand;
+        E.goff = ch_goff_.as<int64_t>();
+        E.outkey = C.outkey;
+        E.outkey2 = C.outkey2;
+        E.ngroups = ns;
+        E.sym = C.sym;
+        E.dev_m = &dctl->m;
+        E.dev_count = C.cnt;
+        const int64_t refine_grid = std::max<int64_t>(refine_waves_ + refine_waves_ / 2, nwords + 64);
+        std::string sig;
+        auto put = [&](const void *p, size_t n) { sig.append((const char *)p, n); };
+        put(&C, sizeof C);
+        put(&E, sizeof E);
+        put(&Q, sizeof Q);
+        const int64_t v[] = {rows,     nwords,          (int64_t)lds_scan, (int64_t)lds_ex,
+                             xblocks,  refine_grid,     (int64_t)(uintptr_t)pin_ch_,
+                             (int64_t)(uintptr_t)B.P.p, (int64_t)(uintptr_t)B.refs.p,
+                             (int64_t)(uintptr_t)par_.p};
+        put(v, sizeof v);
+        ChainGraph &G = chain_graphs_[{(uintptr_t)S[par], cls}];
+        if (G.exec && G.sig == sig) return G.exec;
+        if (G.exec) HIPC(hipGraphExecDestroy(G.exec));
+        G.exec = nullptr;
+        hipGraph_t g = nullptr;
+        HIPC(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
+        try {
+            hipLaunchKernelGGL(k_derive, dim3(grid1(rows, 256 * DERIVE_RPT)), dim3(256), 0,
+                               stream_, B.x.as<float>(), B.ser.as<SerDev>(), B.bser.as<int>(),
+                               B.P.as<double2>(), B.refs.as<double4>(), rows, 0, colv_.as<float4>(),
+                               (double2 *)nullptr, tmax_.as<float2>(), 0, (double2 *)S[par ^ 1], &dctl->m);
+            enqueue_pass(Q, lds_scan, 0, rows, nwords, 0, false, nullptr, false);
+            for (int r = 0; r < rounds; r++) {
+                hipLaunchKernelGGL(k_chain_topk, dim3(ns), dim3(64), 0, stream_, C, r);
+                if (xblocks > 0) launch_exact_chain(E, nu, (unsigned)xblocks, lds_ex, stream_);
+            }
+            hipLaunchKernelGGL(k_chain_topk, dim3(ns), dim3(64), 0, stream_, C, rounds);
+            hipLaunchKernelGGL(k_chain_next, dim3(1), dim3(1024), 0, stream_, C);
+            HIPC(hipMemcpyAsync(pin_ch_, dctl, sizeof(ChainCtl), hipMemcpyDeviceToHost, stream_));
+            HIPC(hipGetLastError());
+        } catch (...) {
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPC(hipStreamEndCapture(stream_, &g));
+        const hipError_t e = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPC(e);
+        G.sig = sig;
+        return G.exec;
+    };
+    std::vector<hipGraphExec_t> gl((size_t)nl);
+    for (int j = j0; j < jend; j++) {
+        const int m = minL + j * step;
+        gl[(size_t)(j - j0)] = graph_for((j - j0) & 1, (int)std::min<int64_t>(round_up(m, 64), MAXM));
+    }
+
+    // initial state, then the lengths
+    std::memset(hc, 0, sizeof(ChainCtl));
+    hc->j = j0;
+    hc->j0 = j0;
+    hc->wit0 = witcur_;
+    for (int q = 0; q < ns; q++) {
+        ChainSer &c = hs[q];
+        std::memset(&c, 0, sizeof(ChainSer));
+        c.nprev = k;
+        for (int u = 0; u < k; u++) {
+            c.prev[u] = prev[(size_t)q][(size_t)u];
+            c.pv[u] = prev_v[(size_t)q][(size_t)u];
+        }
+    }
+    HIPC(hipMemcpyAsync(dctl, hc, sizeof(ChainCtl), hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(ch_ser_.p, hs, (size_t)ns * sizeof(ChainSer), hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(ch_goff_.p, hg, (size_t)(ns + 1) * 8, hipMemcpyHostToDevice, stream_));
+    // the first length's start; every later one is started by its predecessor's last kernel
+    hipLaunchKernelGGL(k_chain_begin, dim3(1), dim3(1024), 0, stream_, C);
+    HIPC(hipGetLastError());
+    wait();  // the staging is reused for the results
+    const int every = std::max(1, device_lengths_);
+    for (int j = j0; j < jend; j++) {
+        HIPC(hipGraphLaunch(gl[(size_t)(j - j0)], stream_));
+        if ((j - j0 + 1) % every == 0 && j + 1 < jend) {
+            wait();
+            if (hc->halt) break;
+        }
+    }
+    wait();
+    const ChainCtl cc = *hc;
+    const int jstop = cc.halt ? cc.halt - 1 : jend;
+    if (debug_)
+        std::fprintf(stderr, "[paddi] device lengths %d..%d: finished %d (halt %d)\n", j0,
+                     jend - 1, jstop - j0, cc.halt);
+    if (jstop > j0) {
+        HIPC(hipMemcpyAsync(hr, ch_res_.p, (size_t)(jstop - j0) * ns * sizeof(ChainRes),
+                            hipMemcpyDeviceToHost, stream_));
+        wait();
+        for (int q = 0; q < ns; q++) {
+            for (int j = j0; j < jstop; j++) {
+                const ChainRes &R = hr[(size_t)(j - j0) * ns + q];
+                const int m = minL + j * step;
+                for (int a = 0; a < R.n; a++)
+                    res[(size_t)q].push_back(paddi_discord{
+                        R.i[a], m, q, R.v[a] >= 3.0e38 ? 3.4028235e38f : (float)R.v[a],
+                        (float)R.r2});
+            }
+            const ChainRes &L = hr[(size_t)(jstop - 1 - j0) * ns + q];
+            prev[(size_t)q].assign(L.i, L.i + L.n);
+            prev_v[(size_t)q].assign(L.v2, L.v2 + L.n);
+        }
+    }
+    if (st) {
+        st->attempts += (int64_t)cc.stats[0];
+        st->survivors += (int64_t)cc.stats[1];
+        st->exact_refines += (int64_t)cc.stats[2];
+        st->pairs += (int64_t)(cc.stats[3] + cc.stats[4]);
+        st->scan_pairs[0] += (int64_t)cc.stats[3];
+        st->scan_pairs[1] += (int64_t)cc.stats[4];
+        st->live_pairs[0] += (int64_t)cc.stats[5];
+        st->live_pairs[1] += (int64_t)cc.stats[6];
+    }
+    // the host path resumes at jstop: its witness parity, m_jstop's statistics in st64n_
+    witcur_ = (cc.wit0 + (jstop - j0)) & 1;
+    wread_valid_ = true;
+    if ((jstop - j0) & 1) st64_.swap(st64n_);
+    derive_m2_ = minL + jstop * step;
+    return jstop;
+}
+
 void Engine::run(const std::vector<const Series *> &ss, int minL, int maxL, int step, int k,
                  const paddi_opts &o, std::vector<paddi_discord> &out, paddi_stats *st)
 {
@@ -1516,7 +1769,7 @@ paddi_status paddi_get_tunables(paddi_ctx *ctx, paddi_tunables *out)
     *out = paddi_tunables{E.select_window_, E.refine_chunk_, E.min_chunk_, E.refine_waves_,
                           E.witness_tiles_, E.spec_, E.use_graph_ ? 1 : 0, E.prev_shift_,
                           E.bootstrap_ ? 1 : 0, E.bound_cands_, E.direct_budget_,
-                          E.direct_rows_};
+                          E.direct_rows_, E.device_lengths_, E.device_rounds_};
     return PADDI_OK;
 }
 
@@ -1543,7 +1796,8 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
         t->witness_tiles > 64 || t->speculate < 1 || t->speculate > 64 || t->prev_shift < 0 ||
         t->prev_shift > 4 || t->bootstrap < 0 || t->bootstrap > 1 ||
         t->bound_candidates < 0 || t->bound_candidates > 256 || t->direct_budget < 0 ||
-        t->direct_rows < 0 || t->direct_rows > paddi::DIRECT_ROWS)
+        t->direct_rows < 0 || t->direct_rows > paddi::DIRECT_ROWS || t->device_lengths < 0 ||
+        t->device_rounds < 1 || t->device_rounds > paddi::CHAIN_ROUNDS)
         return PADDI_ERR_INVALID;
     paddi::Engine &E = *ctx->eng;
     E.select_window_ = (t->select_window + 63) / 64 * 64;
@@ -1558,6 +1812,8 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
     E.bound_cands_ = t->bound_candidates > 0 ? t->bound_candidates : paddi::BOUND_CANDIDATES;
     E.direct_budget_ = t->direct_budget;
     E.direct_rows_ = t->direct_rows;
+    E.device_lengths_ = t->device_lengths;
+    E.device_rounds_ = t->device_rounds;
     return PADDI_OK;
 }
 

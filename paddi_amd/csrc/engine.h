@@ -33,8 +33,6 @@ struct HipError : std::runtime_error {
     } while (0)
 
 const char *status_str(paddi_status s);
-double delta_ed2(int m);
-double conv_ed2(int m, double d);
 
 // Survivors returned with a pass's counters (more: a second copy).  The download is sized
 // from the previous pass's count, between SCAP_MIN and SCAP: on MI355X a device-to-pinned
@@ -183,6 +181,9 @@ public:
     int64_t scap_next_ = SCAP;       // survivors downloaded with the next pass's counters
     int direct_budget_ = DIRECT_BUDGET;  // refine: L live rows on the matrix cores while
     int direct_rows_ = DIRECT_ROWS;      // L <= direct_rows_ and L (m + 15) <= the budget
+    int device_lengths_ = 0;         // lengths on the device between host synchronisations
+                                     // (0: the host runs every length)
+    int device_rounds_ = 2;          // exact rounds per length on the device
     bool excl_sym_ = false;          // exclusion rule: |i - j| >= m - 1 instead of |i - 1 - j| >= m
     bool debug_ = false;             // per-pass trace on stderr (PADDI_DEBUG=1)
     double wait_limit_s_ = 600.0;    // longest wait for queued device work (PADDI_WAIT_LIMIT_S)
@@ -199,6 +200,12 @@ private:
                std::vector<paddi_discord> &out, paddi_stats *st);
     void fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opts &o,
               std::vector<paddi_discord> &out, paddi_stats *st);
+    // Lengths j0 .. jend-1 of fast() on the device (ChainArgs); returns the first one the
+    // device did not finish (jend when all), whose discords and next-length values are
+    // then in prev / prev_v, and its results appended to res.
+    int chain(Batch &B, int minL, int step, int k, int j0, int jend,
+              std::vector<std::vector<int64_t>> &prev, std::vector<std::vector<double>> &prev_v,
+              std::vector<std::vector<paddi_discord>> &res, paddi_stats *st);
 public:
     void upload_ser(Batch &B);
     // Learned proposal of the squared range per series at length m (0 = no proposal).
@@ -216,8 +223,11 @@ public:
 private:
     // One pass's device work (parameter upload .. survivor download) on stream_, eagerly or
     // as a replay of the captured graph of its LDS class.
+    // host_io = false: without the parameter upload and the result download (the device
+    // length loop writes the parameters and reads the results on the device)
     void enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t rows,
-                      int64_t nwords, int64_t scap, bool timed, const uint8_t *keep = nullptr);
+                      int64_t nwords, int64_t scap, bool timed, const uint8_t *keep = nullptr,
+                      bool host_io = true);
     void ensure_pinned(size_t par_bytes, size_t ser_bytes);
     ScanPtrs fill_pass(Batch &B, int m, const std::vector<double> &r2, bool witness, int64_t WA,
                        bool refine, size_t &parb);
@@ -229,6 +239,18 @@ private:
         hipGraphExec_t exec = nullptr;
     };
     std::map<std::pair<int, size_t>, GraphEnt> graphs_;
+    // device length loop: one graph per length class and statistics buffer, keyed (the
+    // buffer holding the length's (mu, invn), the class's largest m); its signature is every
+    // argument it bakes in
+    struct ChainGraph {
+        std::string sig;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::map<std::pair<uintptr_t, int>, ChainGraph> chain_graphs_;
+    DevBuf ch_ctl_, ch_ser_, ch_res_, ch_xv_, ch_xv2_, ch_cand_, ch_cnt_, ch_key_, ch_key2_,
+        ch_goff_;
+    void *pin_ch_ = nullptr;       // pinned: the loop's control block, then its downloads
+    size_t pin_ch_bytes_ = 0;
     void drop_graphs();
 
     Model model_;

@@ -17,7 +17,10 @@ constexpr int64_t XPAD = 2048; // zero padding after each series (floats); the s
 // tables, the scans' tile offsets and the pass counters hold rows and columns as int.
 constexpr int64_t MAX_ROWS = int64_t(1) << 31;
 // Row-space rows of a series of n points at shortest length minL (Engine::build_batch).
-inline int64_t series_rows(int64_t n, int minL) { return (n - minL + 1 + 63) / 64 * 64 + 128; }
+__host__ __device__ inline int64_t series_rows(int64_t n, int minL)
+{
+    return (n - minL + 1 + 63) / 64 * 64 + 128;
+}
 
 // One series of a batch, as the kernels see it.  The batch's table is uploaded once (k_derive,
 // k_exact and the feature kernels take N = n - m + 1 from n and m); a pass carries its own
@@ -95,6 +98,8 @@ struct ScanScal {
     int wsel, rsel;           // witness buffer written / read (0, 1; -1 = none)
     int some_off;             // 1 when the pass switches some series off
     float dk;                 // delta(m) / (2m) (ScanArgs::dk)
+    int off;                  // 1: no pass (a stopped device length loop): k_pass_init clears
+                              // the alive words and counters only
 };
 
 // Everything one pass reads that changes between passes: one host-to-device copy (or one
@@ -143,6 +148,11 @@ struct ExactArgs {
     int m;
     int m2;                   // 0: one length
     int sym;                  // exclusion rule: 0 the CPU rule, 1 the symmetric one (ScanScal::ml)
+    // k_exact_chain only (the device-planned launch): the lengths (dev_m[0] = m, dev_m[1] =
+    // m2) and each series' candidate count, read on the device; ngroups is the number of
+    // series there, goff their first blocks, m and m2 above are unused
+    const int *dev_m;
+    const int *dev_count;
 };
 constexpr int EXACT_EXT = 4;  // largest m2 - m
 
@@ -155,7 +165,8 @@ __global__ void k_derive(const float *__restrict__ x, const SerDev *__restrict__
                          const int *__restrict__ bser, const double2 *__restrict__ P,
                          const double4 *__restrict__ refs, int64_t rows, int m,
                          float4 *__restrict__ colv, double2 *__restrict__ st64,
-                         float2 *__restrict__ tmax, int m2, double2 *__restrict__ st64n);
+                         float2 *__restrict__ tmax, int m2, double2 *__restrict__ st64n,
+                         const int *__restrict__ dev_m);
 // The scan kernels: one 64-row block per workgroup (one wave); LDS from scan_lds_bytes(m).
 void launch_scan_select(const ScanPtrs &Q, const PassDev *P, unsigned grid, size_t lds,
                         hipStream_t st);
@@ -187,6 +198,107 @@ __global__ void k_survivors(ScanPtrs Q, const PassDev *__restrict__ P, int64_t r
                             PassCtl *__restrict__ ctl, Surv *__restrict__ out, int64_t cap);
 // k_exact is a template on the largest group size nu of the launch (1..MAXU).
 void launch_exact(const ExactArgs &E, int nu, unsigned grid, size_t lds, hipStream_t st);
+// k_exact_chain (the device-planned launch of the length loop) for groups of nu <= CHAIN_NU.
+void launch_exact_chain(const ExactArgs &E, int nu, unsigned grid, size_t lds, hipStream_t st);
+
+// Squared-distance bounds shared by the host and the device.
+// delta_ed2: bound on |single - exact| squared distance where the windows a cell's chain
+// crossed are of the scale of the cell's own: correlation error of a seed (m-term FP32 dot
+// product on locally centred data) plus up to 63 recurrence steps, with a wide safety factor
+// over the largest error measured on the test workloads.  The scans scale it by the ratio of
+// the largest scale crossed to the cell's (k_pass_init).
+__host__ __device__ inline double delta_ed2(int m) { return 2.0 * m * (4.0e-5 + 2.0e-7 * m); }
+// conv_ed2: rounding of a scan bound's conversion to a squared distance d (the bound's last
+// addition and scaling by invn_i in the scan, then 2m (1 - c) in k_survivors, all in single
+// precision): at most 10m + 2|d| units of 2^-24 (|c| <= 1 contributes the 10m), with a factor
+// 4 of room.
+__host__ __device__ inline double conv_ed2(int m, double d)
+{
+    return 4.0 * (10.0 * m + 2.0 * (d < 0.0 ? -d : d)) * 6.0e-8;
+}
+// Exact distances within TIE_REL of each other rank as equal, and then by index.  k_exact's
+// value of a pair is not bitwise symmetric (it centres the candidate's side), while a mutual
+// nearest-neighbour pair shares one distance, which the reference's strict '>' ranks lower
+// index first (sets_processing.hpp:28); the evaluation errors are orders of magnitude below
+// TIE_REL, and TIE_REL far below the 1e-4 parity bar.
+constexpr double TIE_REL = 1e-9;
+
+// ---------------------------------------------------------------------------------------
+// The length loop on the device (Engine::device_lengths).  Every length after the first runs
+// as one captured graph: k_derive, the pass (k_pass_init .. k_survivors), then `rounds` x
+// (k_chain_topk, k_exact_chain), a final k_chain_topk, which records the length's discords
+// and their distances at the next length, and k_chain_next, which starts the next length (its
+// range from those distances, the pass parameters, the lengths k_derive and k_exact_chain
+// read; k_chain_begin starts the first).  The host only replays graphs.  A length the device cannot finish (a pass
+// with fewer than k discords above its range, or more exact rounds needed than the graph
+// holds, or more survivors in one series than one workgroup takes) stops the loop: `halt` is
+// set, every later length's kernels find their series switched off, and the host runs that
+// length itself.
+constexpr int CHAIN_KMAX = 8;      // largest top-k of the device loop
+constexpr int CHAIN_NU = 4;        // largest exact group per series and round
+constexpr int CHAIN_ROUNDS = 4;    // largest number of exact rounds per length
+constexpr int CHAIN_SCAP = 4096;   // survivors one series may bring to the device top-k
+
+// Per series: the current length's state, and the discords of the length before the loop's
+// first (later lengths take them from the results of the length before).
+struct ChainSer {
+    int64_t prev[CHAIN_KMAX];  // discords of the length before the first (local rows)
+    double pv[CHAIN_KMAX];     // their exact distances at the first length (DBL_MAX: none)
+    double r2;                 // the current length's squared range
+    int nprev;
+    int status;                // current length: 0 searching, 1 done, 2 fewer than k above r2,
+                               // 3 unfinished (rounds or survivors exhausted)
+    int nreq;                  // exact candidates asked for by the last round
+    int pad;
+    int req_p[CHAIN_NU];       // their survivor indices
+};
+// One series at one length, as the host reads it back.
+struct ChainRes {
+    int64_t i[CHAIN_KMAX];     // discords (local rows), best first
+    double v[CHAIN_KMAX];      // their exact squared NN distances
+    double v2[CHAIN_KMAX];     // and at the next length (DBL_MAX: none), its range's inputs
+    double r2;                 // range of the pass that found them
+    int n;                     // discords (0..k); written once the series finished the length
+    int pad;
+};
+// The loop's scalars.
+struct ChainCtl {
+    int j;            // length running (index from the call's first length)
+    int j0;           // the loop's first length
+    int wit0;         // witness buffer written at length j0
+    int halt;         // 0, or 1 + the index of the first length the device did not finish
+    int m, m2;        // k_derive's and k_exact_chain's lengths (0 once halted)
+    // statistics of the finished lengths: attempts, survivors, exact refinements, pair
+    // evaluations of select / refine, and of those on live rows
+    unsigned long long stats[8];
+};
+// Arguments of the chain kernels (fixed while a batch runs, so graphs can hold them).
+struct ChainArgs {
+    ChainCtl *ctl;
+    ChainSer *cs;
+    ChainRes *res;             // [length index - ChainCtl::j0][series]
+    const SerDev *ser;         // the batch's table (n and offsets)
+    SerDev *pser;              // the pass's table (N, thr per series)
+    PassDev *par;              // the pass's scalars
+    const PassCtl *pctl;       // the pass's counters
+    const Surv *surv;          // the pass's survivors
+    double *xv, *xv2;          // per survivor: exact distance at m and at m2 (NaN: not refined)
+    int64_t *cand;             // [series][nu] exact candidates of a round (k_exact_chain)
+    int *cnt;                  // [series] their number
+    unsigned long long *outkey, *outkey2;  // [series][nu][EXACT_SLOTS] (k_exact_chain)
+    int64_t scap;              // survivors xv holds
+    int64_t WA, max_chunk, min_chunk, items;   // pass scalars that stay fixed (Engine::fill_pass)
+    int witness_max, dbudget, drows, sym;
+    int nser, k, nu, spec, rounds;
+    int minL, step;            // length j is minL + j step
+};
+// Start of the loop's first length: one workgroup.
+__global__ void k_chain_begin(ChainArgs C);
+// Exact top-k of one series per workgroup of one wave; round = 0 .. rounds (the last only
+// finishes).
+__global__ void k_chain_topk(ChainArgs C, int round);
+// Statistics of the length just run and the start of the next: one workgroup.
+__global__ void k_chain_next(ChainArgs C);
 
 // Scan seeds: the MFMA reduction length (m + 15 rounded up to 4) and the series ring
 // capacity (a multiple of 16 holding a 256-column super-tile window of 256 + K4 columns).

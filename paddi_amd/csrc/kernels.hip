@@ -262,8 +262,15 @@ __global__ __launch_bounds__(256) void k_derive(const float *__restrict__ x,
                                                 int m, float4 *__restrict__ colv,
                                                 double2 *__restrict__ st64,
                                                 float2 *__restrict__ tmax, int m2,
-                                                double2 *__restrict__ st64n)
+                                                double2 *__restrict__ st64n,
+                                                const int *__restrict__ dev_m)
 {
+    // the device length loop gives the lengths on the device (m = 0: the loop has stopped)
+    if (dev_m) {
+        m = dev_m[0];
+        m2 = dev_m[1];
+        if (m <= 0) return;
+    }
     const double invm = 1.0 / m, invm2 = m2 ? 1.0 / m2 : 0.0;
 #pragma unroll
     for (int q = 0; q < DERIVE_RPT; q++) {
@@ -1240,6 +1247,11 @@ __global__ __launch_bounds__(256) void k_pass_init(ScanPtrs Q, const PassDev *__
     const ScanArgs A = load_args(Q, P);
     uint64_t *alive = A.alive;
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (P->S.off) {  // no pass: nothing alive, counters cleared, the rest untouched
+        if (g < nwords) alive[g] = 0;
+        if (g < (int64_t)(sizeof(PassCtl) / sizeof(int))) reinterpret_cast<int *>(ctl)[g] = 0;
+        return;
+    }
     if (g < rows) {
         A.best[g] = (int)0x807fffff;  // ord_key(-inf)
         A.besth[g] = (int)0x807fffff;
@@ -1464,46 +1476,53 @@ __device__ __forceinline__ void exact_quad3(const double *xt, int k0, double a0,
     if (B == 12) XQ3(12, 13, 14, 15);
 }
 
-// DUAL: also the distances at the second length E.m2 (a separate instantiation: its
+// Scalars of one k_exact work item, per candidate of its group.
+struct ExactShared {
+    double ainv[MAXU], ainv2[MAXU], aext[MAXU][EXACT_EXT];
+    double asa[MAXU], asb[MAXU];   // sum of a_k over k < m, and over k < m2
+    double apart[MAXU][4];         // per-wave partial sums of a_k
+    int64_t cidx[MAXU];
+    unsigned long long wmin[MAXU][4];
+};
+
+// One work item: the candidates [u0, u0 + nU) of one series (sid) against its columns
+// [chunk EXACT_COLS, (chunk + 1) EXACT_COLS); results into candidate slot `slot` mod
+// EXACT_SLOTS.  m and (DUAL) m2 = the lengths.
+// DUAL: also the distances at the second length m2 (a separate instantiation: its
 // epilogue needs more registers, which would cost the one-length launches occupancy).
 template <int NU, bool DUAL>
-__global__ __launch_bounds__(256) void k_exact(ExactArgs E)
+__device__ __forceinline__ void exact_item(const ExactArgs &E, const int m, const int m2_arg,
+                                           const int sid, const int u0, const int nU,
+                                           const int64_t chunk, const int64_t slot, double *dsh,
+                                           ExactShared &X)
 {
     constexpr int QC = exact_qc(NU);
     constexpr int SUBC = 256 * QC;   // columns per sub-tile
     constexpr int WR = QC + 4;       // window ring
     constexpr int UR = WR / 4;       // k steps of 4 per turn of the ring
-    extern __shared__ double dsh[];
-    const int m = E.m;
     const int m4 = exact_mk(m, NU);                     // a multiple of 4 (of 16 for NU = 3)
     double *xs = dsh;                                   // exact_xs_doubles(m, NU)
     double *A = dsh + exact_xs_doubles(m, NU);          // [NU][m4] candidate windows
-    __shared__ double ainv[MAXU], ainv2[MAXU], aext[MAXU][EXACT_EXT];
-    __shared__ double asa[MAXU], asb[MAXU];   // sum of a_k over k < m, and over k < m2
-    __shared__ double apart[MAXU][4];         // per-wave partial sums of a_k
-    __shared__ int64_t cidx[MAXU];
-    __shared__ unsigned long long wmin[MAXU][4];
+    double(&ainv)[MAXU] = X.ainv;
+    double(&ainv2)[MAXU] = X.ainv2;
+    double(&aext)[MAXU][EXACT_EXT] = X.aext;
+    double(&asa)[MAXU] = X.asa;
+    double(&asb)[MAXU] = X.asb;
+    double(&apart)[MAXU][4] = X.apart;
+    int64_t(&cidx)[MAXU] = X.cidx;
+    unsigned long long(&wmin)[MAXU][4] = X.wmin;
     const int t = threadIdx.x;
-    int lo = 0, hi = E.ngroups - 1;
-    const int64_t item = blockIdx.x;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (E.goff[mid] <= item) lo = mid; else hi = mid - 1;
-    }
-    const int grp = lo;
-    const int u0 = E.gfirst[grp];
-    const int nU = E.gfirst[grp + 1] - u0;
-    const SerDev S = E.ser[E.gser[grp]];
+    const SerDev S = E.ser[sid];
     const float *x = E.x + S.xoff;
     const double2 *st64 = E.st64 + S.roff;
     const int64_t N = S.n - m + 1;
-    const int m2 = DUAL ? E.m2 : 0;
+    const int m2 = DUAL ? m2_arg : 0;
     const int ext = DUAL ? m2 - m : 0;
     const int64_t N2 = DUAL ? S.n - m2 + 1 : 0;
     // pairs iff j - i >= m - 1 or i - j >= ml (ExactArgs::sym)
     const int ml = E.sym ? m - 1 : m + 1, ml2 = E.sym ? m2 - 1 : m2 + 1;
     const double2 *st64n = E.st64n + S.roff;
-    const int64_t cblk = (item - E.goff[grp]) * EXACT_COLS;
+    const int64_t cblk = chunk * EXACT_COLS;
     const double cref = (double)x[cblk];   // the columns' offset (cblk < N)
     for (int u = 0; u < NU; u++) {
         const int64_t i = E.cand[u0 + (u < nU ? u : 0)];
@@ -1647,7 +1666,7 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
     if (t < nU) {
         unsigned long long k = wmin[t][0];
         for (int q = 1; q < 4; q++) k = wmin[t][q] < k ? wmin[t][q] : k;
-        atomicMin(E.outkey + (size_t)(u0 + t) * EXACT_SLOTS + (blockIdx.x & (EXACT_SLOTS - 1)), k);
+        atomicMin(E.outkey + (size_t)(u0 + t) * EXACT_SLOTS + (slot & (EXACT_SLOTS - 1)), k);
     }
     if (DUAL) {
         __syncthreads();  // wmin is read above
@@ -1665,9 +1684,476 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
         if (t < nU) {
             unsigned long long k = wmin[t][0];
             for (int q = 1; q < 4; q++) k = wmin[t][q] < k ? wmin[t][q] : k;
-            atomicMin(E.outkey2 + (size_t)(u0 + t) * EXACT_SLOTS + (blockIdx.x & (EXACT_SLOTS - 1)),
+            atomicMin(E.outkey2 + (size_t)(u0 + t) * EXACT_SLOTS + (slot & (EXACT_SLOTS - 1)),
                       k);
         }
+    }
+}
+
+// Host-planned launch: block = (group, chunk of that group's series), the group found by
+// binary search in goff.
+template <int NU, bool DUAL>
+__global__ __launch_bounds__(256) void k_exact(ExactArgs E)
+{
+    extern __shared__ double dsh[];
+    __shared__ ExactShared X;
+    const int64_t item = blockIdx.x;
+    int lo = 0, hi = E.ngroups - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (E.goff[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const int u0 = E.gfirst[lo];
+    exact_item<NU, DUAL>(E, E.m, E.m2, E.gser[lo], u0, E.gfirst[lo + 1] - u0, item - E.goff[lo],
+                         item, dsh, X);
+}
+
+// Device-planned launch of the length loop on the device: series s may ask for up to NU
+// candidates a round (k_chain_topk writes them at cand[s NU ..] and their number at
+// dev_count[s]); block = (series, chunk of its columns at the loop's shortest length), found by
+// binary search in goff (fixed per batch), so blocks of a series without candidates, and
+// chunks past its windows at the current length, end at once.  The lengths come from the
+// device too (dev_m[0] = m, dev_m[1] = m2).
+template <int NU>
+__global__ __launch_bounds__(256) void k_exact_chain(ExactArgs E)
+{
+    extern __shared__ double dsh[];
+    __shared__ ExactShared X;
+    const int64_t item = blockIdx.x;
+    int lo = 0, hi = E.ngroups - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (E.goff[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const int nU = E.dev_count[lo];
+    const int m = E.dev_m[0], m2 = E.dev_m[1];
+    if (nU <= 0 || m <= 0) return;
+    const int64_t chunk = item - E.goff[lo];
+    if (chunk * EXACT_COLS >= E.ser[lo].n - m + 1) return;
+    exact_item<NU, true>(E, m, m2, lo, lo * NU, nU, chunk, item, dsh, X);
+}
+
+// ---------------------------------------------------------------------------------------
+// The length loop on the device (see ChainArgs).
+
+__device__ __forceinline__ double ord_val64(unsigned long long k)
+{
+    const unsigned long long bits = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)bits);
+}
+
+// Start of length j: per series its squared range from the previous length's discords
+// evaluated exactly at this length, as Engine::fast takes it (the minimum over the discords
+// still inside the series, times 1 - 1e-6; every discord must have its value), the pass's
+// series table (N, thr) and scalars (Engine::fill_pass), and the lengths k_derive and
+// k_exact_chain read.  A series without k such values stops the loop at this length.  Run by
+// one workgroup: k_chain_begin for the loop's first length, k_chain_next for the others.
+__device__ __forceinline__ void chain_begin(const ChainArgs &C, const int j, int *s_bad,
+                            unsigned long long *s_nmax)
+{
+    ChainCtl *ctl = C.ctl;
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int m = C.minL + j * C.step;
+    const bool halted0 = ctl->halt != 0;
+    if (t == 0) {
+        *s_bad = 0;
+        *s_nmax = 0;
+    }
+    __syncthreads();
+    for (int q = t; q < C.nser && !halted0; q += nt) {
+        ChainSer &cs = C.cs[q];
+        const int64_t Nq = C.ser[q].n - m + 1;
+        // the previous length's discords: the loop's initial state, or that length's results
+        // (finished by every series, or the loop would have stopped)
+        const ChainRes *R = j > ctl->j0 ? &C.res[(size_t)(j - 1 - ctl->j0) * C.nser + q] : nullptr;
+        const int64_t *prev = R ? R->i : cs.prev;
+        const double *pv = R ? R->v2 : cs.pv;
+        int np = R ? R->n : cs.nprev;
+        np = np < 0 ? 0 : (np > CHAIN_KMAX ? CHAIN_KMAX : np);
+        bool ok = np >= C.k;
+        double mn = INFINITY;
+        int cnt = 0;
+        for (int u = 0; u < np; u++) {
+            ok &= pv[u] >= 0.0;
+            if (prev[u] >= Nq) continue;
+            cnt++;
+            if (pv[u] < 3.0e38) mn = fmin(mn, pv[u]);
+        }
+        ok &= cnt >= C.k && mn < INFINITY && mn > 0.0;
+        cs.r2 = ok ? mn * (1.0 - 1e-6) : 0.0;
+        cs.status = 0;
+        cs.nreq = 0;
+        if (!ok) atomicOr(s_bad, 1);
+        if (Nq > 0) atomicMax(s_nmax, (unsigned long long)Nq);
+    }
+    __syncthreads();
+    const bool halted = halted0 || *s_bad != 0;
+    for (int q = t; q < C.nser; q += nt) {
+        SerDev S = C.ser[q];
+        S.N = halted ? 0 : S.n - m + 1;
+        S.thr = (float)(1.0 - C.cs[q].r2 / (2.0 * m));
+        C.pser[q] = S;
+    }
+    if (t == 0) {
+        if (!halted0 && *s_bad) ctl->halt = j + 1;
+        ScanScal &A = C.par->S;
+        A.WA = C.WA;
+        A.U0 = (int64_t)(m + 64 + 63) / 64 * 64;
+        A.m = m;
+        A.ml = C.sym ? m - 1 : m + 1;
+        A.dbudget = C.dbudget;
+        A.drows = C.drows;
+        A.witness_max = C.witness_max;
+        const int w = (ctl->wit0 + (j - ctl->j0)) & 1;
+        A.wsel = w;
+        A.rsel = w ^ 1;
+        A.some_off = halted ? 1 : 0;
+        A.dk = (float)(delta_ed2(m) / (2.0 * m));
+        A.off = halted ? 1 : 0;
+        RefineArgs &R = C.par->R;
+        const int64_t end = (halted ? 0 : (int64_t)*s_nmax) + A.U0;
+        const int64_t b0 = A.U0 + A.WA, b1 = b0 + 2 * C.max_chunk, b2 = b0 + 8 * C.max_chunk;
+        R.bound[0] = b0 < end ? b0 : end;
+        R.bound[1] = b1 < end ? b1 : end;
+        R.bound[2] = b2 < end ? b2 : end;
+        R.bound[3] = end;
+        R.min_chunk = C.min_chunk;
+        R.max_chunk = C.max_chunk;
+        R.items = C.items;
+        ctl->j = j;
+        ctl->m = halted ? 0 : m;
+        ctl->m2 = halted ? 0 : m + C.step;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_chain_begin(ChainArgs C)
+{
+    __shared__ int s_bad;
+    __shared__ unsigned long long s_nmax;
+    chain_begin(C, C.ctl->j, &s_bad, &s_nmax);
+}
+
+// A candidate of the device top-k: its upper bound hi, local row i and position p in the
+// workgroup's list; the larger hi ranks first, ties by lower row (TopK::advance's T).
+struct TkKey {
+    double h;
+    int i;
+    int p;  // -1: none
+};
+__device__ __forceinline__ bool tk_better(const TkKey &a, const TkKey &b)
+{
+    if (a.p < 0) return false;
+    if (b.p < 0) return true;
+    return a.h > b.h || (a.h == b.h && a.i < b.i);
+}
+// Best key over the wave (k_chain_topk is one wave); every lane gets it.
+__device__ __forceinline__ TkKey tk_reduce(TkKey k)
+{
+    for (int o = 32; o >= 1; o >>= 1) {
+        TkKey b;
+        b.h = __shfl_xor(k.h, o, 64);
+        b.i = __shfl_xor(k.i, o, 64);
+        b.p = __shfl_xor(k.p, o, 64);
+        if (tk_better(b, k)) k = b;
+    }
+    return k;
+}
+__device__ __forceinline__ int wave_sum_i(int v)
+{
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Exact greedy top-k of series q (one wave per series) over its survivors, as
+// TopK::advance: every survivor holds [lo, hi] (its single-precision bounds widened by
+// conv_ed2, or its exact value once refined); T = argmax hi; the exact maximum lies in
+// U = {hi >= lo_T (less the tie tolerance)}; once U is exact its best (ranks_before: TIE_REL,
+// then lower row) is accepted if above r2 and its trivial matches dropped.  The greedy is
+// replayed from the exact values in every round (it depends on nothing else), so a round only
+// adds values: round 0 marks the survivors unrefined, rounds > 0 first fold the previous
+// round's k_exact_chain results.  A round that needs values asks for up to nu of U's unrefined
+// members (highest bound first), filled up to min(spec, nu) with the candidates most likely to
+// be accepted next (descending bound, apart by m from T and from each other), as
+// TopK::advance.  Round `rounds` only finishes: a series that still needs values is left
+// unfinished (the host runs the length).
+__global__ __launch_bounds__(64) void k_chain_topk(ChainArgs C, int round)
+{
+    // the series' survivors, staged once per round: survivor index, local row, bounds (or the
+    // exact value in both), the exact value (NaN: not refined), dropped flag
+    __shared__ int s_p[CHAIN_SCAP];
+    __shared__ int s_row[CHAIN_SCAP];  // local rows (a call holds fewer than 2^31 rows)
+    __shared__ double s_lo[CHAIN_SCAP], s_hi[CHAIN_SCAP], s_v[CHAIN_SCAP];
+    __shared__ unsigned char s_gone[CHAIN_SCAP];
+    __shared__ int s_n, s_sel[CHAIN_NU], s_anchor[CHAIN_NU + 1], s_acc[CHAIN_KMAX];
+    const int q = blockIdx.x, t = threadIdx.x;
+    ChainCtl *ctl = C.ctl;
+    ChainSer &cs = C.cs[q];
+    const int j = ctl->j;
+    const int m = C.minL + j * C.step;
+    const int nu = C.nu;
+    // lane 0's reading for the whole wave: another series may stop the loop meanwhile
+    const int run = __shfl(t == 0 ? (ctl->m > 0 && ctl->halt == 0 && cs.status == 0) : 0, 0, 64);
+    int nreq = 0;
+    if (run) {
+        const double r2 = cs.r2;
+        const SerDev S = C.ser[q];
+        const int64_t roff = S.roff, rend = roff + series_rows(S.n, C.minL);
+        const int64_t nsv = C.pctl->survivors;
+        // fold the previous round's results: lane t reads slot t of every candidate (the
+        // loads issued together), then the minima over the wave
+        const int pr = round > 0 ? cs.nreq : 0;
+        unsigned long long ka[CHAIN_NU], kb[CHAIN_NU];
+#pragma unroll
+        for (int u = 0; u < CHAIN_NU; u++) {
+            const size_t o = ((size_t)q * nu + u) * EXACT_SLOTS + t;
+            ka[u] = u < pr ? C.outkey[o] : ~0ull;
+            kb[u] = u < pr ? C.outkey2[o] : ~0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < CHAIN_NU; u++) {
+            for (int w = 32; w >= 1; w >>= 1) {
+                const unsigned long long a2 = __shfl_xor(ka[u], w, 64);
+                const unsigned long long b2 = __shfl_xor(kb[u], w, 64);
+                ka[u] = a2 < ka[u] ? a2 : ka[u];
+                kb[u] = b2 < kb[u] ? b2 : kb[u];
+            }
+            if (t == 0 && u < pr) {
+                C.xv[cs.req_p[u]] = ord_val64(ka[u]);
+                C.xv2[cs.req_p[u]] = ord_val64(kb[u]);
+            }
+        }
+        if (t == 0) s_n = 0;
+        __syncthreads();
+        const int64_t ns = nsv < C.scap ? nsv : C.scap;
+        for (int64_t p = t; p < ns; p += 64) {
+            const Surv sv = C.surv[p];
+            if (sv.g < roff || sv.g >= rend) continue;
+            const int e = atomicAdd(&s_n, 1);
+            if (e >= CHAIN_SCAP) continue;
+            double v = __longlong_as_double(0x7ff8000000000000ll);  // NaN: not refined
+            if (round == 0)
+                C.xv[p] = v;
+            else
+                v = C.xv[p];
+            s_p[e] = (int)p;
+            s_row[e] = (int)(sv.g - roff);
+            s_v[e] = v;
+            s_gone[e] = 0;
+            if (!isnan(v)) {
+                s_lo[e] = v;
+                s_hi[e] = v;
+            } else if (sv.d2 >= 3.0e38f) {
+                s_lo[e] = INFINITY;
+                s_hi[e] = INFINITY;
+            } else {
+                s_lo[e] = (double)sv.d2lo - conv_ed2(m, sv.d2lo);
+                s_hi[e] = (double)sv.d2hi + conv_ed2(m, sv.d2hi);
+            }
+        }
+        __syncthreads();
+        const int n = s_n < CHAIN_SCAP ? s_n : CHAIN_SCAP;
+        // status: 0 searching, -1 no discord above r2 left, 3 unfinished
+        int status = (nsv > C.scap || s_n > CHAIN_SCAP) ? 3 : 0;
+        int nacc = 0;
+        while (status == 0 && nacc < C.k) {
+            // T = argmax hi over the candidates left
+            TkKey best{-INFINITY, 0, -1};
+            for (int e = t; e < n; e += 64)
+                if (!s_gone[e]) {
+                    const TkKey c{s_hi[e], s_row[e], e};
+                    if (tk_better(c, best)) best = c;
+                }
+            const TkKey T = tk_reduce(best);
+            if (T.p < 0) break;
+            const double loT = s_lo[T.p];
+            const double loTt = loT - 2.0 * TIE_REL * fabs(loT);
+            // U and its unrefined members
+            int cu = 0, cn = 0;
+            for (int e = t; e < n; e += 64)
+                if (!s_gone[e] && s_hi[e] >= loTt) {
+                    cu++;
+                    cn += isnan(s_v[e]) ? 1 : 0;
+                }
+            cu = wave_sum_i(cu);
+            cn = wave_sum_i(cn);
+            const bool tonly = cu == 0;  // only when distances are not ordered (NaN)
+            const int nneed = tonly ? (isnan(s_v[T.p]) ? 1 : 0) : cn;
+            if (nneed > 0) {
+                if (round == C.rounds) {
+                    status = 3;
+                    break;
+                }
+                // the requests: unrefined members of U by descending bound (T first), then
+                // speculative ones (apart by m from T and from each other), up to
+                // min(spec, nu) in all
+                int nanchor = 1;
+                if (t == 0) s_anchor[0] = T.i;
+                const int cap = C.spec < nu ? C.spec : nu;
+                const int take = nneed < nu ? nneed : nu;
+                const int want = take < cap ? cap : take;
+                __syncthreads();
+                for (int r = 0; r < want; r++) {
+                    const bool spec = r >= take;
+                    if (spec && nneed >= C.spec) break;  // TopK::advance speculates below spec only
+                    TkKey b{-INFINITY, 0, -1};
+                    for (int e = t; e < n; e += 64) {
+                        if (s_gone[e] || !isnan(s_v[e])) continue;
+                        bool used = false;
+                        for (int u = 0; u < r; u++) used |= s_sel[u] == e;
+                        if (used) continue;
+                        if (!spec) {
+                            if (tonly ? e != T.p : !(s_hi[e] >= loTt)) continue;
+                        } else {
+                            bool apart = true;
+                            for (int a2 = 0; a2 < nanchor; a2++) {
+                                const int d = s_row[e] - s_anchor[a2];
+                                apart &= (d < 0 ? -d : d) >= m;
+                            }
+                            if (!apart) continue;
+                        }
+                        const TkKey c{s_hi[e], s_row[e], e};
+                        if (tk_better(c, b)) b = c;
+                    }
+                    const TkKey w = tk_reduce(b);
+                    if (w.p < 0) break;
+                    if (t == 0) {
+                        s_sel[r] = w.p;
+                        if (spec) s_anchor[nanchor] = w.i;
+                    }
+                    if (spec) nanchor++;
+                    nreq = r + 1;
+                    __syncthreads();
+                }
+                break;
+            }
+            // U is exact: accept its best if above r2 (ranks_before), drop its trivial matches.
+            // Lane-wise best under ranks_before, then the lanes' bests in lane order: the
+            // tolerance makes the order matter only between values within TIE_REL.
+            int B = -1;
+            double vb = 0.0;
+            int ib = 0;
+            for (int e = t; e < n; e += 64) {
+                if (s_gone[e]) continue;
+                if (tonly ? e != T.p : !(s_hi[e] >= loTt)) continue;
+                const double v = s_v[e];
+                const int i = s_row[e];
+                bool before = B < 0;
+                if (!before) {
+                    const double tol = TIE_REL * fmax(fabs(v), fabs(vb));
+                    before = v > vb + tol ? true : (vb > v + tol ? false : i < ib);
+                }
+                if (before) {
+                    B = e;
+                    vb = v;
+                    ib = i;
+                }
+            }
+            for (int l = 1; l < 64; l++) {
+                const int Bl = __shfl(B, l, 64);
+                const double vl = __shfl(vb, l, 64);
+                const int il = __shfl(ib, l, 64);
+                if (t != 0 || Bl < 0) continue;
+                bool before = B < 0;
+                if (!before) {
+                    const double tol = TIE_REL * fmax(fabs(vl), fabs(vb));
+                    before = vl > vb + tol ? true : (vb > vl + tol ? false : il < ib);
+                }
+                if (before) {
+                    B = Bl;
+                    vb = vl;
+                    ib = il;
+                }
+            }
+            B = __shfl(B, 0, 64);
+            vb = __shfl(vb, 0, 64);
+            ib = __shfl(ib, 0, 64);
+            if (!(vb > r2)) {
+                status = -1;  // no more discords above r2
+                break;
+            }
+            if (t == 0) s_acc[nacc] = B;
+            nacc++;
+            for (int e = t; e < n; e += 64) {
+                const int d = s_row[e] - ib;
+                if ((d < 0 ? -d : d) < m) s_gone[e] = 1;
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+        // waiting for values (nreq > 0), finished (k discords, or fewer with none above r2
+        // left), short of k (2) or unfinished (3)
+        int st = status == 3 ? 3 : (nreq > 0 ? 0 : ((nacc >= C.k || r2 <= 0.0) ? 1 : 2));
+        if (t == 0) {
+            if (st == 1) {
+                ChainRes &R = C.res[(size_t)(j - ctl->j0) * C.nser + q];
+                R.n = nacc;
+                R.r2 = r2;
+                for (int a = 0; a < nacc; a++) {
+                    const int e = s_acc[a];
+                    R.i[a] = s_row[e];
+                    R.v[a] = s_v[e];
+                    R.v2[a] = C.xv2[s_p[e]];
+                }
+            }
+            if (st >= 2) atomicCAS(&ctl->halt, 0, j + 1);
+            cs.status = st;
+            cs.nreq = st == 0 ? nreq : 0;
+            for (int u = 0; u < nreq; u++) {
+                cs.req_p[u] = s_p[s_sel[u]];
+                C.cand[(size_t)q * nu + u] = s_row[s_sel[u]];
+            }
+            for (int u = nreq; u < nu && nreq > 0; u++)  // spare rows of the group: copies
+                C.cand[(size_t)q * nu + u] = s_row[s_sel[0]];
+            if (nreq > 0) atomicAdd(&ctl->stats[2], (unsigned long long)nreq);
+        }
+        if (st != 0) nreq = 0;
+        for (int e = t; e < nreq * EXACT_SLOTS; e += 64) {
+            C.outkey[(size_t)q * nu * EXACT_SLOTS + e] = ~0ull;
+            C.outkey2[(size_t)q * nu * EXACT_SLOTS + e] = ~0ull;
+        }
+    }
+    if (t == 0) C.cnt[q] = nreq;
+}
+
+// Close of a length and start of the next (one workgroup, after the length's final
+// k_chain_topk): the pass's statistics, when it ran, then chain_begin.
+__global__ __launch_bounds__(1024) void k_chain_next(ChainArgs C)
+{
+    __shared__ int s_bad;
+    __shared__ unsigned long long s_nmax;
+    ChainCtl *ctl = C.ctl;
+    const int t = threadIdx.x;
+    const int j = ctl->j;
+    if (ctl->m > 0 && t < 64) {
+        const PassCtl *P = C.pctl;
+        unsigned long long a0 = P->pairs[0][t], a1 = P->pairs[1][t], a2 = P->live[0][t],
+                           a3 = P->live[1][t];
+        for (int w = 32; w >= 1; w >>= 1) {
+            a0 += __shfl_xor(a0, w, 64);
+            a1 += __shfl_xor(a1, w, 64);
+            a2 += __shfl_xor(a2, w, 64);
+            a3 += __shfl_xor(a3, w, 64);
+        }
+        if (t == 0) {
+            ctl->stats[0] += 1;
+            ctl->stats[1] += (unsigned long long)P->survivors;
+            ctl->stats[3] += a0;
+            ctl->stats[4] += a1;
+            ctl->stats[5] += a2;
+            ctl->stats[6] += a3;
+        }
+    }
+    __syncthreads();  // every thread has read ctl->j and ctl->m
+    chain_begin(C, j + 1, &s_bad, &s_nmax);
+}
+
+void launch_exact_chain(const ExactArgs &E, int nu, unsigned grid, size_t lds, hipStream_t st)
+{
+    switch (nu) {
+    case 1: hipLaunchKernelGGL((k_exact_chain<1>), dim3(grid), dim3(256), lds, st, E); break;
+    case 2: hipLaunchKernelGGL((k_exact_chain<2>), dim3(grid), dim3(256), lds, st, E); break;
+    case 3: hipLaunchKernelGGL((k_exact_chain<3>), dim3(grid), dim3(256), lds, st, E); break;
+    default: hipLaunchKernelGGL((k_exact_chain<4>), dim3(grid), dim3(256), lds, st, E); break;
     }
 }
 

@@ -67,10 +67,11 @@ class Tunables(C.Structure):
                 ("witness_tiles", C.c_int32), ("speculate", C.c_int32),
                 ("use_graph", C.c_int32), ("prev_shift", C.c_int32),
                 ("bootstrap", C.c_int32), ("bound_candidates", C.c_int32),
-                ("direct_budget", C.c_int32), ("direct_rows", C.c_int32)]
+                ("direct_budget", C.c_int32), ("direct_rows", C.c_int32),
+                ("device_lengths", C.c_int32), ("device_rounds", C.c_int32)]
     NAMES = ("select_window", "refine_chunk", "min_chunk", "refine_items", "witness_tiles",
              "speculate", "use_graph", "prev_shift", "bootstrap", "bound_candidates",
-             "direct_budget", "direct_rows")
+             "direct_budget", "direct_rows", "device_lengths", "device_rounds")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.NAMES}

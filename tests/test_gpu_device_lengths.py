@@ -1,0 +1,112 @@
+"""The length loop on the device (paddi_tunables.device_lengths).
+
+Every length after the first runs as one captured graph: its range from the previous length's
+discords, the pass, the exact top-k (k_chain_topk, k_exact_chain), and the next length's range
+inputs; the host only replays graphs and synchronises every `device_lengths` lengths.  The loop
+must return what the host loop (src/main.cu:136-233's length loop as Engine::fast runs it)
+returns, bit for bit: the same kernels compute every exact value, the greedy top-k and the range
+rule are the same.  A length the device cannot finish (more exact rounds than its graph holds,
+or a pass with fewer than k discords above its range) is finished by the host, without changing
+the answer.  The oracle checks a small case end to end.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from paddi_amd import synth
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-4
+
+
+def uneven(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(count):
+        n = int(rng.integers(6_000, 40_000))
+        if q % 2:
+            out.append(synth.ecg_like(n, seed + q, 1, 0))
+        else:
+            out.append(synth.rw_planted(n, seed + q, 1, 48, 8.0))
+    return out
+
+
+CASES = {
+    # name: (series, minL, maxL, step, k)
+    "rw_k3": (lambda: [synth.rw_planted(150_000, 5, 3, 96, 12.0)], 64, 100, 1, 3),
+    "ecg_k2_step2": (lambda: [synth.ecg_like(120_000, 6, 2, 2)], 60, 96, 2, 2),
+    "ar2_k3_step4": (lambda: [synth.ar2_mix(100_000, 7)], 128, 192, 4, 3),
+    "uneven_k1": (lambda: uneven(40, 11), 40, 60, 1, 1),
+    "uneven_k2_step3": (lambda: uneven(12, 12), 48, 84, 3, 2),
+}
+_HOST = {}
+
+
+def key(recs):
+    return [(r.index, r.length, r.series, r.nn_dist2, r.range2) for r in recs]
+
+
+@pytest.fixture
+def restore(ctx):
+    saved = ctx.get_tunables()
+    yield
+    ctx.set_tunables(**saved)
+
+
+def run(ctx, name, mode, **tun):
+    series, minL, maxL, step, k = CASES[name]
+    ctx.set_tunables(prev_shift=0, **tun)
+    return ctx.discords_batch(series(), minL, maxL, step, k, mode=mode, with_stats=True)
+
+
+def host(ctx, name, mode):
+    if (name, mode) not in _HOST:
+        _HOST[(name, mode)] = run(ctx, name, mode, device_lengths=0)
+    return _HOST[(name, mode)]
+
+
+@pytest.mark.parametrize("mode", ["batched", "learned"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_device_loop_equals_host_loop(ctx, restore, name, mode):
+    ref, st0 = host(ctx, name, mode)
+    print(f"device loop {name} {mode}: {len(ref)} discords", flush=True)
+    for every in (1, 5, 64):
+        got, st = run(ctx, name, mode, device_lengths=every, device_rounds=2)
+        assert key(got) == key(ref), (name, mode, every)
+        # the lengths the device ran were no host passes: fewer eager scan launches
+        assert st["launches"]["select"] < st0["launches"]["select"], (st, st0)
+    # twice in a row: the graphs captured by the first call are replayed by the second
+    again, _ = run(ctx, name, mode, device_lengths=5, device_rounds=2)
+    assert key(again) == key(ref)
+
+
+@pytest.mark.parametrize("name", ["rw_k3", "uneven_k2_step3"])
+def test_lengths_the_device_cannot_finish_go_back_to_the_host(ctx, restore, name):
+    """One exact round per length and one candidate per round: every length whose top-k needs
+    a second round stops the device loop there; the host finishes it and restarts the loop."""
+    ref, _ = host(ctx, name, "batched")
+    got, st = run(ctx, name, "batched", device_lengths=3, device_rounds=1, speculate=1)
+    assert key(got) == key(ref)
+    got4, _ = run(ctx, name, "batched", device_lengths=3, device_rounds=4, speculate=4)
+    assert key(got4) == key(ref)
+
+
+def test_device_loop_matches_the_oracle(ctx, restore):
+    x = synth.rw_planted(8000, 9, 2, 64, 16.0)
+    minL, maxL, k = 40, 52, 2
+    ref = O.palmad(x, minL, maxL, 1, k)
+    ctx.set_tunables(prev_shift=0, device_lengths=4, device_rounds=2)
+    got = ctx.discords(x, minL, maxL, 1, k, mode="batched")
+    exp = [(int(i), r["m"], float(d)) for r in ref for i, d in zip(r["idx"], r["d2"])]
+    assert [(g.index, g.length) for g in got] == [(i, m) for i, m, _ in exp]
+    for g, (_, _, d) in zip(got, exp):
+        assert abs(g.nn_dist2 - d) <= RTOL * d, (g, d)
+
+
+def test_device_tunables_validation(ctx, restore):
+    from paddi_amd.engine import PaddiError
+    t = ctx.get_tunables()
+    for bad in (dict(device_lengths=-1), dict(device_rounds=0), dict(device_rounds=5)):
+        with pytest.raises(PaddiError):
+            ctx.set_tunables(**bad)
+        assert ctx.get_tunables() == t
