@@ -58,7 +58,10 @@ enum {
                                 an r taken from the previous length's discords (exact values at
                                 the new length), retried lower on failure; a pass's kernels run
                                 without a host round trip and may replay as a hipGraph
-                                (paddi_tunables.use_graph) */
+                                (paddi_tunables.use_graph); with paddi_tunables.device_lengths
+                                the lengths after the first run on the device, one captured
+                                graph per length (its range, pass and exact top-k), the host
+                                synchronising every device_lengths lengths */
     PADDI_MODE_LEARNED = 2   /* batched, with the first length's r from the learned proposal;
                                 the descent remains the fallback */
 };

@@ -1177,11 +1177,11 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
     wread_valid_ = false;
     // lengths on the device (from the second, up to the last but one, which has a next
     // length for its DUAL refinements); twice without progress turns it off for the call
-    bool on_device = device_lengths_ > 0 && ahead && !att_log_ && k <= CHAIN_KMAX;
+    bool on_device = device_lengths_ > 0 && !att_log_ && k <= CHAIN_KMAX;
     int stalls = 0;
     for (int j = 0; j < nlen; j++) {
         if (on_device && j > 0 && j + 1 < nlen) {
-            const int js = chain(B, minL, step, k, j, nlen - 1, prev, prev_v, res, st);
+            const int js = chain(B, minL, step, k, j, nlen - 1, ahead, prev, prev_v, res, st);
             stalls = js == j ? stalls + 1 : 0;
             if (stalls >= 2) on_device = false;
             j = js;
@@ -1356,22 +1356,32 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
 // at entry) and k_derive writes the next length's into the other, so two graphs per length
 // class cover the loop.  On return the host state is that of the host path about to run the
 // returned length: witness parity, statistics buffers, prev / prev_v.
-int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend,
+int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ahead,
                   std::vector<std::vector<int64_t>> &prev, std::vector<std::vector<double>> &prev_v,
                   std::vector<std::vector<paddi_discord>> &res, paddi_stats *st)
 {
     const int ns = B.nser;
     const int m0 = minL + j0 * step;
-    if (jend <= j0 || k < 1 || k > CHAIN_KMAX || derive_m2_ != m0 || step > EXACT_EXT ||
-        !wread_valid_ || st64n_.bytes < (size_t)B.rows * sizeof(double2))
+    const int shift = ahead ? 0 : prev_shift_;
+    if (jend <= j0 || k < 1 || k > CHAIN_KMAX || !wread_valid_ ||
+        (ahead && (derive_m2_ != m0 || step > EXACT_EXT ||
+                   st64n_.bytes < (size_t)B.rows * sizeof(double2))))
         return j0;
     for (int q = 0; q < ns; q++)
-        if ((int)prev[(size_t)q].size() != k || prev_v[(size_t)q].size() != (size_t)k) return j0;
+        if ((int)prev[(size_t)q].size() != k ||
+            (ahead && prev_v[(size_t)q].size() != (size_t)k))
+            return j0;
     const int64_t rows = B.rows, nwords = B.nblocks;
     const int mlast = minL + (jend - 1) * step;
     const int cls_max = (int)std::min<int64_t>(round_up(mlast, 64), MAXM);
-    const int nu = std::max(1, std::min({spec_, CHAIN_NU, exact_max_u(cls_max)}));
+    // candidates per round: the speculation width, at least two where they fit (one candidate
+    // at two lengths runs one wave per SIMD, see launch_exact)
+    const int umax = std::min(CHAIN_NU, exact_max_u(cls_max));
+    const int nu = std::max(std::min(2, umax), std::min(spec_, umax));
     const int rounds = std::max(1, std::min(device_rounds_, CHAIN_ROUNDS));
+    // range launch: gr group slots of nur rows per series, the rows split evenly
+    const int gr = (k * (shift + 1) + nu - 1) / nu;
+    const int nur = (k * (shift + 1) + gr - 1) / gr;
     const int64_t scap = std::min<int64_t>(rows, int64_t(1) << 20);
     const int nl = jend - j0;
 
@@ -1398,6 +1408,10 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend,
     ch_key_.reserve((size_t)ns * CHAIN_NU * EXACT_SLOTS * 8);
     ch_key2_.reserve((size_t)ns * CHAIN_NU * EXACT_SLOTS * 8);
     ch_goff_.reserve((size_t)(ns + 1) * 8);
+    ch_candr_.reserve((size_t)ns * gr * nur * 8);
+    ch_cntr_.reserve((size_t)ns * gr * 4);
+    ch_keyr_.reserve((size_t)ns * gr * nur * EXACT_SLOTS * 8);
+    st64_.reserve((size_t)rows * sizeof(double2));
     // pinned: control block (the graphs' download target) | series state | results / goff
     const size_t o_ser = 256, o_res = o_ser + round_up((int64_t)ns * sizeof(ChainSer), 256);
     const size_t pbytes =
@@ -1450,6 +1464,12 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend,
     C.rounds = rounds;
     C.minL = minL;
     C.step = step;
+    C.shift = shift;
+    C.gr = gr;
+    C.nur = nur;
+    C.candr = ch_candr_.as<int64_t>();
+    C.cntr = ch_cntr_.as<int>();
+    C.keyr = ch_keyr_.as<unsigned long long>();
     ScanPtrs Q{};
     Q.x = B.x.as<float>();
     Q.ser = C.pser;
@@ -1471,7 +1491,10 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend,
         hg[q + 1] = hg[q] + (N + EXACT_COLS - 1) / EXACT_COLS;
     }
     const int64_t xblocks = hg[ns];
-    void *const S[2] = {st64n_.p, st64_.p};
+    // statistics buffers: with the range from the previous exact rounds, length j reads
+    // S[(j - j0) & 1] and k_derive writes the next length's into the other; with a shifted
+    // range every length's k_derive writes st64_
+    void *const S[2] = {ahead ? st64n_.p : st64_.p, ahead ? st64_.p : st64_.p};
 
     // graphs: one per (statistics buffer read, length class)
     auto graph_for = [&](int par, int cls) -> hipGraphExec_t {
@@ -1480,7 +1503,7 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend,
         E.x = B.x.as<float>();
         E.ser = B.ser.as<SerDev>();
         E.st64 = (const double2 *)S[par];
-        E.st64n = (const double2 *)S[par ^ 1];
+        E.st64n = (const double2 *)(ahead ? S[par ^ 1] : st64n_.p);
         E.cand = C.cand;
         E.goff = ch_goff_.as<int64_t>();
         E.outkey = C.outkey;
@@ -1489,11 +1512,19 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend,
         E.sym = C.sym;
         E.dev_m = &dctl->m;
         E.dev_count = C.cnt;
+        E.gslots = 1;
+        ExactArgs Er = E;  // the shifted range's launch
+        Er.cand = C.candr;
+        Er.outkey = C.keyr;
+        Er.outkey2 = C.keyr;
+        Er.dev_count = C.cntr;
+        Er.gslots = gr;
         const int64_t refine_grid = std::max<int64_t>(refine_waves_ + refine_waves_ / 2, nwords + 64);
         std::string sig;
         auto put = [&](const void *p, size_t n) { sig.append((const char *)p, n); };
         put(&C, sizeof C);
         put(&E, sizeof E);
+        put(&Er, sizeof Er);
         put(&Q, sizeof Q);
         const int64_t v[] = {rows,     nwords,          (int64_t)lds_scan, (int64_t)lds_ex,
                              xblocks,  refine_grid,     (int64_t)(uintptr_t)pin_ch_,
@@ -1507,14 +1538,24 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend,
         hipGraph_t g = nullptr;
         HIPC(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
         try {
+            if (!ahead) hipLaunchKernelGGL(k_chain_rreq, dim3(ns), dim3(64), 0, stream_, C);
             hipLaunchKernelGGL(k_derive, dim3(grid1(rows, 256 * DERIVE_RPT)), dim3(256), 0,
                                stream_, B.x.as<float>(), B.ser.as<SerDev>(), B.bser.as<int>(),
                                B.P.as<double2>(), B.refs.as<double4>(), rows, 0, colv_.as<float4>(),
-                               (double2 *)nullptr, tmax_.as<float2>(), 0, (double2 *)S[par ^ 1], &dctl->m);
+                               ahead ? (double2 *)nullptr : (double2 *)S[par],
+                               tmax_.as<float2>(), 0,
+                               ahead ? (double2 *)S[par ^ 1] : st64n_.as<double2>(), &dctl->m);
+            if (!ahead) {
+                if (xblocks > 0)
+                    launch_exact_chain(Er, nur, false, (unsigned)(xblocks * gr),
+                                       exact_lds_bytes(cls, nur), stream_);
+                hipLaunchKernelGGL(k_chain_range, dim3(ns), dim3(64), 0, stream_, C);
+            }
             enqueue_pass(Q, lds_scan, 0, rows, nwords, 0, false, nullptr, false);
             for (int r = 0; r < rounds; r++) {
                 hipLaunchKernelGGL(k_chain_topk, dim3(ns), dim3(64), 0, stream_, C, r);
-                if (xblocks > 0) launch_exact_chain(E, nu, (unsigned)xblocks, lds_ex, stream_);
+                if (xblocks > 0)
+                    launch_exact_chain(E, nu, ahead, (unsigned)xblocks, lds_ex, stream_);
             }
             hipLaunchKernelGGL(k_chain_topk, dim3(ns), dim3(64), 0, stream_, C, rounds);
             hipLaunchKernelGGL(k_chain_next, dim3(1), dim3(1024), 0, stream_, C);
@@ -1535,12 +1576,13 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend,
     std::vector<hipGraphExec_t> gl((size_t)nl);
     for (int j = j0; j < jend; j++) {
         const int m = minL + j * step;
-        gl[(size_t)(j - j0)] = graph_for((j - j0) & 1, (int)std::min<int64_t>(round_up(m, 64), MAXM));
+        gl[(size_t)(j - j0)] = graph_for(ahead ? (j - j0) & 1 : 0,
+                                         (int)std::min<int64_t>(round_up(m, 64), MAXM));
     }
 
     // initial state, then the lengths
     std::memset(hc, 0, sizeof(ChainCtl));
-    hc->j = j0;
+    hc->j = j0 - 1;  // k_chain_next below starts length j0 (m = 0: no length to close)
     hc->j0 = j0;
     hc->wit0 = witcur_;
     for (int q = 0; q < ns; q++) {
@@ -1549,14 +1591,14 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend,
         c.nprev = k;
         for (int u = 0; u < k; u++) {
             c.prev[u] = prev[(size_t)q][(size_t)u];
-            c.pv[u] = prev_v[(size_t)q][(size_t)u];
+            c.pv[u] = ahead ? prev_v[(size_t)q][(size_t)u] : 0.0;
         }
     }
     HIPC(hipMemcpyAsync(dctl, hc, sizeof(ChainCtl), hipMemcpyHostToDevice, stream_));
     HIPC(hipMemcpyAsync(ch_ser_.p, hs, (size_t)ns * sizeof(ChainSer), hipMemcpyHostToDevice, stream_));
     HIPC(hipMemcpyAsync(ch_goff_.p, hg, (size_t)(ns + 1) * 8, hipMemcpyHostToDevice, stream_));
     // the first length's start; every later one is started by its predecessor's last kernel
-    hipLaunchKernelGGL(k_chain_begin, dim3(1), dim3(1024), 0, stream_, C);
+    hipLaunchKernelGGL(k_chain_next, dim3(1), dim3(1024), 0, stream_, C);
     HIPC(hipGetLastError());
     wait();  // the staging is reused for the results
     const int every = std::max(1, device_lengths_);
@@ -1588,7 +1630,10 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend,
             }
             const ChainRes &L = hr[(size_t)(jstop - 1 - j0) * ns + q];
             prev[(size_t)q].assign(L.i, L.i + L.n);
-            prev_v[(size_t)q].assign(L.v2, L.v2 + L.n);
+            if (ahead)
+                prev_v[(size_t)q].assign(L.v2, L.v2 + L.n);
+            else
+                prev_v[(size_t)q].clear();
         }
     }
     if (st) {
@@ -1604,8 +1649,12 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend,
     // the host path resumes at jstop: its witness parity, m_jstop's statistics in st64n_
     witcur_ = (cc.wit0 + (jstop - j0)) & 1;
     wread_valid_ = true;
-    if ((jstop - j0) & 1) st64_.swap(st64n_);
-    derive_m2_ = minL + jstop * step;
+    if (ahead) {
+        if ((jstop - j0) & 1) st64_.swap(st64n_);
+        derive_m2_ = minL + jstop * step;
+    } else {
+        derive_m2_ = 0;  // st64_ holds the last length's statistics; the host derives again
+    }
     return jstop;
 }
 

@@ -203,7 +203,9 @@ private:
     // Lengths j0 .. jend-1 of fast() on the device (ChainArgs); returns the first one the
     // device did not finish (jend when all), whose discords and next-length values are
     // then in prev / prev_v, and its results appended to res.
-    int chain(Batch &B, int minL, int step, int k, int j0, int jend,
+    // ahead: the range from the previous exact rounds' second length (prev_v), else shifted
+    // by prev_shift_ with a launch of its own per length.
+    int chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ahead,
               std::vector<std::vector<int64_t>> &prev, std::vector<std::vector<double>> &prev_v,
               std::vector<std::vector<paddi_discord>> &res, paddi_stats *st);
 public:
@@ -248,7 +250,7 @@ private:
     };
     std::map<std::pair<uintptr_t, int>, ChainGraph> chain_graphs_;
     DevBuf ch_ctl_, ch_ser_, ch_res_, ch_xv_, ch_xv2_, ch_cand_, ch_cnt_, ch_key_, ch_key2_,
-        ch_goff_;
+        ch_goff_, ch_candr_, ch_cntr_, ch_keyr_;
     void *pin_ch_ = nullptr;       // pinned: the loop's control block, then its downloads
     size_t pin_ch_bytes_ = 0;
     void drop_graphs();

@@ -149,10 +149,12 @@ struct ExactArgs {
     int m2;                   // 0: one length
     int sym;                  // exclusion rule: 0 the CPU rule, 1 the symmetric one (ScanScal::ml)
     // k_exact_chain only (the device-planned launch): the lengths (dev_m[0] = m, dev_m[1] =
-    // m2) and each series' candidate count, read on the device; ngroups is the number of
-    // series there, goff their first blocks, m and m2 above are unused
+    // m2) and the candidate count of each of a series' gslots groups, read on the device;
+    // ngroups is the number of series there, goff their first blocks, m and m2 above are
+    // unused
     const int *dev_m;
     const int *dev_count;
+    int gslots;
 };
 constexpr int EXACT_EXT = 4;  // largest m2 - m
 
@@ -198,8 +200,10 @@ __global__ void k_survivors(ScanPtrs Q, const PassDev *__restrict__ P, int64_t r
                             PassCtl *__restrict__ ctl, Surv *__restrict__ out, int64_t cap);
 // k_exact is a template on the largest group size nu of the launch (1..MAXU).
 void launch_exact(const ExactArgs &E, int nu, unsigned grid, size_t lds, hipStream_t st);
-// k_exact_chain (the device-planned launch of the length loop) for groups of nu <= CHAIN_NU.
-void launch_exact_chain(const ExactArgs &E, int nu, unsigned grid, size_t lds, hipStream_t st);
+// k_exact_chain (the device-planned launch of the length loop) for groups of nu <= CHAIN_NU;
+// dual: also at the second length.
+void launch_exact_chain(const ExactArgs &E, int nu, bool dual, unsigned grid, size_t lds,
+                        hipStream_t st);
 
 // Squared-distance bounds shared by the host and the device.
 // delta_ed2: bound on |single - exact| squared distance where the windows a cell's chain
@@ -229,11 +233,12 @@ constexpr double TIE_REL = 1e-9;
 // (k_chain_topk, k_exact_chain), a final k_chain_topk, which records the length's discords
 // and their distances at the next length, and k_chain_next, which starts the next length (its
 // range from those distances, the pass parameters, the lengths k_derive and k_exact_chain
-// read; k_chain_begin starts the first).  The host only replays graphs.  A length the device cannot finish (a pass
-// with fewer than k discords above its range, or more exact rounds needed than the graph
-// holds, or more survivors in one series than one workgroup takes) stops the loop: `halt` is
-// set, every later length's kernels find their series switched off, and the host runs that
-// length itself.
+// read).  With a shifted range (ChainArgs::shift) the graph starts with the range's own exact
+// launch instead: k_chain_rreq, k_derive, k_exact_chain, k_chain_range.  The host only
+// replays graphs.  A length the device cannot finish (a pass with fewer than k discords above
+// its range, or more exact rounds needed than the graph holds, or more survivors in one series
+// than one workgroup takes) stops the loop: `halt` is set, every later length's kernels find
+// their series switched off, and the host runs that length itself.
 constexpr int CHAIN_KMAX = 8;      // largest top-k of the device loop
 constexpr int CHAIN_NU = 4;        // largest exact group per series and round
 constexpr int CHAIN_ROUNDS = 4;    // largest number of exact rounds per length
@@ -249,8 +254,9 @@ struct ChainSer {
     int status;                // current length: 0 searching, 1 done, 2 fewer than k above r2,
                                // 3 unfinished (rounds or survivors exhausted)
     int nreq;                  // exact candidates asked for by the last round
-    int pad;
+    int nrq;                   // shifted range: rows evaluated for the range (k_chain_rreq)
     int req_p[CHAIN_NU];       // their survivor indices
+    unsigned char rq_u[CHAIN_KMAX * (EXACT_EXT + 1)];  // the discord each range row shifts
 };
 // One series at one length, as the host reads it back.
 struct ChainRes {
@@ -291,14 +297,25 @@ struct ChainArgs {
     int witness_max, dbudget, drows, sym;
     int nser, k, nu, spec, rounds;
     int minL, step;            // length j is minL + j step
+    // The range of a length: shift = 0, from the previous discords' distances at this length
+    // computed by the previous length's exact rounds (m2); shift > 0, from a launch of its own
+    // on rows i, i-1 .. i-shift of each previous discord i (Engine::prev_shift_), in gr group
+    // slots of nur rows per series
+    int shift, gr, nur;
+    int64_t *candr;            // [series][gr][nur] rows of the range launch
+    int *cntr;                 // [series][gr] their number
+    unsigned long long *keyr;  // [series][gr][nur][EXACT_SLOTS] its results
 };
-// Start of the loop's first length: one workgroup.
-__global__ void k_chain_begin(ChainArgs C);
 // Exact top-k of one series per workgroup of one wave; round = 0 .. rounds (the last only
 // finishes).
 __global__ void k_chain_topk(ChainArgs C, int round);
-// Statistics of the length just run and the start of the next: one workgroup.
+// Statistics of the length just run and the start of the next (also of the loop's first,
+// launched once with ChainCtl::j = j0 - 1 and m = 0): one workgroup.
 __global__ void k_chain_next(ChainArgs C);
+// Shifted range (shift > 0), one wave per series: the rows of the range launch, after
+// k_chain_next; then, after that launch, the range and the series' pass table entry.
+__global__ void k_chain_rreq(ChainArgs C);
+__global__ void k_chain_range(ChainArgs C);
 
 // Scan seeds: the MFMA reduction length (m + 15 rounded up to 4) and the series ring
 // capacity (a multiple of 16 holding a 256-column super-tile window of 256 + K4 columns).

@@ -1708,29 +1708,32 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs E)
                          item, dsh, X);
 }
 
-// Device-planned launch of the length loop on the device: series s may ask for up to NU
-// candidates a round (k_chain_topk writes them at cand[s NU ..] and their number at
-// dev_count[s]); block = (series, chunk of its columns at the loop's shortest length), found by
-// binary search in goff (fixed per batch), so blocks of a series without candidates, and
-// chunks past its windows at the current length, end at once.  The lengths come from the
-// device too (dev_m[0] = m, dev_m[1] = m2).
-template <int NU>
+// Device-planned launch of the length loop on the device: series s has `gslots` groups of up
+// to NU candidates (k_chain_topk or k_chain_rreq writes group g's candidates at
+// cand[(s gslots + g) NU ..] and their number at dev_count[s gslots + g]); block = (group slot,
+// series, chunk of its columns at the loop's first, longest windows), the series found by
+// binary search in goff (fixed per batch; E.ngroups = the number of series), so blocks of an
+// empty group, and chunks past its series' windows at the current length, end at once.  The
+// lengths come from the device too (dev_m[0] = m, dev_m[1] = m2).
+template <int NU, bool DUAL>
 __global__ __launch_bounds__(256) void k_exact_chain(ExactArgs E)
 {
     extern __shared__ double dsh[];
     __shared__ ExactShared X;
-    const int64_t item = blockIdx.x;
+    const int64_t total = E.goff[E.ngroups];
+    const int64_t slot = blockIdx.x / total, item = blockIdx.x - slot * total;
     int lo = 0, hi = E.ngroups - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (E.goff[mid] <= item) lo = mid; else hi = mid - 1;
     }
-    const int nU = E.dev_count[lo];
+    const int gi = lo * E.gslots + (int)slot;
+    const int nU = E.dev_count[gi];
     const int m = E.dev_m[0], m2 = E.dev_m[1];
     if (nU <= 0 || m <= 0) return;
     const int64_t chunk = item - E.goff[lo];
     if (chunk * EXACT_COLS >= E.ser[lo].n - m + 1) return;
-    exact_item<NU, true>(E, m, m2, lo, lo * NU, nU, chunk, item, dsh, X);
+    exact_item<NU, DUAL>(E, m, m2, lo, gi * NU, nU, chunk, blockIdx.x, dsh, X);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1746,8 +1749,10 @@ __device__ __forceinline__ double ord_val64(unsigned long long k)
 // evaluated exactly at this length, as Engine::fast takes it (the minimum over the discords
 // still inside the series, times 1 - 1e-6; every discord must have its value), the pass's
 // series table (N, thr) and scalars (Engine::fill_pass), and the lengths k_derive and
-// k_exact_chain read.  A series without k such values stops the loop at this length.  Run by
-// one workgroup: k_chain_begin for the loop's first length, k_chain_next for the others.
+// k_exact_chain read.  A series without k such values stops the loop at this length.  With a
+// shifted range (C.shift > 0) the range and the series' pass table entries come later
+// (k_chain_range); here only the number of discords is checked.  Run by k_chain_next (one
+// workgroup).
 __device__ __forceinline__ void chain_begin(const ChainArgs &C, const int j, int *s_bad,
                             unsigned long long *s_nmax)
 {
@@ -1771,16 +1776,18 @@ __device__ __forceinline__ void chain_begin(const ChainArgs &C, const int j, int
         int np = R ? R->n : cs.nprev;
         np = np < 0 ? 0 : (np > CHAIN_KMAX ? CHAIN_KMAX : np);
         bool ok = np >= C.k;
-        double mn = INFINITY;
-        int cnt = 0;
-        for (int u = 0; u < np; u++) {
-            ok &= pv[u] >= 0.0;
-            if (prev[u] >= Nq) continue;
-            cnt++;
-            if (pv[u] < 3.0e38) mn = fmin(mn, pv[u]);
-        }
-        ok &= cnt >= C.k && mn < INFINITY && mn > 0.0;
-        cs.r2 = ok ? mn * (1.0 - 1e-6) : 0.0;
+        if (C.shift == 0) {
+            double mn = INFINITY;
+            int cnt = 0;
+            for (int u = 0; u < np; u++) {
+                ok &= pv[u] >= 0.0;
+                if (prev[u] >= Nq) continue;
+                cnt++;
+                if (pv[u] < 3.0e38) mn = fmin(mn, pv[u]);
+            }
+            ok &= cnt >= C.k && mn < INFINITY && mn > 0.0;
+            cs.r2 = ok ? mn * (1.0 - 1e-6) : 0.0;
+        }  // shifted: the range comes from the length's own launch (k_chain_range)
         cs.status = 0;
         cs.nreq = 0;
         if (!ok) atomicOr(s_bad, 1);
@@ -1788,7 +1795,7 @@ __device__ __forceinline__ void chain_begin(const ChainArgs &C, const int j, int
     }
     __syncthreads();
     const bool halted = halted0 || *s_bad != 0;
-    for (int q = t; q < C.nser; q += nt) {
+    for (int q = t; q < C.nser && (C.shift == 0 || halted); q += nt) {
         SerDev S = C.ser[q];
         S.N = halted ? 0 : S.n - m + 1;
         S.thr = (float)(1.0 - C.cs[q].r2 / (2.0 * m));
@@ -1822,15 +1829,107 @@ __device__ __forceinline__ void chain_begin(const ChainArgs &C, const int j, int
         R.items = C.items;
         ctl->j = j;
         ctl->m = halted ? 0 : m;
-        ctl->m2 = halted ? 0 : m + C.step;
+        ctl->m2 = (halted || C.shift) ? 0 : m + C.step;
     }
 }
 
-__global__ __launch_bounds__(1024) void k_chain_begin(ChainArgs C)
+// Shifted range, rows: each previous discord i of series q (one wave) at rows i, i-1 ..
+// i-shift inside the series at this length, as Engine::fast asks for them, in group slots of
+// nu rows (spare rows of a group copy its first), their result slots cleared.
+__global__ __launch_bounds__(64) void k_chain_rreq(ChainArgs C)
 {
-    __shared__ int s_bad;
-    __shared__ unsigned long long s_nmax;
-    chain_begin(C, C.ctl->j, &s_bad, &s_nmax);
+    __shared__ int64_t s_row[CHAIN_KMAX * (EXACT_EXT + 1)];
+    __shared__ int s_nrq;
+    const int q = blockIdx.x, t = threadIdx.x;
+    ChainCtl *ctl = C.ctl;
+    ChainSer &cs = C.cs[q];
+    const int m = ctl->m;  // 0: the loop has stopped
+    const int G = C.gr, nu = C.nur;
+    if (t == 0) {
+        int nrq = 0;
+        if (m > 0) {
+            const int j = ctl->j;
+            const ChainRes *R =
+                j > ctl->j0 ? &C.res[(size_t)(j - 1 - ctl->j0) * C.nser + q] : nullptr;
+            const int64_t *prev = R ? R->i : cs.prev;
+            int np = R ? R->n : cs.nprev;
+            np = np < 0 ? 0 : (np > CHAIN_KMAX ? CHAIN_KMAX : np);
+            const int64_t Nq = C.ser[q].n - m + 1;
+            for (int u = 0; u < np; u++)
+                for (int d = 0; d <= C.shift && nrq < G * nu; d++) {
+                    const int64_t row = prev[u] - d;
+                    if (row < 0 || row >= Nq) continue;
+                    s_row[nrq] = row;
+                    cs.rq_u[nrq] = (unsigned char)u;
+                    nrq++;
+                }
+        }
+        cs.nrq = nrq;
+        s_nrq = nrq;
+    }
+    __syncthreads();
+    const int nrq = s_nrq;
+    for (int g = 0; g < G; g++) {
+        const int c = nrq - g * nu;
+        const int cg = c < 0 ? 0 : (c > nu ? nu : c);
+        if (t == 0) C.cntr[q * G + g] = cg;
+        if (t < nu && cg > 0) C.candr[((size_t)q * G + g) * nu + t] = s_row[g * nu + (t < cg ? t : 0)];
+        for (int e = t; e < cg * EXACT_SLOTS; e += 64)
+            C.keyr[((size_t)q * G + g) * nu * EXACT_SLOTS + e] = ~0ull;
+    }
+}
+
+// Shifted range, value (one wave per series): per previous discord the largest of its rows'
+// exact distances, the minimum of those over the discords, times 1 - 1e-6 (Engine::fast's
+// rule); the series' pass table entry.  A series without k values switches itself off and
+// stops the loop at this length.
+__global__ __launch_bounds__(64) void k_chain_range(ChainArgs C)
+{
+    __shared__ double s_best[CHAIN_KMAX];
+    __shared__ int s_any[CHAIN_KMAX];
+    const int q = blockIdx.x, t = threadIdx.x;
+    ChainCtl *ctl = C.ctl;
+    const int m = ctl->m;
+    if (m <= 0) return;  // stopped: chain_begin switched the pass table off
+    ChainSer &cs = C.cs[q];
+    const int nrq = cs.nrq, G = C.gr, nu = C.nur;
+    if (t < CHAIN_KMAX) {
+        s_best[t] = -INFINITY;
+        s_any[t] = 0;
+    }
+    __syncthreads();
+    for (int r = 0; r < nrq; r++) {
+        const size_t idx = ((size_t)q * G + r / nu) * nu + r % nu;
+        unsigned long long k = C.keyr[idx * EXACT_SLOTS + t];
+        for (int w = 32; w >= 1; w >>= 1) {
+            const unsigned long long o = __shfl_xor(k, w, 64);
+            k = o < k ? o : k;
+        }
+        if (t == 0) {
+            const double v = ord_val64(k);
+            const int u = cs.rq_u[r];
+            s_any[u] = 1;
+            if (v < 3.0e38) s_best[u] = fmax(s_best[u], v);
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        double mn = INFINITY;
+        int cnt = 0;
+        for (int u = 0; u < CHAIN_KMAX; u++)
+            if (s_any[u]) {
+                mn = fmin(mn, s_best[u] > -INFINITY ? s_best[u] : INFINITY);
+                cnt++;
+            }
+        const bool ok = cnt >= C.k && mn < INFINITY && mn > 0.0;
+        const double r2 = ok ? mn * (1.0 - 1e-6) : 0.0;
+        cs.r2 = r2;
+        SerDev S = C.ser[q];
+        S.N = ok ? S.n - m + 1 : 0;
+        S.thr = (float)(1.0 - r2 / (2.0 * m));
+        C.pser[q] = S;
+        if (!ok) atomicCAS(&ctl->halt, 0, ctl->j + 1);
+    }
 }
 
 // A candidate of the device top-k: its upper bound hi, local row i and position p in the
@@ -2147,14 +2246,25 @@ __global__ __launch_bounds__(1024) void k_chain_next(ChainArgs C)
     chain_begin(C, j + 1, &s_bad, &s_nmax);
 }
 
-void launch_exact_chain(const ExactArgs &E, int nu, unsigned grid, size_t lds, hipStream_t st)
+template <bool DUAL>
+static void launch_exact_chain_t(const ExactArgs &E, int nu, unsigned grid, size_t lds,
+                                 hipStream_t st)
 {
     switch (nu) {
-    case 1: hipLaunchKernelGGL((k_exact_chain<1>), dim3(grid), dim3(256), lds, st, E); break;
-    case 2: hipLaunchKernelGGL((k_exact_chain<2>), dim3(grid), dim3(256), lds, st, E); break;
-    case 3: hipLaunchKernelGGL((k_exact_chain<3>), dim3(grid), dim3(256), lds, st, E); break;
-    default: hipLaunchKernelGGL((k_exact_chain<4>), dim3(grid), dim3(256), lds, st, E); break;
+    case 1: hipLaunchKernelGGL((k_exact_chain<1, DUAL>), dim3(grid), dim3(256), lds, st, E); break;
+    case 2: hipLaunchKernelGGL((k_exact_chain<2, DUAL>), dim3(grid), dim3(256), lds, st, E); break;
+    case 3: hipLaunchKernelGGL((k_exact_chain<3, DUAL>), dim3(grid), dim3(256), lds, st, E); break;
+    default: hipLaunchKernelGGL((k_exact_chain<4, DUAL>), dim3(grid), dim3(256), lds, st, E); break;
     }
+}
+
+void launch_exact_chain(const ExactArgs &E, int nu, bool dual, unsigned grid, size_t lds,
+                        hipStream_t st)
+{
+    if (dual)
+        launch_exact_chain_t<true>(E, nu, grid, lds, st);
+    else
+        launch_exact_chain_t<false>(E, nu, grid, lds, st);
 }
 
 template <bool DUAL>
@@ -2174,6 +2284,12 @@ static void launch_exact_t(const ExactArgs &E, int nu, unsigned grid, size_t lds
 
 void launch_exact(const ExactArgs &E, int nu, unsigned grid, size_t lds, hipStream_t st)
 {
+    // One candidate at two lengths compiles to 256 registers (one wave per SIMD); the group of
+    // two (the spare row a copy of the first, its result ignored) runs three waves per SIMD.
+    if (E.m2 && nu == 1 && exact_lds_bytes(E.m, 2) <= 65536 - 1024) {
+        nu = 2;
+        lds = exact_lds_bytes(E.m, 2);
+    }
     if (E.m2)
         launch_exact_t<true>(E, nu, grid, lds, st);
     else

@@ -1,8 +1,10 @@
 """The length loop on the device (paddi_tunables.device_lengths).
 
 Every length after the first runs as one captured graph: its range from the previous length's
-discords, the pass, the exact top-k (k_chain_topk, k_exact_chain), and the next length's range
-inputs; the host only replays graphs and synchronises every `device_lengths` lengths.  The loop
+discords (their distances at the new length from the previous length's exact rounds, or, with
+prev_shift > 0, from a launch of its own on the shifted rows), the pass, the exact top-k
+(k_chain_topk, k_exact_chain); the host only replays graphs and synchronises every
+`device_lengths` lengths.  The loop
 must return what the host loop (src/main.cu:136-233's length loop as Engine::fast runs it)
 returns, bit for bit: the same kernels compute every exact value, the greedy top-k and the range
 rule are the same.  A length the device cannot finish (more exact rounds than its graph holds,
@@ -53,49 +55,52 @@ def restore(ctx):
     ctx.set_tunables(**saved)
 
 
-def run(ctx, name, mode, **tun):
+def run(ctx, name, mode, shift=0, **tun):
     series, minL, maxL, step, k = CASES[name]
-    ctx.set_tunables(prev_shift=0, **tun)
+    ctx.set_tunables(prev_shift=shift, **tun)
     return ctx.discords_batch(series(), minL, maxL, step, k, mode=mode, with_stats=True)
 
 
-def host(ctx, name, mode):
-    if (name, mode) not in _HOST:
-        _HOST[(name, mode)] = run(ctx, name, mode, device_lengths=0)
-    return _HOST[(name, mode)]
+def host(ctx, name, mode, shift=0):
+    if (name, mode, shift) not in _HOST:
+        _HOST[(name, mode, shift)] = run(ctx, name, mode, shift, device_lengths=0)
+    return _HOST[(name, mode, shift)]
 
 
+@pytest.mark.parametrize("shift", [0, 2])
 @pytest.mark.parametrize("mode", ["batched", "learned"])
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_device_loop_equals_host_loop(ctx, restore, name, mode):
-    ref, st0 = host(ctx, name, mode)
-    print(f"device loop {name} {mode}: {len(ref)} discords", flush=True)
+def test_device_loop_equals_host_loop(ctx, restore, name, mode, shift):
+    ref, st0 = host(ctx, name, mode, shift)
+    print(f"device loop {name} {mode} shift {shift}: {len(ref)} discords", flush=True)
     for every in (1, 5, 64):
-        got, st = run(ctx, name, mode, device_lengths=every, device_rounds=2)
-        assert key(got) == key(ref), (name, mode, every)
+        got, st = run(ctx, name, mode, shift, device_lengths=every, device_rounds=2)
+        assert key(got) == key(ref), (name, mode, shift, every)
         # the lengths the device ran were no host passes: fewer eager scan launches
         assert st["launches"]["select"] < st0["launches"]["select"], (st, st0)
     # twice in a row: the graphs captured by the first call are replayed by the second
-    again, _ = run(ctx, name, mode, device_lengths=5, device_rounds=2)
+    again, _ = run(ctx, name, mode, shift, device_lengths=5, device_rounds=2)
     assert key(again) == key(ref)
 
 
+@pytest.mark.parametrize("shift", [0, 1])
 @pytest.mark.parametrize("name", ["rw_k3", "uneven_k2_step3"])
-def test_lengths_the_device_cannot_finish_go_back_to_the_host(ctx, restore, name):
+def test_lengths_the_device_cannot_finish_go_back_to_the_host(ctx, restore, name, shift):
     """One exact round per length and one candidate per round: every length whose top-k needs
     a second round stops the device loop there; the host finishes it and restarts the loop."""
-    ref, _ = host(ctx, name, "batched")
-    got, st = run(ctx, name, "batched", device_lengths=3, device_rounds=1, speculate=1)
+    ref, _ = host(ctx, name, "batched", shift)
+    got, st = run(ctx, name, "batched", shift, device_lengths=3, device_rounds=1, speculate=1)
     assert key(got) == key(ref)
-    got4, _ = run(ctx, name, "batched", device_lengths=3, device_rounds=4, speculate=4)
+    got4, _ = run(ctx, name, "batched", shift, device_lengths=3, device_rounds=4, speculate=4)
     assert key(got4) == key(ref)
 
 
-def test_device_loop_matches_the_oracle(ctx, restore):
+@pytest.mark.parametrize("shift", [0, 3])
+def test_device_loop_matches_the_oracle(ctx, restore, shift):
     x = synth.rw_planted(8000, 9, 2, 64, 16.0)
     minL, maxL, k = 40, 52, 2
     ref = O.palmad(x, minL, maxL, 1, k)
-    ctx.set_tunables(prev_shift=0, device_lengths=4, device_rounds=2)
+    ctx.set_tunables(prev_shift=shift, device_lengths=4, device_rounds=2)
     got = ctx.discords(x, minL, maxL, 1, k, mode="batched")
     exp = [(int(i), r["m"], float(d)) for r in ref for i, d in zip(r["idx"], r["d2"])]
     assert [(g.index, g.length) for g in got] == [(i, m) for i, m, _ in exp]
