@@ -39,6 +39,8 @@ GRID = {
     "bound_candidates": [16, 32, 64],
     "direct_budget": [0, 400, 550, 700, 850, 1000],
     "direct_rows": [1, 2, 3, 4],
+    "device_lengths": [0, 8, 32],
+    "device_rounds": [1, 2, 3],
 }
 
 
