@@ -290,6 +290,8 @@ Engine::Engine(int device) : device_(device)
     scratch_ = scratch_setup(device);
     const char *dbg = std::getenv("PADDI_DEBUG");
     debug_ = dbg && dbg[0] == '1';
+    const char *enn = std::getenv("PADDI_EXACT_NN");
+    if (enn && enn[0] == '0') exact_nn_ = false;
     const char *lim = std::getenv("PADDI_WAIT_LIMIT_S");
     if (lim && *lim) wait_limit_s_ = std::atof(lim);
 }
@@ -880,7 +882,7 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
 // Exact (double) NN distance of (series, local row) requests.
 void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &req,
                    std::vector<double> &out, paddi_stats *st, int m2,
-                   std::vector<double> *out2)
+                   std::vector<double> *out2, const std::vector<double> *ub, bool lb)
 {
     out.assign(req.size(), 0.0);
     if (out2) out2->assign(req.size(), DBL_MAX);
@@ -891,8 +893,9 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
             const size_t e = std::min(req.size(), a + EXACT_NOALLOC_CHUNK);
             const std::vector<std::pair<int, int64_t>> sub(req.begin() + (std::ptrdiff_t)a,
                                                            req.begin() + (std::ptrdiff_t)e);
-            std::vector<double> o, o2;
-            exact(B, m, sub, o, st, m2, out2 ? &o2 : nullptr);
+            std::vector<double> o, o2, subub;
+            if (ub) subub.assign(ub->begin() + (std::ptrdiff_t)a, ub->begin() + (std::ptrdiff_t)e);
+            exact(B, m, sub, o, st, m2, out2 ? &o2 : nullptr, ub ? &subub : nullptr, lb);
             std::copy(o.begin(), o.end(), out.begin() + (std::ptrdiff_t)a);
             if (out2) std::copy(o2.begin(), o2.end(), out2->begin() + (std::ptrdiff_t)a);
         }
@@ -901,7 +904,10 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
     // the second length needs its statistics from the last derive
     if (!out2 || m2 <= m || m2 > m + EXACT_EXT || m2 != derive_m2_) m2 = 0;
     // the series table on the device is derive(m)'s: every series at N = n - m + 1
-    const int umax = exact_max_u(m);
+    // k_nn (matrix cores, double precision where it matters) unless turned off; the
+    // no-allocation call keeps k_exact, whose staging it sizes when the context is created
+    const bool nn = exact_nn_ && !t_noalloc;
+    const int umax = nn ? nn_max_u(m) : exact_max_u(m);
     std::vector<size_t> order(req.size());
     for (size_t q = 0; q < req.size(); q++) order[q] = q;
     std::stable_sort(order.begin(), order.end(),
@@ -960,7 +966,16 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
     const size_t nc = cand.size(), ngx = gfirst.size();
     const size_t nk = nc * EXACT_SLOTS, nkeys = m2 ? 2 * nk : nk;
     const size_t o_cand = nkeys * 8, o_goff = o_cand + nc * 8, o_gfirst = o_goff + ngx * 8,
-                 o_gser = o_gfirst + ngx * 4, xbytes = o_gser + ngx * 4;
+                 o_gser = o_gfirst + ngx * 4, o_ub = (o_gser + ngx * 4 + 7) & ~(size_t)7,
+                 o_run = o_ub + (nn ? nc * 8 : 0), o_csid = o_run + (nn ? nk * 4 : 0),
+                 xbytes = o_csid + (nn ? nc * 4 : 0);
+    // k_nn's filter pass: a lower bound per candidate and 256-column super-tile
+    int64_t tiles = 0;
+    if (nn) {
+        for (size_t q = 0; q < req.size(); q++)
+            tiles = std::max<int64_t>(tiles, (B.n[(size_t)req[q].first] - m + 1 + 255) / 256);
+        if (!lb) nnlo_.reserve((size_t)nc * (size_t)std::max<int64_t>(tiles, 1) * 4);
+    }
     if (xbytes > pin_x_bytes_) {
         if (t_noalloc) refuse_growth("the pinned exact-refinement staging");
         if (pin_x_) HIPC(hipHostFree(pin_x_));
@@ -975,6 +990,16 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
     std::memcpy(hx + o_goff, goff.data(), ngx * 8);
     std::memcpy(hx + o_gfirst, gfirst.data(), ngx * 4);
     std::memcpy(hx + o_gser, gser.data(), ngx * 4);
+    if (nn) {
+        double *hu = (double *)(hx + o_ub);
+        int *hr = (int *)(hx + o_run);
+        int *hs = (int *)(hx + o_csid);
+        for (size_t q = 0; q < nc; q++) {
+            hu[q] = ub ? (*ub)[slot[q]] : INFINITY;
+            hs[q] = req[slot[q]].first;
+        }
+        for (size_t q = 0; q < nk; q++) hr[q] = 0x7fffffff;  // ord_key above every float
+    }
     xarg_.reserve(xbytes);
     char *dx = (char *)xarg_.p;
     HIPC(hipMemcpyAsync(dx, hx, xbytes, hipMemcpyHostToDevice, stream_));
@@ -995,7 +1020,37 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
         E.m = m;
         E.m2 = m2;
         E.sym = excl_sym_ ? 1 : 0;
-        launch_exact(E, L.nu, (unsigned)L.blocks, exact_lds_bytes(m, L.nu), stream_);
+        E.colv = colv_.as<float4>();
+        E.tmax = tmax_.as<float2>();
+        E.ub = (const double *)(dx + o_ub);
+        E.runub = (int *)(dx + o_run);
+        E.lb = lb ? 1 : 0;
+        E.tilelo = nnlo_.as<float>();
+        E.tiles = (int)tiles;
+        if (nn)
+            launch_nn(E, L.nu, m2 != 0, lb, false, (unsigned)L.blocks, nn_lds_bytes(m, L.nu),
+                      stream_);
+        else
+            launch_exact(E, L.nu, (unsigned)L.blocks, exact_lds_bytes(m, L.nu), stream_);
+        HIPC(hipGetLastError());
+    }
+    if (nn && !lb && nc > 0 && tiles > 0) {
+        // the double-precision pass over the candidates' qualifying super-tiles
+        ExactArgs F{};
+        F.x = B.x.as<float>();
+        F.ser = B.ser.as<SerDev>();
+        F.st64 = st64_.as<double2>();
+        F.cand = (const int64_t *)(dx + o_cand);
+        F.outkey = (unsigned long long *)dx;
+        F.m = m;
+        F.sym = excl_sym_ ? 1 : 0;
+        F.ub = (const double *)(dx + o_ub);
+        F.runub = (int *)(dx + o_run);
+        F.tilelo = nnlo_.as<float>();
+        F.tiles = (int)tiles;
+        F.tb = (int)((tiles + 63) / 64);
+        F.csid = (const int *)(dx + o_csid);
+        launch_nn_fix(F, false, (unsigned)(nc * (size_t)F.tb), stream_);
         HIPC(hipGetLastError());
     }
     HIPC(hipEventRecord(ev_[7], stream_));
@@ -1023,6 +1078,32 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
         st->launches[3] += (int64_t)launches.size();
         st->exact_refines += (int64_t)cand.size();
     }
+    // A value k_nn did not produce (no super-tile qualified, or no valid column: never when
+    // its error bound holds) is computed by k_exact over every column.
+    if (nn) {
+        std::vector<std::pair<int, int64_t>> again;
+        std::vector<size_t> where;
+        for (size_t q = 0; q < req.size(); q++)
+            if (!std::isfinite(out[q]) || (out2 && m2 && !std::isfinite((*out2)[q]))) {
+                again.push_back(req[q]);
+                where.push_back(q);
+            }
+        if (!again.empty()) {
+            std::vector<double> o, o2;
+            exact_nn_ = false;
+            try {
+                exact(B, m, again, o, st, m2, out2 ? &o2 : nullptr);
+            } catch (...) {
+                exact_nn_ = true;
+                throw;
+            }
+            exact_nn_ = true;
+            for (size_t q = 0; q < where.size(); q++) {
+                out[where[q]] = o[q];
+                if (out2) (*out2)[where[q]] = o2[q];
+            }
+        }
+    }
 }
 
 void Engine::topk_all(Batch &B, int m, std::vector<TopK> &tk, paddi_stats *st, int m2)
@@ -1030,19 +1111,23 @@ void Engine::topk_all(Batch &B, int m, std::vector<TopK> &tk, paddi_stats *st, i
     for (;;) {
         std::vector<std::pair<int, int64_t>> req;
         std::vector<std::pair<int, size_t>> who;
+        std::vector<double> ub;  // the scans' upper bound of each candidate's distance
         std::vector<size_t> need;
         for (int q = 0; q < B.nser; q++) {
             TopK &T = tk[(size_t)q];
             if (T.done) continue;
             if (T.advance(need))
                 for (size_t c : need) {
-                    req.emplace_back(q, T.cands[c].i);
+                    const Cand &d = T.cands[c];
+                    req.emplace_back(q, d.i);
                     who.emplace_back(q, c);
+                    ub.push_back(d.a >= 3.0e38f ? INFINITY
+                                                : (double)d.ahi + conv_ed2(m, d.ahi));
                 }
         }
         if (req.empty()) break;
         std::vector<double> v, v2;
-        exact(B, m, req, v, st, m2, &v2);
+        exact(B, m, req, v, st, m2, &v2, &ub);
         for (size_t u = 0; u < who.size(); u++) {
             Cand &c = tk[(size_t)who[u].first].cands[who[u].second];
             c.exact = true;
@@ -1238,8 +1323,9 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
                             ngrp++;
                         }
                     }
+            // lower bounds suffice for the range (k_nn in single precision only)
             std::vector<double> v;
-            exact(B, m, req, v, st);
+            exact(B, m, req, v, st, 0, nullptr, nullptr, true);
             std::vector<double> best((size_t)ngrp, -INFINITY);
             for (size_t u = 0; u < req.size(); u++)
                 if (v[u] < 3.0e38) best[(size_t)grp[u]] = std::max(best[(size_t)grp[u]], v[u]);
@@ -1376,7 +1462,7 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ah
     const int cls_max = (int)std::min<int64_t>(round_up(mlast, 64), MAXM);
     // candidates per round: the speculation width, at least two where they fit (one candidate
     // at two lengths runs one wave per SIMD, see launch_exact)
-    const int umax = std::min(CHAIN_NU, exact_max_u(cls_max));
+    const int umax = std::min(CHAIN_NU, exact_nn_ ? nn_max_u(cls_max) : exact_max_u(cls_max));
     const int nu = std::max(std::min(2, umax), std::min(spec_, umax));
     const int rounds = std::max(1, std::min(device_rounds_, CHAIN_ROUNDS));
     // range launch: gr group slots of nur rows per series, the rows split evenly
@@ -1405,6 +1491,12 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ah
     ch_xv2_.reserve((size_t)scap * 8);
     ch_cand_.reserve((size_t)ns * CHAIN_NU * 8);
     ch_cnt_.reserve((size_t)ns * 4);
+    ch_ub_.reserve((size_t)ns * CHAIN_NU * 8);
+    int64_t ttiles = 1;  // super-tiles per series at the first (longest) length
+    for (int q = 0; q < ns; q++)
+        ttiles = std::max<int64_t>(ttiles, (B.n[(size_t)q] - m0 + 1 + 255) / 256);
+    ch_tilelo_.reserve((size_t)ns * CHAIN_NU * (size_t)ttiles * 4);
+    ch_runub_.reserve((size_t)ns * CHAIN_NU * EXACT_SLOTS * 4);
     ch_key_.reserve((size_t)ns * CHAIN_NU * EXACT_SLOTS * 8);
     ch_key2_.reserve((size_t)ns * CHAIN_NU * EXACT_SLOTS * 8);
     ch_goff_.reserve((size_t)(ns + 1) * 8);
@@ -1446,6 +1538,8 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ah
     C.xv2 = ch_xv2_.as<double>();
     C.cand = ch_cand_.as<int64_t>();
     C.cnt = ch_cnt_.as<int>();
+    C.ub = ch_ub_.as<double>();
+    C.runub = ch_runub_.as<int>();
     C.outkey = ch_key_.as<unsigned long long>();
     C.outkey2 = ch_key2_.as<unsigned long long>();
     C.scap = scap;
@@ -1513,12 +1607,24 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ah
         E.dev_m = &dctl->m;
         E.dev_count = C.cnt;
         E.gslots = 1;
-        ExactArgs Er = E;  // the shifted range's launch
+        E.colv = colv_.as<float4>();
+        E.tmax = tmax_.as<float2>();
+        E.ub = C.ub;
+        E.runub = C.runub;
+        E.tilelo = ch_tilelo_.as<float>();
+        E.tiles = (int)ttiles;
+        E.tb = (int)((ttiles + 63) / 64);
+        E.nu_dev = nu;
+        ExactArgs Er = E;  // the shifted range's launch: lower bounds
         Er.cand = C.candr;
         Er.outkey = C.keyr;
         Er.outkey2 = C.keyr;
         Er.dev_count = C.cntr;
         Er.gslots = gr;
+        Er.ub = nullptr;
+        Er.runub = nullptr;
+        Er.lb = 1;
+        const bool nn = exact_nn_;
         const int64_t refine_grid = std::max<int64_t>(refine_waves_ + refine_waves_ / 2, nwords + 64);
         std::string sig;
         auto put = [&](const void *p, size_t n) { sig.append((const char *)p, n); };
@@ -1526,7 +1632,7 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ah
         put(&E, sizeof E);
         put(&Er, sizeof Er);
         put(&Q, sizeof Q);
-        const int64_t v[] = {rows,     nwords,          (int64_t)lds_scan, (int64_t)lds_ex,
+        const int64_t v[] = {rows, nwords, (int64_t)lds_scan, (int64_t)lds_ex, (int64_t)exact_nn_,
                              xblocks,  refine_grid,     (int64_t)(uintptr_t)pin_ch_,
                              (int64_t)(uintptr_t)B.P.p, (int64_t)(uintptr_t)B.refs.p,
                              (int64_t)(uintptr_t)par_.p};
@@ -1546,7 +1652,10 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ah
                                tmax_.as<float2>(), 0,
                                ahead ? (double2 *)S[par ^ 1] : st64n_.as<double2>(), &dctl->m);
             if (!ahead) {
-                if (xblocks > 0)
+                if (xblocks > 0 && nn)
+                    launch_nn(Er, nur, false, true, true, (unsigned)(xblocks * gr),
+                              nn_lds_bytes(cls, nur), stream_);
+                else if (xblocks > 0)
                     launch_exact_chain(Er, nur, false, (unsigned)(xblocks * gr),
                                        exact_lds_bytes(cls, nur), stream_);
                 hipLaunchKernelGGL(k_chain_range, dim3(ns), dim3(64), 0, stream_, C);
@@ -1554,7 +1663,12 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ah
             enqueue_pass(Q, lds_scan, 0, rows, nwords, 0, false, nullptr, false);
             for (int r = 0; r < rounds; r++) {
                 hipLaunchKernelGGL(k_chain_topk, dim3(ns), dim3(64), 0, stream_, C, r);
-                if (xblocks > 0)
+                if (xblocks > 0 && nn) {
+                    launch_nn(E, nu, ahead, false, true, (unsigned)xblocks,
+                              nn_lds_bytes(cls, nu), stream_);
+                    launch_nn_fix(E, true, (unsigned)(ns * nu * E.tb), stream_);
+                }
+                else if (xblocks > 0)
                     launch_exact_chain(E, nu, ahead, (unsigned)xblocks, lds_ex, stream_);
             }
             hipLaunchKernelGGL(k_chain_topk, dim3(ns), dim3(64), 0, stream_, C, rounds);

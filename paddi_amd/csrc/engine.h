@@ -151,9 +151,13 @@ public:
                           std::vector<std::vector<int64_t>> &cand, paddi_stats *st);
     // Exact NN ED^2 of (series, local row) requests at length m (derive(m) must be current);
     // with m2 in (m, m + EXACT_EXT] also at m2 into out2 (DBL_MAX: none).
+    // ub: per request an upper bound of its distance (the scans' bound; null: none); lb:
+    // lower bounds suffice (k_nn without double precision).  With k_nn the second length's
+    // values are lower bounds.
     void exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &req,
                std::vector<double> &out, paddi_stats *st, int m2 = 0,
-               std::vector<double> *out2 = nullptr);
+               std::vector<double> *out2 = nullptr, const std::vector<double> *ub = nullptr,
+               bool lb = false);
     // Run the exact top-k of every series to completion (batched exact refines); with m2,
     // every refined candidate also gets its exact distance at m2.
     void topk_all(Batch &B, int m, std::vector<TopK> &tk, paddi_stats *st, int m2 = 0);
@@ -185,6 +189,7 @@ public:
                                      // (0: the host runs every length)
     int device_rounds_ = 2;          // exact rounds per length on the device
     bool excl_sym_ = false;          // exclusion rule: |i - j| >= m - 1 instead of |i - 1 - j| >= m
+    bool exact_nn_ = true;           // k_nn for exact refinement (PADDI_EXACT_NN=0: k_exact)
     bool debug_ = false;             // per-pass trace on stderr (PADDI_DEBUG=1)
     double wait_limit_s_ = 600.0;    // longest wait for queued device work (PADDI_WAIT_LIMIT_S)
     std::vector<paddi_attempt_stats> *att_log_ = nullptr;  // per-attempt records, when kept
@@ -250,7 +255,7 @@ private:
     };
     std::map<std::pair<uintptr_t, int>, ChainGraph> chain_graphs_;
     DevBuf ch_ctl_, ch_ser_, ch_res_, ch_xv_, ch_xv2_, ch_cand_, ch_cnt_, ch_key_, ch_key2_,
-        ch_goff_, ch_candr_, ch_cntr_, ch_keyr_;
+        ch_goff_, ch_candr_, ch_cntr_, ch_keyr_, ch_ub_, ch_runub_, ch_tilelo_;
     void *pin_ch_ = nullptr;       // pinned: the loop's control block, then its downloads
     size_t pin_ch_bytes_ = 0;
     void drop_graphs();
@@ -277,6 +282,7 @@ private:
     DevBuf st64n_;                 // (mu, invn) at the last derive's second length
     int derive_m2_ = 0;            // that length (0: none)
     DevBuf xarg_;                  // exact-refine arguments and results (one block)
+    DevBuf nnlo_;                  // k_nn's super-tile lower bounds (its filter pass)
     DevBuf bkey_;                  // per-block candidate keys of bound_candidates
     DevBuf keep_;                  // per-block flags of a restricted pass
     void *pin_x_ = nullptr;        // its pinned staging

@@ -155,6 +155,22 @@ struct ExactArgs {
     const int *dev_m;
     const int *dev_count;
     int gslots;
+    // k_nn only: the columns' single-precision terms at m (ScanArgs::colv) and block scales
+    // (ScanArgs::tmax), per candidate an upper bound of its nearest-neighbour distance (null:
+    // none) and EXACT_SLOTS single-precision upper bounds (ord_key of a float, set to
+    // 0x7fffffff before the launch), and lb = 1 for lower bounds only (no double precision)
+    const float4 *colv;
+    const float2 *tmax;
+    const double *ub;
+    int *runub;
+    int lb;
+    // k_nn's filter pass and k_nn_fix: per candidate `tiles` super-tile lower bounds, k_nn_fix
+    // blocks of 64 super-tiles per candidate (tb), the candidates' series (host-planned) or
+    // the group size (device-planned)
+    float *tilelo;
+    int tiles, tb;
+    const int *csid;
+    int nu_dev;
 };
 constexpr int EXACT_EXT = 4;  // largest m2 - m
 
@@ -204,6 +220,31 @@ void launch_exact(const ExactArgs &E, int nu, unsigned grid, size_t lds, hipStre
 // dual: also at the second length.
 void launch_exact_chain(const ExactArgs &E, int nu, bool dual, unsigned grid, size_t lds,
                         hipStream_t st);
+
+// k_nn (the matrix-core nearest-neighbour kernel, replacing k_exact): groups of nu <= NN_MAXU
+// candidates; dual: also a lower bound at the second length; lb: lower bounds only; dev: the
+// device-planned launch of the length loop (as k_exact_chain).
+constexpr int NN_MAXU = 4;
+void launch_nn(const ExactArgs &E, int nu, bool dual, bool lb, bool dev, unsigned grid,
+               size_t lds, hipStream_t st);
+// k_nn_fix: the double-precision pass after k_nn's filter pass (grid: candidates x tb).
+void launch_nn_fix(const ExactArgs &E, bool dev, unsigned grid, hipStream_t st);
+// Dynamic LDS of k_nn: the chunk's series window (padded every 16 floats) and per candidate
+// its single-precision window (16 zeros either side of K4 + 16).
+inline size_t nn_lds_bytes(int m, int nu)
+{
+    const int K4 = (m + 15 + 3) & ~3;
+    const int xs = 4096 + K4 + 16;
+    const size_t xsf = (size_t)(xs + (xs >> 4) + 2);
+    return ((xsf * 4 + 7) & ~(size_t)7) + (size_t)nu * (K4 + 32) * 4;
+}
+// Largest group of k_nn at length m within 64 KiB of LDS.
+inline int nn_max_u(int m)
+{
+    int u = 1;
+    while (u < NN_MAXU && nn_lds_bytes(m, u + 1) <= 65536 - 2048) u++;
+    return u;
+}
 
 // Squared-distance bounds shared by the host and the device.
 // delta_ed2: bound on |single - exact| squared distance where the windows a cell's chain
@@ -291,6 +332,8 @@ struct ChainArgs {
     double *xv, *xv2;          // per survivor: exact distance at m and at m2 (NaN: not refined)
     int64_t *cand;             // [series][nu] exact candidates of a round (k_exact_chain)
     int *cnt;                  // [series] their number
+    double *ub;                // [series][nu] their upper bounds (k_nn)
+    int *runub;                // [series][nu][EXACT_SLOTS] k_nn's single-precision bounds
     unsigned long long *outkey, *outkey2;  // [series][nu][EXACT_SLOTS] (k_exact_chain)
     int64_t scap;              // survivors xv holds
     int64_t WA, max_chunk, min_chunk, items;   // pass scalars that stay fixed (Engine::fill_pass)

@@ -1737,6 +1737,443 @@ __global__ __launch_bounds__(256) void k_exact_chain(ExactArgs E)
 }
 
 // ---------------------------------------------------------------------------------------
+// k_nn: nearest-neighbour distance of a few candidates on the matrix cores, exact where it
+// matters (replaces k_exact; same arguments and results).
+//
+// A work item is the candidates [u0, u0 + nU) of one series against the EXACT_COLS columns of
+// a chunk, whose series window is staged in LDS once.  Each wave takes four of the chunk's
+// 256-column super-tiles and, per candidate, forms the 256 single-precision covariances of a
+// super-tile as one 16 x 16 x K4 product (v_mfma_f32_16x16x4f32 in the Toeplitz arrangement
+// of super_seeds): A[a][k'] = x[c+16a+k'] - x[c+16a], B[k'][j] = q[k'-j] with q the
+// candidate's window centred on its mean, so D[a][j] = sum_k q[k] (x[c+16a+j+k] - x[c+16a])
+// and cov = D - (mu_n - x[c+16a]) sum(q) at column n = 16a + j.  The super-tile's smallest
+// distance d_t and its error bound E_t decide whether it can hold the nearest neighbour: E_t
+// is twice the scans' single-precision bound delta(m) times the spread ratio of the columns'
+// blocks and their neighbours (largest sqrt(M2) times largest 1/sqrt(M2), from k_derive's
+// block maxima), plus the rounding of the conversion to a distance (conv_ed2).  The exact
+// distance of every column n of the super-tile is at least d_t - E_t, and the nearest
+// neighbour is no farther than UB = min(the candidate's bound from the scans, d_t' + E_t' over
+// the super-tiles seen so far by any workgroup: a running minimum in global memory).  A
+// super-tile with d_t - E_t <= UB is evaluated in double precision, the covariance centred as
+// k_exact's (candidate on its mean, columns offset by the super-tile's first value); the
+// result is the smallest of those values.  With E.lb only the lower bound min_t (d_t - E_t) is
+// formed, nothing in double precision.  DUAL: also a lower bound at the second length m2,
+// from the same product extended by the window's m2 - m extra terms (still centred on its mean
+// at m) and the columns' statistics at m2 (st64n).
+struct NnShared {
+    float invn[NN_MAXU], invn2[NN_MAXU], sq[NN_MAXU], sq2[NN_MAXU];
+    float qe[NN_MAXU][EXACT_EXT];
+    double dinv[NN_MAXU], sa[NN_MAXU], ub[NN_MAXU];
+    double part[NN_MAXU][4], fpart[NN_MAXU][4];
+    int64_t cidx[NN_MAXU];
+    unsigned long long wk[NN_MAXU][4], wk2[NN_MAXU][4];
+    int hk[NN_MAXU][4];
+    int rel[NN_MAXU];   // candidate row - chunk start (the pair tests in int)
+    int in2[NN_MAXU];   // the candidate has a window at m2
+};
+
+__device__ __forceinline__ int nn_pad(int p) { return p + (p >> 4); }
+
+__device__ __forceinline__ float nn_wave_min(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double nn_wave_min(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <int NU, bool DUAL, bool LB>
+__device__ __forceinline__ void nn_item(const ExactArgs &E, const int m, const int m2_arg,
+                                        const int sid, const int u0, const int nU,
+                                        const int64_t chunk, const int64_t slot, char *dsh,
+                                        NnShared &X)
+{
+    // No contraction beyond the fma calls written out: the values must not depend on the
+    // instantiation (group size) that computed them.
+#pragma clang fp contract(off)
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int K4 = (m + 15 + 3) & ~3;
+    const int XS = EXACT_COLS + K4 + 16;
+    const int QZ = K4 + 32;
+    float *xs = (float *)dsh;
+    float *qz = xs + ((XS + (XS >> 4) + 2 + 1) & ~1);  // [NU][QZ]
+    const SerDev S = E.ser[sid];
+    const float *x = E.x + S.xoff;
+    const double2 *st64 = E.st64 + S.roff;
+    const int64_t N = S.n - m + 1;
+    const int m2 = DUAL ? m2_arg : 0;
+    const int ext = DUAL ? m2 - m : 0;
+    const int64_t N2 = DUAL ? S.n - m2 + 1 : 0;
+    // pairs iff j - i >= m - 1 or i - j >= ml (ExactArgs::sym)
+    const int ml = E.sym ? m - 1 : m + 1, ml2 = E.sym ? m2 - 1 : m2 + 1;
+    const int64_t cblk = chunk * EXACT_COLS;
+    // the chunk's window (values past the series only meet columns >= N: zeros)
+    for (int p = t; p < XS; p += 256) {
+        const int64_t g = cblk + p;
+        xs[nn_pad(p)] = g < S.n ? x[g] : 0.f;
+    }
+    // the candidates' windows, centred on their means
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+        const int64_t i = E.cand[u0 + (u < nU ? u : 0)];
+        const double mu = st64[i].x;
+        double pf = 0.0, pd = 0.0;
+        for (int k = t; k < QZ; k += 256) {
+            const int kq = k - 16;
+            const double a = (kq >= 0 && kq < m) ? (double)x[i + kq] - mu : 0.0;
+            const float af = (float)a;
+            qz[u * QZ + k] = af;
+            pf += (double)af;
+            pd += a;
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            pf += __shfl_xor(pf, o, 64);
+            pd += __shfl_xor(pd, o, 64);
+        }
+        if (lane == 0) {
+            X.fpart[u][w] = pf;
+            X.part[u][w] = pd;
+        }
+        if (t == 0) {
+            const double2 s1 = st64[i];
+            X.cidx[u] = i;
+            X.rel[u] = (int)(i - cblk);
+            X.in2[u] = DUAL && i < N2;
+            X.dinv[u] = s1.y;
+            X.invn[u] = (float)s1.y;
+            X.ub[u] = E.ub ? E.ub[u0 + (u < nU ? u : 0)] : INFINITY;
+            const bool in2 = DUAL && i < N2;
+            X.invn2[u] = in2 ? (float)E.st64n[S.roff + i].y : 0.f;
+            for (int e = 0; e < EXACT_EXT; e++)
+                X.qe[u][e] = (e < ext && in2) ? (float)((double)x[i + m + e] - mu) : 0.f;
+        }
+    }
+    __syncthreads();
+    if (t < NU) {
+        const double f = (X.fpart[t][0] + X.fpart[t][1]) + (X.fpart[t][2] + X.fpart[t][3]);
+        X.sq[t] = (float)f;
+        double f2 = f;
+        for (int e = 0; e < EXACT_EXT; e++) f2 += (double)X.qe[t][e];
+        X.sq2[t] = (float)f2;
+        X.sa[t] = (X.part[t][0] + X.part[t][1]) + (X.part[t][2] + X.part[t][3]);
+    }
+    __syncthreads();
+    const int i16 = lane & 15, kk = lane >> 4;
+    const float m2f = 2.0f * (float)m, m2f2 = 2.0f * (float)m2;
+    // (K + 4) u per term count K of the single-precision products (u = 2^-24, with a factor
+    // 1.25 of room)
+    const float ek = m2f * 1.25f * (float)(K4 + 4) * 5.9604645e-8f;
+    const float ek2 = m2f2 * 1.25f * (float)(K4 + 4 + EXACT_EXT) * 5.9604645e-8f;
+    const float sqm = sqrtf((float)m), sqm2 = sqrtf((float)m2);
+    // the conversion's rounding at the largest distance, 4m (conv_ed2)
+    const float cv1 = (float)conv_ed2(m, 4.0 * m), cv2 = (float)conv_ed2(m2, 4.0 * m2);
+    const int64_t cbase = cblk;  // columns relative to the chunk, candidates too (X.rel)
+    float lbm[NU], lb2[NU], hib[NU];
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+        lbm[u] = INFINITY;
+        lb2[u] = INFINITY;
+        hib[u] = INFINITY;
+    }
+    for (int sct = 0; sct < 4; sct++) {
+        const int cl = 256 * (w + 4 * sct);
+        const int64_t c = cblk + cl;
+        if (c >= N) break;
+        f4 acc0[NU], acc1[NU];
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+            acc0[u] = f4{0.f, 0.f, 0.f, 0.f};
+            acc1[u] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+        const float *ap = xs + nn_pad(cl + 16 * i16) + kk;
+        const float refi = xs[nn_pad(cl + 16 * i16)];
+        const float *bp = qz + 16 + kk - i16;
+        int k0 = 0;
+        for (; k0 + 16 <= K4; k0 += 16, ap += 17, bp += 16) {
+            const float a0 = ap[0] - refi, a1 = ap[4] - refi, a2 = ap[8] - refi,
+                        a3 = ap[12] - refi;
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+                const float *b = bp + u * QZ;
+                acc0[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b[0], acc0[u], 0, 0, 0);
+                acc1[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b[4], acc1[u], 0, 0, 0);
+                acc0[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b[8], acc0[u], 0, 0, 0);
+                acc1[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, b[12], acc1[u], 0, 0, 0);
+            }
+        }
+        for (int e = 0; k0 < K4; k0 += 4, e += 4) {
+            const float a0 = ap[e] - refi;
+#pragma unroll
+            for (int u = 0; u < NU; u++)
+                acc0[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bp[u * QZ + e], acc0[u], 0, 0, 0);
+        }
+        float dmin[NU];
+#pragma unroll
+        for (int u = 0; u < NU; u++) dmin[u] = INFINITY;
+        float emax = 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int n = 16 * (4 * kk + r) + i16;
+            const int64_t j = c + n;
+            if (j >= N) continue;
+            const float ref = xs[nn_pad(cl + 16 * (4 * kk + r))];
+            const float4 cv = E.colv[S.roff + j];
+            // |cov32 - cov| <= (K4 + 4) u |q| |y| with y = x - ref over the column's window
+            // (K4 accumulations, the rounding of q and of x - ref), |q| = sqrt(M2_i) and
+            // |y|^2 = M2_n + m (mu_n - ref)^2; in distance units 2m (K4 + 4) u f_n with
+            // f_n = |y| invn_n = sqrt(1 + m ((mu_n - ref) invn_n)^2) <= 1 + sqrt(m) |..|
+            const float tn = (cv.w - ref) * cv.z;
+            emax = fmaxf(emax, fmaf(sqm, fabsf(tn), 1.0f));
+            float mu2 = 0.f, inv2 = 0.f, xe[EXACT_EXT], en2 = 0.f;
+            if (DUAL && j < N2) {
+                const double2 s2 = E.st64n[S.roff + j];
+                mu2 = (float)s2.x;
+                inv2 = (float)s2.y;
+#pragma unroll
+                for (int e = 0; e < EXACT_EXT; e++)
+                    xe[e] = e < ext ? xs[nn_pad(cl + n + m + e)] - ref : 0.f;
+                en2 = fmaf(ek2, fmaf(sqm2, fabsf((mu2 - ref) * inv2), 1.0f), cv2);
+            }
+            const int dj = (int)(j - cbase);  // j - i = dj - (i - cbase), both within int
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+                const int di = dj - X.rel[u];  // j - i
+                const float D = acc0[u][r] + acc1[u][r];
+                if (di >= m - 1 || -di >= ml) {
+                    const float cov = fmaf(-(cv.w - ref), X.sq[u], D);
+                    const float d = (X.invn[u] > 0.f && cv.z > 0.f)
+                                        ? m2f * fmaf(-cov, X.invn[u] * cv.z, 1.0f) : m2f;
+                    dmin[u] = fminf(dmin[u], d);
+                }
+                if (DUAL && j < N2 && X.in2[u] && (di >= m2 - 1 || -di >= ml2)) {
+                    float c2 = D;
+#pragma unroll
+                    for (int e = 0; e < EXACT_EXT; e++) c2 = fmaf(X.qe[u][e], xe[e], c2);
+                    c2 = fmaf(-(mu2 - ref), X.sq2[u], c2);
+                    const float d2 = (X.invn2[u] > 0.f && inv2 > 0.f)
+                                         ? m2f2 * fmaf(-c2, X.invn2[u] * inv2, 1.0f) : m2f2;
+                    lb2[u] = fminf(lb2[u], d2 - en2);
+                }
+            }
+        }
+        // the super-tile's bound: lo_t = min d - E_t <= every column's exact distance, and
+        // min d + E_t >= the exact distance of the column attaining min d
+        const float Et = fmaf(ek, -nn_wave_min(-emax), cv1);
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+            const float dt = nn_wave_min(dmin[u]);
+            const float lo = dt - Et;  // infinite: no valid column
+            if (LB) {
+                lbm[u] = fminf(lbm[u], lo);
+                continue;
+            }
+            // the super-tile's lower bound for k_nn_fix, and the candidate's upper bound
+            hib[u] = fminf(hib[u], dt + Et);
+            if (lane == 0 && u < nU) E.tilelo[(size_t)(u0 + u) * E.tiles + (c >> 8)] = lo;
+        }
+    }
+    // per candidate: minimum over the waves, one atomic per workgroup and result (the lower
+    // bound; in the filter pass only the second length's)
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+        const double v = (double)lbm[u];
+        const double v2 = DUAL ? (double)nn_wave_min(lb2[u]) : INFINITY;
+        if (lane == 0) {
+            X.wk[u][w] = (LB && v < INFINITY) ? ord_key64(v) : ~0ull;
+            X.wk2[u][w] = v2 < INFINITY ? ord_key64(v2) : ~0ull;
+            X.hk[u][w] = ord_key(hib[u]);
+        }
+    }
+    __syncthreads();
+    if (t < nU) {
+        unsigned long long k = X.wk[t][0], k2 = X.wk2[t][0];
+        int h = X.hk[t][0];
+        for (int q = 1; q < 4; q++) {
+            k = X.wk[t][q] < k ? X.wk[t][q] : k;
+            k2 = X.wk2[t][q] < k2 ? X.wk2[t][q] : k2;
+            h = X.hk[t][q] < h ? X.hk[t][q] : h;
+        }
+        // one atomic per workgroup and result, spread over EXACT_SLOTS addresses (the
+        // workgroups' atomics on one address serialise)
+        const size_t o = (size_t)(u0 + t) * EXACT_SLOTS + (slot & (EXACT_SLOTS - 1));
+        if (LB && k != ~0ull) atomicMin(E.outkey + o, k);
+        if (DUAL && k2 != ~0ull) atomicMin(E.outkey2 + o, k2);
+        if (!LB && h < 0x7f800000) atomicMin(E.runub + o, h);
+    }
+}
+
+// Double-precision pass of k_nn (after its filter pass): wave = (candidate c, super-tiles
+// 64 tb .. 64 tb + 63 of its series).  The super-tiles whose lower bound reaches the
+// candidate's upper bound (the scans' bound, and the smallest single-precision upper bound over
+// all its columns from the filter pass) are evaluated in double precision, centred as k_exact
+// (candidate on its mean, columns offset by the super-tile's first value); the minimum over
+// them is the exact nearest-neighbour distance, every other column being farther.  Host-
+// planned: candidate c of the launch, series csid[c]; device-planned: c = s nu + u, valid
+// when u < dev_count[s].
+template <bool DEV>
+__global__ __launch_bounds__(64) void k_nn_fix(ExactArgs E)
+{
+    __shared__ double aw[MAXM];              // the candidate's window, centred on its mean
+    __shared__ float xw[256 + MAXM + 8];     // a super-tile's series window
+    const int lane = threadIdx.x;
+    const int c = (int)(blockIdx.x / E.tb), tb = (int)(blockIdx.x % E.tb);
+    int sid, m;
+    if (DEV) {
+        const int q = c / E.nu_dev, u = c - q * E.nu_dev;
+        m = E.dev_m[0];
+        if (m <= 0 || u >= E.dev_count[q]) return;
+        sid = q;
+    } else {
+        sid = E.csid[c];
+        m = E.m;
+    }
+    const SerDev S = E.ser[sid];
+    const float *x = E.x + S.xoff;
+    const double2 *st64 = E.st64 + S.roff;
+    const int64_t N = S.n - m + 1;
+    const int T = (int)((N + 255) >> 8);
+    const int t0 = tb * 64 + lane;
+    const float lo = t0 < T ? E.tilelo[(size_t)c * E.tiles + t0] : INFINITY;
+    int hk = E.runub[(size_t)c * EXACT_SLOTS + lane];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const int h2 = __shfl_xor(hk, o, 64);
+        hk = h2 < hk ? h2 : hk;
+    }
+    double ub = (double)ord_val(hk);
+    if (E.ub) ub = fmin(ub, E.ub[c]);
+    unsigned long long mask = __ballot((double)lo <= ub);
+    if (!mask) return;
+    const int64_t i = E.cand[c];
+    const double2 si = st64[i];
+    double part = 0.0;
+    for (int k = lane; k < m; k += 64) {
+        const double a = (double)x[i + k] - si.x;
+        aw[k] = a;
+        part += a;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
+    const double sa = part;
+    const int ml = E.sym ? m - 1 : m + 1;
+    double best = INFINITY;
+    while (mask) {
+        const int tt = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const int64_t c0 = (int64_t)(tb * 64 + tt) * 256;
+        __syncthreads();  // the previous super-tile's readers of xw are done
+        for (int p = lane; p < 256 + m; p += 64) xw[p] = c0 + p < S.n ? x[c0 + p] : 0.f;
+        __syncthreads();
+        const double cref = (double)xw[0];
+        double cv4[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < m; k++) {
+            const double a = aw[k];
+#pragma unroll
+            for (int h = 0; h < 4; h++) cv4[h] = fma(a, (double)xw[lane + 64 * h + k] - cref, cv4[h]);
+        }
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            const int64_t j = c0 + lane + 64 * h;
+            if (j >= N || !((j - i >= m - 1) || (i - j >= ml))) continue;
+            const double2 sj = st64[j];
+            const double cov = fma(-(sj.x - cref), sa, cv4[h]);
+            const double dd = (si.y > 0.0 && sj.y > 0.0) ? (2.0 * m) * fma(-cov, si.y * sj.y, 1.0)
+                                                         : 2.0 * m;
+            best = fmin(best, dd);
+        }
+    }
+    best = nn_wave_min(best);
+    if (lane == 0 && best < INFINITY)
+        atomicMin(E.outkey + (size_t)c * EXACT_SLOTS + (tb & (EXACT_SLOTS - 1)), ord_key64(best));
+}
+
+// Host-planned launch (as k_exact): block = (group, chunk of that group's series).
+template <int NU, bool DUAL, bool LB>
+__global__ __launch_bounds__(256) void k_nn(ExactArgs E)
+{
+    extern __shared__ char nsh[];
+    __shared__ NnShared X;
+    const int64_t item = blockIdx.x;
+    int lo = 0, hi = E.ngroups - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (E.goff[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const int u0 = E.gfirst[lo];
+    nn_item<NU, DUAL, LB>(E, E.m, E.m2, E.gser[lo], u0, E.gfirst[lo + 1] - u0,
+                          item - E.goff[lo], item, nsh, X);
+}
+
+// Device-planned launch of the length loop (as k_exact_chain).
+template <int NU, bool DUAL, bool LB>
+__global__ __launch_bounds__(256) void k_nn_chain(ExactArgs E)
+{
+    extern __shared__ char nsh[];
+    __shared__ NnShared X;
+    const int64_t total = E.goff[E.ngroups];
+    const int64_t slot = blockIdx.x / total, item = blockIdx.x - slot * total;
+    int lo = 0, hi = E.ngroups - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (E.goff[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const int gi = lo * E.gslots + (int)slot;
+    const int nU = E.dev_count[gi];
+    const int m = E.dev_m[0], m2 = E.dev_m[1];
+    if (nU <= 0 || m <= 0) return;
+    const int64_t chunk = item - E.goff[lo];
+    if (chunk * EXACT_COLS >= E.ser[lo].n - m + 1) return;
+    nn_item<NU, DUAL, LB>(E, m, m2, lo, gi * NU, nU, chunk, blockIdx.x, nsh, X);
+}
+
+template <int NU>
+static void launch_nn_u(const ExactArgs &E, bool dual, bool lb, bool dev, unsigned grid,
+                        size_t lds, hipStream_t st)
+{
+    if (dev) {
+        if (lb)
+            hipLaunchKernelGGL((k_nn_chain<NU, false, true>), dim3(grid), dim3(256), lds, st, E);
+        else if (dual)
+            hipLaunchKernelGGL((k_nn_chain<NU, true, false>), dim3(grid), dim3(256), lds, st, E);
+        else
+            hipLaunchKernelGGL((k_nn_chain<NU, false, false>), dim3(grid), dim3(256), lds, st, E);
+    } else {
+        if (lb)
+            hipLaunchKernelGGL((k_nn<NU, false, true>), dim3(grid), dim3(256), lds, st, E);
+        else if (dual)
+            hipLaunchKernelGGL((k_nn<NU, true, false>), dim3(grid), dim3(256), lds, st, E);
+        else
+            hipLaunchKernelGGL((k_nn<NU, false, false>), dim3(grid), dim3(256), lds, st, E);
+    }
+}
+
+void launch_nn_fix(const ExactArgs &E, bool dev, unsigned grid, hipStream_t st)
+{
+    if (dev)
+        hipLaunchKernelGGL((k_nn_fix<true>), dim3(grid), dim3(64), 0, st, E);
+    else
+        hipLaunchKernelGGL((k_nn_fix<false>), dim3(grid), dim3(64), 0, st, E);
+}
+
+void launch_nn(const ExactArgs &E, int nu, bool dual, bool lb, bool dev, unsigned grid,
+               size_t lds, hipStream_t st)
+{
+    switch (nu) {
+    case 1: launch_nn_u<1>(E, dual, lb, dev, grid, lds, st); break;
+    case 2: launch_nn_u<2>(E, dual, lb, dev, grid, lds, st); break;
+    case 3: launch_nn_u<3>(E, dual, lb, dev, grid, lds, st); break;
+    default: launch_nn_u<4>(E, dual, lb, dev, grid, lds, st); break;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // The length loop on the device (see ChainArgs).
 
 __device__ __forceinline__ double ord_val64(unsigned long long k)
@@ -2200,15 +2637,19 @@ __global__ __launch_bounds__(64) void k_chain_topk(ChainArgs C, int round)
             for (int u = 0; u < nreq; u++) {
                 cs.req_p[u] = s_p[s_sel[u]];
                 C.cand[(size_t)q * nu + u] = s_row[s_sel[u]];
+                C.ub[(size_t)q * nu + u] = s_hi[s_sel[u]];
             }
-            for (int u = nreq; u < nu && nreq > 0; u++)  // spare rows of the group: copies
+            for (int u = nreq; u < nu && nreq > 0; u++) {  // spare rows of the group: copies
                 C.cand[(size_t)q * nu + u] = s_row[s_sel[0]];
+                C.ub[(size_t)q * nu + u] = s_hi[s_sel[0]];
+            }
             if (nreq > 0) atomicAdd(&ctl->stats[2], (unsigned long long)nreq);
         }
         if (st != 0) nreq = 0;
         for (int e = t; e < nreq * EXACT_SLOTS; e += 64) {
             C.outkey[(size_t)q * nu * EXACT_SLOTS + e] = ~0ull;
             C.outkey2[(size_t)q * nu * EXACT_SLOTS + e] = ~0ull;
+            C.runub[(size_t)q * nu * EXACT_SLOTS + e] = 0x7fffffff;
         }
     }
     if (t == 0) C.cnt[q] = nreq;
