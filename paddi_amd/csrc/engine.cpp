@@ -1672,8 +1672,8 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ah
                     launch_exact_chain(E, nu, ahead, (unsigned)xblocks, lds_ex, stream_);
             }
             hipLaunchKernelGGL(k_chain_topk, dim3(ns), dim3(64), 0, stream_, C, rounds);
-            hipLaunchKernelGGL(k_chain_next, dim3(1), dim3(1024), 0, stream_, C);
-            HIPC(hipMemcpyAsync(pin_ch_, dctl, sizeof(ChainCtl), hipMemcpyDeviceToHost, stream_));
+            if (ns > 1)  // one series: the final k_chain_topk starts the next length itself
+                hipLaunchKernelGGL(k_chain_next, dim3(1), dim3(1024), 0, stream_, C);
             HIPC(hipGetLastError());
         } catch (...) {
             (void)hipStreamEndCapture(stream_, &g);
@@ -1716,14 +1716,18 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ah
     HIPC(hipGetLastError());
     wait();  // the staging is reused for the results
     const int every = std::max(1, device_lengths_);
+    auto poll = [&] {  // the control block, once per synchronisation
+        HIPC(hipMemcpyAsync(pin_ch_, dctl, sizeof(ChainCtl), hipMemcpyDeviceToHost, stream_));
+        wait();
+    };
     for (int j = j0; j < jend; j++) {
         HIPC(hipGraphLaunch(gl[(size_t)(j - j0)], stream_));
         if ((j - j0 + 1) % every == 0 && j + 1 < jend) {
-            wait();
+            poll();
             if (hc->halt) break;
         }
     }
-    wait();
+    poll();
     const ChainCtl cc = *hc;
     const int jstop = cc.halt ? cc.halt - 1 : jend;
     if (debug_)

@@ -2369,6 +2369,47 @@ __global__ __launch_bounds__(64) void k_chain_range(ChainArgs C)
     }
 }
 
+// Close of length j (m = cm, 0 if the loop had stopped) and start of the next, by one
+// workgroup after the length's final k_chain_topk: the pass's statistics, when it ran, then
+// chain_begin.
+__device__ __forceinline__ void chain_close(const ChainArgs &C, const int j, const int cm,
+                                            int *s_bad, unsigned long long *s_nmax)
+{
+    ChainCtl *ctl = C.ctl;
+    const int t = threadIdx.x;
+    if (cm > 0 && t < 64) {
+        const PassCtl *P = C.pctl;
+        unsigned long long a0 = P->pairs[0][t], a1 = P->pairs[1][t], a2 = P->live[0][t],
+                           a3 = P->live[1][t];
+        for (int w = 32; w >= 1; w >>= 1) {
+            a0 += __shfl_xor(a0, w, 64);
+            a1 += __shfl_xor(a1, w, 64);
+            a2 += __shfl_xor(a2, w, 64);
+            a3 += __shfl_xor(a3, w, 64);
+        }
+        if (t == 0) {
+            ctl->stats[0] += 1;
+            ctl->stats[1] += (unsigned long long)P->survivors;
+            ctl->stats[3] += a0;
+            ctl->stats[4] += a1;
+            ctl->stats[5] += a2;
+            ctl->stats[6] += a3;
+        }
+    }
+    __syncthreads();  // every thread has read ctl->j and ctl->m; the top-k's writes are seen
+    chain_begin(C, j + 1, s_bad, s_nmax);
+}
+
+// chain_close as a kernel of its own (several series: their final k_chain_topk workgroups
+// must all have finished).
+__global__ __launch_bounds__(1024) void k_chain_next(ChainArgs C)
+{
+    __shared__ int s_bad;
+    __shared__ unsigned long long s_nmax;
+    const int j = C.ctl->j, cm = C.ctl->m;
+    chain_close(C, j, cm, &s_bad, &s_nmax);
+}
+
 // A candidate of the device top-k: its upper bound hi, local row i and position p in the
 // workgroup's list; the larger hi ranks first, ties by lower row (TopK::advance's T).
 struct TkKey {
@@ -2421,30 +2462,45 @@ __global__ __launch_bounds__(64) void k_chain_topk(ChainArgs C, int round)
     __shared__ double s_lo[CHAIN_SCAP], s_hi[CHAIN_SCAP], s_v[CHAIN_SCAP];
     __shared__ unsigned char s_gone[CHAIN_SCAP];
     __shared__ int s_n, s_sel[CHAIN_NU], s_anchor[CHAIN_NU + 1], s_acc[CHAIN_KMAX];
+    __shared__ int s_cb_bad;  // chain_close's
+    __shared__ unsigned long long s_cb_nmax;
     const int q = blockIdx.x, t = threadIdx.x;
     ChainCtl *ctl = C.ctl;
     ChainSer &cs = C.cs[q];
-    const int j = ctl->j;
-    const int m = C.minL + j * C.step;
     const int nu = C.nu;
+    // Everything the round reads from global memory that depends on nothing else, issued
+    // together (a few dependent round trips otherwise make this small kernel slow): the
+    // loop's and the series' state, the previous round's result slots (lane t: slot t of
+    // every candidate), and the first 64 survivors with their exact values.
+    const int j = ctl->j, j0 = ctl->j0, cm = ctl->m, halt = ctl->halt;
+    const int status = cs.status, pnreq = cs.nreq;
+    const double r2 = cs.r2;
+    const SerDev S = C.ser[q];
+    const int64_t nsv = C.pctl->survivors;
+    unsigned long long ka[CHAIN_NU], kb[CHAIN_NU];
+    int rq[CHAIN_NU];
+#pragma unroll
+    for (int u = 0; u < CHAIN_NU; u++) {
+        const size_t o = ((size_t)q * nu + (u < nu ? u : 0)) * EXACT_SLOTS + t;
+        ka[u] = C.outkey[o];
+        kb[u] = C.outkey2[o];
+        rq[u] = cs.req_p[u];
+    }
+    Surv sv0{};
+    double xv0 = 0.0;
+    if (t < C.scap) {  // scap <= the buffers' rows
+        sv0 = C.surv[t];
+        xv0 = C.xv[t];
+    }
+    const int m = C.minL + j * C.step;
     // lane 0's reading for the whole wave: another series may stop the loop meanwhile
-    const int run = __shfl(t == 0 ? (ctl->m > 0 && ctl->halt == 0 && cs.status == 0) : 0, 0, 64);
+    const int run = __shfl(cm > 0 && halt == 0 && status == 0 ? 1 : 0, 0, 64);
     int nreq = 0;
     if (run) {
-        const double r2 = cs.r2;
-        const SerDev S = C.ser[q];
         const int64_t roff = S.roff, rend = roff + series_rows(S.n, C.minL);
-        const int64_t nsv = C.pctl->survivors;
-        // fold the previous round's results: lane t reads slot t of every candidate (the
-        // loads issued together), then the minima over the wave
-        const int pr = round > 0 ? cs.nreq : 0;
-        unsigned long long ka[CHAIN_NU], kb[CHAIN_NU];
-#pragma unroll
-        for (int u = 0; u < CHAIN_NU; u++) {
-            const size_t o = ((size_t)q * nu + u) * EXACT_SLOTS + t;
-            ka[u] = u < pr ? C.outkey[o] : ~0ull;
-            kb[u] = u < pr ? C.outkey2[o] : ~0ull;
-        }
+        // fold the previous round's results: the minimum over the slots
+        const int pr = round > 0 ? pnreq : 0;
+        double fold[CHAIN_NU];
 #pragma unroll
         for (int u = 0; u < CHAIN_NU; u++) {
             for (int w = 32; w >= 1; w >>= 1) {
@@ -2453,24 +2509,29 @@ __global__ __launch_bounds__(64) void k_chain_topk(ChainArgs C, int round)
                 ka[u] = a2 < ka[u] ? a2 : ka[u];
                 kb[u] = b2 < kb[u] ? b2 : kb[u];
             }
+            fold[u] = ord_val64(ka[u]);
             if (t == 0 && u < pr) {
-                C.xv[cs.req_p[u]] = ord_val64(ka[u]);
-                C.xv2[cs.req_p[u]] = ord_val64(kb[u]);
+                C.xv[rq[u]] = fold[u];
+                C.xv2[rq[u]] = ord_val64(kb[u]);
             }
         }
         if (t == 0) s_n = 0;
         __syncthreads();
         const int64_t ns = nsv < C.scap ? nsv : C.scap;
         for (int64_t p = t; p < ns; p += 64) {
-            const Surv sv = C.surv[p];
+            const Surv sv = p == t ? sv0 : C.surv[p];
             if (sv.g < roff || sv.g >= rend) continue;
             const int e = atomicAdd(&s_n, 1);
             if (e >= CHAIN_SCAP) continue;
             double v = __longlong_as_double(0x7ff8000000000000ll);  // NaN: not refined
-            if (round == 0)
+            if (round == 0) {
                 C.xv[p] = v;
-            else
-                v = C.xv[p];
+            } else {
+                v = p == t ? xv0 : C.xv[p];
+#pragma unroll
+                for (int u = 0; u < CHAIN_NU; u++)  // refined by the previous round: folded here
+                    if (u < pr && rq[u] == (int)p) v = fold[u];
+            }
             s_p[e] = (int)p;
             s_row[e] = (int)(sv.g - roff);
             s_v[e] = v;
@@ -2584,11 +2645,15 @@ __global__ __launch_bounds__(64) void k_chain_topk(ChainArgs C, int round)
                     ib = i;
                 }
             }
-            for (int l = 1; l < 64; l++) {
+            // only the lanes holding a member of U take part (usually a few): their bits
+            unsigned long long act = __ballot(B >= 0) & ~1ull;
+            while (act) {
+                const int l = __ffsll((long long)act) - 1;
+                act &= act - 1;
                 const int Bl = __shfl(B, l, 64);
                 const double vl = __shfl(vb, l, 64);
                 const int il = __shfl(ib, l, 64);
-                if (t != 0 || Bl < 0) continue;
+                if (t != 0) continue;
                 bool before = B < 0;
                 if (!before) {
                     const double tol = TIE_REL * fmax(fabs(vl), fabs(vb));
@@ -2621,7 +2686,7 @@ __global__ __launch_bounds__(64) void k_chain_topk(ChainArgs C, int round)
         int st = status == 3 ? 3 : (nreq > 0 ? 0 : ((nacc >= C.k || r2 <= 0.0) ? 1 : 2));
         if (t == 0) {
             if (st == 1) {
-                ChainRes &R = C.res[(size_t)(j - ctl->j0) * C.nser + q];
+                ChainRes &R = C.res[(size_t)(j - j0) * C.nser + q];
                 R.n = nacc;
                 R.r2 = r2;
                 for (int a = 0; a < nacc; a++) {
@@ -2653,38 +2718,8 @@ __global__ __launch_bounds__(64) void k_chain_topk(ChainArgs C, int round)
         }
     }
     if (t == 0) C.cnt[q] = nreq;
-}
-
-// Close of a length and start of the next (one workgroup, after the length's final
-// k_chain_topk): the pass's statistics, when it ran, then chain_begin.
-__global__ __launch_bounds__(1024) void k_chain_next(ChainArgs C)
-{
-    __shared__ int s_bad;
-    __shared__ unsigned long long s_nmax;
-    ChainCtl *ctl = C.ctl;
-    const int t = threadIdx.x;
-    const int j = ctl->j;
-    if (ctl->m > 0 && t < 64) {
-        const PassCtl *P = C.pctl;
-        unsigned long long a0 = P->pairs[0][t], a1 = P->pairs[1][t], a2 = P->live[0][t],
-                           a3 = P->live[1][t];
-        for (int w = 32; w >= 1; w >>= 1) {
-            a0 += __shfl_xor(a0, w, 64);
-            a1 += __shfl_xor(a1, w, 64);
-            a2 += __shfl_xor(a2, w, 64);
-            a3 += __shfl_xor(a3, w, 64);
-        }
-        if (t == 0) {
-            ctl->stats[0] += 1;
-            ctl->stats[1] += (unsigned long long)P->survivors;
-            ctl->stats[3] += a0;
-            ctl->stats[4] += a1;
-            ctl->stats[5] += a2;
-            ctl->stats[6] += a3;
-        }
-    }
-    __syncthreads();  // every thread has read ctl->j and ctl->m
-    chain_begin(C, j + 1, &s_bad, &s_nmax);
+    // one series: its final round is the length's last work, so it starts the next length
+    if (round == C.rounds && C.nser == 1) chain_close(C, j, cm, &s_cb_bad, &s_cb_nmax);
 }
 
 template <bool DUAL>
