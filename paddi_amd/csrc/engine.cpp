@@ -1050,7 +1050,7 @@ void Engine::exact(Batch &B, int m, const std::vector<std::pair<int, int64_t>> &
         F.tiles = (int)tiles;
         F.tb = (int)((tiles + 63) / 64);
         F.csid = (const int *)(dx + o_csid);
-        launch_nn_fix(F, false, (unsigned)(nc * (size_t)nn_fix_groups(F.tb)), stream_);
+        launch_nn_fix(F, false, (unsigned)(nc * (size_t)F.tb), stream_);
         HIPC(hipGetLastError());
     }
     HIPC(hipEventRecord(ev_[7], stream_));
@@ -1666,7 +1666,7 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ah
                 if (xblocks > 0 && nn) {
                     launch_nn(E, nu, ahead, false, true, (unsigned)xblocks,
                               nn_lds_bytes(cls, nu), stream_);
-                    launch_nn_fix(E, true, (unsigned)(ns * nu * nn_fix_groups(E.tb)), stream_);
+                    launch_nn_fix(E, true, (unsigned)(ns * nu * E.tb), stream_);
                 }
                 else if (xblocks > 0)
                     launch_exact_chain(E, nu, ahead, (unsigned)xblocks, lds_ex, stream_);

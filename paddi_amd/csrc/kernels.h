@@ -227,10 +227,7 @@ void launch_exact_chain(const ExactArgs &E, int nu, bool dual, unsigned grid, si
 constexpr int NN_MAXU = 4;
 void launch_nn(const ExactArgs &E, int nu, bool dual, bool lb, bool dev, unsigned grid,
                size_t lds, hipStream_t st);
-// k_nn_fix: the double-precision pass after k_nn's filter pass; a workgroup takes FIX_TB
-// blocks of 64 super-tiles of one candidate (grid: candidates x nn_fix_groups(tb)).
-constexpr int FIX_TB = 4;
-__host__ __device__ inline int nn_fix_groups(int tb) { return (tb + FIX_TB - 1) / FIX_TB; }
+// k_nn_fix: the double-precision pass after k_nn's filter pass (grid: candidates x tb).
 void launch_nn_fix(const ExactArgs &E, bool dev, unsigned grid, hipStream_t st);
 // Dynamic LDS of k_nn: the chunk's series window (padded every 16 floats) and per candidate
 // its single-precision window (16 zeros either side of K4 + 16).

@@ -2009,14 +2009,12 @@ __device__ __forceinline__ void nn_item(const ExactArgs &E, const int m, const i
     }
 }
 
-// Double-precision pass of k_nn (after its filter pass): wave = (candidate c, blocks of 64
-// super-tiles FIX_TB g .. FIX_TB g + FIX_TB - 1 of its series).  The super-tiles whose lower
-// bound reaches the candidate's upper bound (the scans' bound, and the smallest single-
-// precision upper bound over all its columns from the filter pass) are evaluated in double
-// precision, centred as k_exact (candidate on its mean, columns offset by the super-tile's
-// first value); the minimum over them is the exact nearest-neighbour distance, every other
-// column being farther.  Most launches qualify few super-tiles: a wave takes several blocks
-// (their bounds loaded together), so that launches end after few, short workgroups.  Host-
+// Double-precision pass of k_nn (after its filter pass): wave = (candidate c, super-tiles
+// 64 tb .. 64 tb + 63 of its series).  The super-tiles whose lower bound reaches the
+// candidate's upper bound (the scans' bound, and the smallest single-precision upper bound over
+// all its columns from the filter pass) are evaluated in double precision, centred as k_exact
+// (candidate on its mean, columns offset by the super-tile's first value); the minimum over
+// them is the exact nearest-neighbour distance, every other column being farther.  Host-
 // planned: candidate c of the launch, series csid[c]; device-planned: c = s nu + u, valid
 // when u < dev_count[s].
 template <bool DEV>
@@ -2025,8 +2023,7 @@ __global__ __launch_bounds__(64) void k_nn_fix(ExactArgs E)
     __shared__ double aw[MAXM];              // the candidate's window, centred on its mean
     __shared__ float xw[256 + MAXM + 8];     // a super-tile's series window
     const int lane = threadIdx.x;
-    const int ng = nn_fix_groups(E.tb);
-    const int c = (int)(blockIdx.x / ng), g = (int)(blockIdx.x % ng);
+    const int c = (int)(blockIdx.x / E.tb), tb = (int)(blockIdx.x % E.tb);
     int sid, m;
     if (DEV) {
         const int q = c / E.nu_dev, u = c - q * E.nu_dev;
@@ -2042,12 +2039,8 @@ __global__ __launch_bounds__(64) void k_nn_fix(ExactArgs E)
     const double2 *st64 = E.st64 + S.roff;
     const int64_t N = S.n - m + 1;
     const int T = (int)((N + 255) >> 8);
-    float lo[FIX_TB];
-#pragma unroll
-    for (int h = 0; h < FIX_TB; h++) {
-        const int t0 = (g * FIX_TB + h) * 64 + lane;
-        lo[h] = t0 < T ? E.tilelo[(size_t)c * E.tiles + t0] : INFINITY;
-    }
+    const int t0 = tb * 64 + lane;
+    const float lo = t0 < T ? E.tilelo[(size_t)c * E.tiles + t0] : INFINITY;
     int hk = E.runub[(size_t)c * EXACT_SLOTS + lane];
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
@@ -2056,13 +2049,8 @@ __global__ __launch_bounds__(64) void k_nn_fix(ExactArgs E)
     }
     double ub = (double)ord_val(hk);
     if (E.ub) ub = fmin(ub, E.ub[c]);
-    unsigned long long mk[FIX_TB], any = 0;
-#pragma unroll
-    for (int h = 0; h < FIX_TB; h++) {
-        mk[h] = __ballot((double)lo[h] <= ub);
-        any |= mk[h];
-    }
-    if (!any) return;
+    unsigned long long mask = __ballot((double)lo <= ub);
+    if (!mask) return;
     const int64_t i = E.cand[c];
     const double2 si = st64[i];
     double part = 0.0;
@@ -2076,40 +2064,34 @@ __global__ __launch_bounds__(64) void k_nn_fix(ExactArgs E)
     const double sa = part;
     const int ml = E.sym ? m - 1 : m + 1;
     double best = INFINITY;
-    for (int h = 0; h < FIX_TB; h++) {
-        unsigned long long mask = mk[h];
-        const int tb = g * FIX_TB + h;
-        while (mask) {
-            const int tt = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            const int64_t c0 = (int64_t)(tb * 64 + tt) * 256;
-            __syncthreads();  // the previous super-tile's readers of xw are done
-            for (int p = lane; p < 256 + m; p += 64) xw[p] = c0 + p < S.n ? x[c0 + p] : 0.f;
-            __syncthreads();
-            const double cref = (double)xw[0];
-            double cv4[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int k = 0; k < m; k++) {
-                const double a = aw[k];
+    while (mask) {
+        const int tt = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const int64_t c0 = (int64_t)(tb * 64 + tt) * 256;
+        __syncthreads();  // the previous super-tile's readers of xw are done
+        for (int p = lane; p < 256 + m; p += 64) xw[p] = c0 + p < S.n ? x[c0 + p] : 0.f;
+        __syncthreads();
+        const double cref = (double)xw[0];
+        double cv4[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < m; k++) {
+            const double a = aw[k];
 #pragma unroll
-                for (int e = 0; e < 4; e++)
-                    cv4[e] = fma(a, (double)xw[lane + 64 * e + k] - cref, cv4[e]);
-            }
+            for (int h = 0; h < 4; h++) cv4[h] = fma(a, (double)xw[lane + 64 * h + k] - cref, cv4[h]);
+        }
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int64_t j = c0 + lane + 64 * e;
-                if (j >= N || !((j - i >= m - 1) || (i - j >= ml))) continue;
-                const double2 sj = st64[j];
-                const double cov = fma(-(sj.x - cref), sa, cv4[e]);
-                const double dd = (si.y > 0.0 && sj.y > 0.0)
-                                      ? (2.0 * m) * fma(-cov, si.y * sj.y, 1.0)
-                                      : 2.0 * m;
-                best = fmin(best, dd);
-            }
+        for (int h = 0; h < 4; h++) {
+            const int64_t j = c0 + lane + 64 * h;
+            if (j >= N || !((j - i >= m - 1) || (i - j >= ml))) continue;
+            const double2 sj = st64[j];
+            const double cov = fma(-(sj.x - cref), sa, cv4[h]);
+            const double dd = (si.y > 0.0 && sj.y > 0.0) ? (2.0 * m) * fma(-cov, si.y * sj.y, 1.0)
+                                                         : 2.0 * m;
+            best = fmin(best, dd);
         }
     }
     best = nn_wave_min(best);
     if (lane == 0 && best < INFINITY)
-        atomicMin(E.outkey + (size_t)c * EXACT_SLOTS + (g & (EXACT_SLOTS - 1)), ord_key64(best));
+        atomicMin(E.outkey + (size_t)c * EXACT_SLOTS + (tb & (EXACT_SLOTS - 1)), ord_key64(best));
 }
 
 // Host-planned launch (as k_exact): block = (group, chunk of that group's series).
