@@ -170,7 +170,7 @@ def main():
     ctx = Context(0)
     handles = [ctx.upload(x) for x in series]
     if not a.no_tuned:
-        ctx.apply_tuned(w["minL"])
+        ctx.apply_tuned(w["minL"], nseries=len(w["series"]))
     tunables = ctx.get_tunables()
 
     primary = a.primary

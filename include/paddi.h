@@ -115,6 +115,8 @@ typedef struct {
                                 device_lengths lengths (0 = every length on the host) */
     int32_t device_rounds;   /* exact rounds a length's graph holds (1..4); a length needing more
                                 is finished by the host */
+    int32_t refine_rounds;   /* refine rounds per pass (3..4): cyclic offsets past the select
+                                window up to 2, 8 (and 32) x refine_chunk, then the rest */
 } paddi_tunables;
 
 const char *paddi_status_str(paddi_status s);

@@ -8,7 +8,7 @@ of its candidates; a candidate is timed as the median of `--reps` complete searc
 the sample (the workload's first `--lengths` lengths, series resident in HBM) and kept if it
 is faster.  Every configuration must return the same discord set as the starting one (the
 schedule never changes results; a mismatch aborts).  The winner is stored in
-paddi_amd/model/tuned.json under the key device/length-bucket (see engine.tuned_key), which
+paddi_amd/model/tuned.json under the key device/length-bucket[/batch] (engine.tuned_key), which
 Context.apply_tuned() and bench.py read.
 """
 from __future__ import annotations
@@ -41,6 +41,7 @@ GRID = {
     "direct_rows": [1, 2, 3, 4],
     "device_lengths": [0, 8, 32],
     "device_rounds": [1, 2, 3],
+    "refine_rounds": [3, 4],
 }
 
 
@@ -65,7 +66,8 @@ def main():
     maxL = w["maxL"] if a.lengths <= 0 else \
         min(w["maxL"], w["minL"] + (a.lengths - 1) * w["step"])
     if not a.fresh:
-        ctx.apply_tuned(w["minL"], path=a.out)  # continue from the stored entry, if any
+        # continue from the stored entry, if any
+        ctx.apply_tuned(w["minL"], path=a.out, nseries=len(w["series"]))
     base = ctx.get_tunables()
 
     def run(tun):
@@ -107,7 +109,7 @@ def main():
     if os.path.exists(a.out):
         with open(a.out) as f:
             table = json.load(f)
-    table[tuned_key(w["minL"])] = dict(
+    table[tuned_key(w["minL"], nseries=len(w["series"]))] = dict(
         tunables=best, seconds=best_t, baseline_seconds=log[0]["seconds"],
         sample=f"{a.workload}: {len(w['series'])} series, m={w['minL']}..{maxL} "
                f"step {w['step']}, top-{w['k']}, mode {a.mode}, median of {a.reps}",

@@ -510,7 +510,7 @@ void Engine::copy_st64(double2 *dst, int64_t N)
 }
 
 // The device work of one pass, in stream order: parameter upload, k_pass_init, k_scan_select,
-// then REFINE_ROUNDS k_scan_refine rounds each preceded by a listing of the live
+// then refine_rounds k_scan_refine rounds each preceded by a listing of the live
 // blocks, k_survivors, and the download of the counters and the first scap survivors.  No
 // step depends on a value read back by the host, so the sequence is captured once per
 // LDS class and replayed.  Eager launches are bracketed by events for the statistics;
@@ -537,7 +537,7 @@ void Engine::enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t ro
     hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
                        (const uint64_t *)alive_.p, nwords, blist_.as<int>(), &ctl->list[0]);
     if (timed) HIPC(hipEventRecord(ev_[2], stream_));
-    for (int r = 0; r < REFINE_ROUNDS; r++) {
+    for (int r = 0; r < refine_rounds_; r++) {
         if (r > 0)
             hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
                                (const uint64_t *)alive_.p, nwords, blist_.as<int>(),
@@ -615,9 +615,7 @@ ScanPtrs Engine::fill_pass(Batch &B, int m, const std::vector<double> &r2, bool 
     // the few blocks the near ones did not kill.
     RefineArgs &R = P.R;
     const int64_t end = refine ? nmax + A.U0 : A.U0 + A.WA;
-    const int64_t raw[4] = {A.U0 + A.WA, A.U0 + A.WA + 2 * refine_chunk_,
-                            A.U0 + A.WA + 8 * refine_chunk_, end};
-    for (int r = 0; r < 4; r++) R.bound[r] = std::min(raw[r], end);
+    refine_bounds(A.U0 + A.WA, refine_chunk_, end, refine_rounds_, R.bound);
     R.min_chunk = min_chunk_;
     R.max_chunk = refine_chunk_;
     R.items = refine_waves_;
@@ -790,7 +788,8 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
             (uintptr_t)Q.x,      (uintptr_t)Q.bser,   (uintptr_t)Q.colv,   (uintptr_t)Q.best,
             (uintptr_t)Q.wit0,   (uintptr_t)Q.wit1,   (uintptr_t)Q.besth,  (uintptr_t)Q.bestl,
             (uintptr_t)Q.tmax,   (uintptr_t)Q.rho,    (uintptr_t)rows,     (uintptr_t)nwords,
-            (uintptr_t)parb,     (uintptr_t)scap,     (uintptr_t)refine_waves_};
+            (uintptr_t)parb,     (uintptr_t)scap,     (uintptr_t)refine_waves_,
+            (uintptr_t)refine_rounds_};
         GraphEnt &G = graphs_[{0, lds}];
         if (!G.exec || G.sig != sig) {
             if (G.exec) HIPC(hipGraphExecDestroy(G.exec));
@@ -833,9 +832,10 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
             HIPC(hipEventElapsedTime(&b, ev_[2], ev_[3]));
         }
         std::fprintf(stderr,
-                     "[paddi] m=%d select %.2f ms %.3g pairs | refine blocks %d/%d/%d %.2f ms "
+                     "[paddi] m=%d select %.2f ms %.3g pairs | refine blocks %d/%d/%d/%d %.2f ms "
                      "%.3g pairs | survivors %d\n",
-                     m, a, pairs_sel, C.list[0], C.list[1], C.list[2], b, pairs_ref, ns);
+                     m, a, pairs_sel, C.list[0], C.list[1], C.list[2],
+                     refine_rounds_ > 3 ? C.list[3] : 0, b, pairs_ref, ns);
     }
     out.assign((size_t)B.nser, {});
     if (ns > 0) {
@@ -865,7 +865,7 @@ void Engine::pass(Batch &B, int m, const std::vector<double> &r2, bool witness,
             st->kernel_ms[1] += a;
             st->launches[1] += 1;
             st->kernel_ms[2] += b;
-            st->launches[2] += REFINE_ROUNDS;
+            st->launches[2] += refine_rounds_;
         }
         if (!keep) {  // a candidate scan is not an (m, r) attempt of the search
             st->attempts += 1;
@@ -1558,6 +1558,7 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ah
     C.rounds = rounds;
     C.minL = minL;
     C.step = step;
+    C.rrounds = refine_rounds_;
     C.shift = shift;
     C.gr = gr;
     C.nur = nur;
@@ -1936,7 +1937,8 @@ paddi_status paddi_get_tunables(paddi_ctx *ctx, paddi_tunables *out)
     *out = paddi_tunables{E.select_window_, E.refine_chunk_, E.min_chunk_, E.refine_waves_,
                           E.witness_tiles_, E.spec_, E.use_graph_ ? 1 : 0, E.prev_shift_,
                           E.bootstrap_ ? 1 : 0, E.bound_cands_, E.direct_budget_,
-                          E.direct_rows_, E.device_lengths_, E.device_rounds_};
+                          E.direct_rows_, E.device_lengths_, E.device_rounds_,
+                          E.refine_rounds_};
     return PADDI_OK;
 }
 
@@ -1964,7 +1966,8 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
         t->prev_shift > 4 || t->bootstrap < 0 || t->bootstrap > 1 ||
         t->bound_candidates < 0 || t->bound_candidates > 256 || t->direct_budget < 0 ||
         t->direct_rows < 0 || t->direct_rows > paddi::DIRECT_ROWS || t->device_lengths < 0 ||
-        t->device_rounds < 1 || t->device_rounds > paddi::CHAIN_ROUNDS)
+        t->device_rounds < 1 || t->device_rounds > paddi::CHAIN_ROUNDS ||
+        t->refine_rounds < 3 || t->refine_rounds > paddi::REFINE_ROUNDS_MAX)
         return PADDI_ERR_INVALID;
     paddi::Engine &E = *ctx->eng;
     E.select_window_ = (t->select_window + 63) / 64 * 64;
@@ -1981,6 +1984,7 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
     E.direct_rows_ = t->direct_rows;
     E.device_lengths_ = t->device_lengths;
     E.device_rounds_ = t->device_rounds;
+    E.refine_rounds_ = t->refine_rounds;
     return PADDI_OK;
 }
 

@@ -188,6 +188,7 @@ public:
     int device_lengths_ = 0;         // lengths on the device between host synchronisations
                                      // (0: the host runs every length)
     int device_rounds_ = 2;          // exact rounds per length on the device
+    int refine_rounds_ = 3;          // refine rounds per pass (refine_bounds)
     bool excl_sym_ = false;          // exclusion rule: |i - j| >= m - 1 instead of |i - 1 - j| >= m
     bool exact_nn_ = true;           // k_nn for exact refinement (PADDI_EXACT_NN=0: k_exact)
     bool debug_ = false;             // per-pass trace on stderr (PADDI_DEBUG=1)

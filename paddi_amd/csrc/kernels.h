@@ -62,10 +62,24 @@ struct ScanArgs {
 // Refine rounds of one pass: round r scans cyclic offsets [bound[r], bound[r+1]) of the
 // blocks still alive when it starts, in work items of CH columns, where CH puts about
 // `items` work items in the round and is clamped to [min_chunk, max_chunk] (multiple of 64).
+constexpr int REFINE_ROUNDS_MAX = 4;
 struct RefineArgs {
-    int64_t bound[4];
+    int64_t bound[REFINE_ROUNDS_MAX + 1];
     int64_t min_chunk, max_chunk, items;
 };
+// Bounds of `rounds` refine rounds (3 or 4): offsets from b0 = U0 + select window up to 2, 8
+// (and 32) times the largest chunk, the last round to the end of the longest series (offset
+// end); the bounds past the last round are end (empty rounds).
+__host__ __device__ inline void refine_bounds(int64_t b0, int64_t ch, int64_t end, int rounds,
+                                              int64_t *bound)
+{
+    int64_t span = 2 * ch;
+    bound[0] = b0 < end ? b0 : end;
+    for (int r = 1; r <= REFINE_ROUNDS_MAX; r++, span *= 4) {
+        const int64_t b = b0 + span;
+        bound[r] = (r < rounds && b < end) ? b : end;
+    }
+}
 
 // Buffers of a pass, passed by value as a kernel argument: they stay fixed while a batch
 // runs, so a captured graph can hold them, and the compiler knows they are global memory.
@@ -114,7 +128,7 @@ struct PassDev {
 // meet on one address.
 constexpr int PAIR_SLOTS = 64;
 struct PassCtl {
-    int list[4];                  // live blocks listed before refine round r
+    int list[REFINE_ROUNDS_MAX];  // live blocks listed before refine round r
     int survivors;
     int pad[3];
     unsigned long long pairs[2][PAIR_SLOTS];  // [0] k_scan_select, [1] k_scan_refine
@@ -122,7 +136,6 @@ struct PassCtl {
     // useful share of the lane-steps), right after pairs
     unsigned long long live[2][PAIR_SLOTS];
 };
-constexpr int REFINE_ROUNDS = 3;
 // k_scan_refine evaluates a block with L live rows on the matrix cores alone
 // (scan_range_direct) when L <= DIRECT_ROWS and L (m + 15) <= the budget: each row costs one
 // product of K4 / 4 ~ (m + 15) / 4 matrix-core steps per 256 columns, the recurrence's four
@@ -337,7 +350,7 @@ struct ChainArgs {
     unsigned long long *outkey, *outkey2;  // [series][nu][EXACT_SLOTS] (k_exact_chain)
     int64_t scap;              // survivors xv holds
     int64_t WA, max_chunk, min_chunk, items;   // pass scalars that stay fixed (Engine::fill_pass)
-    int witness_max, dbudget, drows, sym;
+    int witness_max, dbudget, drows, sym, rrounds;  // rrounds: refine rounds per pass
     int nser, k, nu, spec, rounds;
     int minL, step;            // length j is minL + j step
     // The range of a length: shift = 0, from the previous discords' distances at this length

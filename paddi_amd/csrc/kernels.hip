@@ -1740,26 +1740,29 @@ __global__ __launch_bounds__(256) void k_exact_chain(ExactArgs E)
 // k_nn: nearest-neighbour distance of a few candidates on the matrix cores, exact where it
 // matters (replaces k_exact; same arguments and results).
 //
-// A work item is the candidates [u0, u0 + nU) of one series against the EXACT_COLS columns of
-// a chunk, whose series window is staged in LDS once.  Each wave takes four of the chunk's
-// 256-column super-tiles and, per candidate, forms the 256 single-precision covariances of a
-// super-tile as one 16 x 16 x K4 product (v_mfma_f32_16x16x4f32 in the Toeplitz arrangement
-// of super_seeds): A[a][k'] = x[c+16a+k'] - x[c+16a], B[k'][j] = q[k'-j] with q the
-// candidate's window centred on its mean, so D[a][j] = sum_k q[k] (x[c+16a+j+k] - x[c+16a])
-// and cov = D - (mu_n - x[c+16a]) sum(q) at column n = 16a + j.  The super-tile's smallest
-// distance d_t and its error bound E_t decide whether it can hold the nearest neighbour: E_t
-// is twice the scans' single-precision bound delta(m) times the spread ratio of the columns'
-// blocks and their neighbours (largest sqrt(M2) times largest 1/sqrt(M2), from k_derive's
-// block maxima), plus the rounding of the conversion to a distance (conv_ed2).  The exact
-// distance of every column n of the super-tile is at least d_t - E_t, and the nearest
-// neighbour is no farther than UB = min(the candidate's bound from the scans, d_t' + E_t' over
-// the super-tiles seen so far by any workgroup: a running minimum in global memory).  A
-// super-tile with d_t - E_t <= UB is evaluated in double precision, the covariance centred as
-// k_exact's (candidate on its mean, columns offset by the super-tile's first value); the
-// result is the smallest of those values.  With E.lb only the lower bound min_t (d_t - E_t) is
-// formed, nothing in double precision.  DUAL: also a lower bound at the second length m2,
-// from the same product extended by the window's m2 - m extra terms (still centred on its mean
-// at m) and the columns' statistics at m2 (st64n).
+// Filter pass (k_nn / k_nn_chain).  A work item is the candidates [u0, u0 + nU) of one series
+// against the EXACT_COLS columns of a chunk, whose series window is staged in LDS once.  Each
+// wave takes four of the chunk's 256-column super-tiles and, per candidate, forms the 256
+// single-precision covariances of a super-tile as one 16 x 16 x K4 product
+// (v_mfma_f32_16x16x4f32 in the Toeplitz arrangement of super_seeds):
+// A[a][k'] = x[c+16a+k'] - x[c+16a], B[k'][j] = q[k'-j] with q the candidate's window centred
+// on its mean, so D[a][j] = sum_k q[k] (x[c+16a+j+k] - x[c+16a]) and
+// cov = D - (mu_n - x[c+16a]) sum(q) at column n = 16a + j.  The rounding of a column's
+// distance is bounded per column: |cov32 - cov| <= (K4 + 4) u |q| |y| (K4 accumulations and
+// the roundings of q and of x - ref; y the column's window less ref), which in distance units
+// is 2m (K4 + 4) u f_n, f_n = |y| invn_n <= 1 + sqrt(m) |(mu_n - ref) invn_n|, plus the
+// conversion's rounding (conv_ed2 at 4m); E_t is the largest over the super-tile's columns.
+// With d_t the super-tile's smallest single-precision distance, every column's exact distance
+// is at least lo_t = d_t - E_t, and the nearest neighbour is no farther than d_t + E_t.  The
+// filter writes lo_t per super-tile (tilelo) and the smallest d_t + E_t per workgroup (runub,
+// one of EXACT_SLOTS addresses).  k_nn_fix then evaluates in double precision exactly the
+// super-tiles with lo_t <= min(the scans' bound for the candidate, every d_t + E_t); the
+// minimum over those is the exact nearest-neighbour distance, every other column being
+// farther.  With E.lb only the lower bound min_t lo_t is formed (the shifted range, which
+// needs a bound below the value).  DUAL: also a lower bound at the second length m2, from the
+// same product extended by the window's m2 - m extra terms (still centred on its mean at m)
+// and the columns' statistics at m2 (st64n): the next length's range only needs a value at
+// most the exact one.
 struct NnShared {
     float invn[NN_MAXU], invn2[NN_MAXU], sq[NN_MAXU], sq2[NN_MAXU];
     float qe[NN_MAXU][EXACT_EXT];
@@ -2256,11 +2259,7 @@ __device__ __forceinline__ void chain_begin(const ChainArgs &C, const int j, int
         A.off = halted ? 1 : 0;
         RefineArgs &R = C.par->R;
         const int64_t end = (halted ? 0 : (int64_t)*s_nmax) + A.U0;
-        const int64_t b0 = A.U0 + A.WA, b1 = b0 + 2 * C.max_chunk, b2 = b0 + 8 * C.max_chunk;
-        R.bound[0] = b0 < end ? b0 : end;
-        R.bound[1] = b1 < end ? b1 : end;
-        R.bound[2] = b2 < end ? b2 : end;
-        R.bound[3] = end;
+        refine_bounds(A.U0 + A.WA, C.max_chunk, end, C.rrounds, R.bound);
         R.min_chunk = C.min_chunk;
         R.max_chunk = C.max_chunk;
         R.items = C.items;

@@ -68,10 +68,12 @@ class Tunables(C.Structure):
                 ("use_graph", C.c_int32), ("prev_shift", C.c_int32),
                 ("bootstrap", C.c_int32), ("bound_candidates", C.c_int32),
                 ("direct_budget", C.c_int32), ("direct_rows", C.c_int32),
-                ("device_lengths", C.c_int32), ("device_rounds", C.c_int32)]
+                ("device_lengths", C.c_int32), ("device_rounds", C.c_int32),
+                ("refine_rounds", C.c_int32)]
     NAMES = ("select_window", "refine_chunk", "min_chunk", "refine_items", "witness_tiles",
              "speculate", "use_graph", "prev_shift", "bootstrap", "bound_candidates",
-             "direct_budget", "direct_rows", "device_lengths", "device_rounds")
+             "direct_budget", "direct_rows", "device_lengths", "device_rounds",
+             "refine_rounds")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.NAMES}
@@ -80,10 +82,11 @@ class Tunables(C.Structure):
 TUNED_PATH = os.path.join(_HERE, "model", "tuned.json")
 
 
-def tuned_key(minL, arch="gfx950"):
-    """Autotuner table key: device and length bucket (m < 128, < 256, < 512, >= 512)."""
+def tuned_key(minL, arch="gfx950", nseries=1):
+    """Autotuner table key: device, length bucket (m < 128, < 256, < 512, >= 512) and, for
+    calls of several series, "/batch" (one long series and many short ones scan differently)."""
     b = 64 if minL < 128 else 128 if minL < 256 else 256 if minL < 512 else 512
-    return f"{arch}/m{b}"
+    return f"{arch}/m{b}" + ("/batch" if nseries > 1 else "")
 
 
 EXPORTS = ("paddi_get_tunables", "paddi_set_tunables", "paddi_status_str", "paddi_version", "paddi_create", "paddi_destroy",
@@ -243,14 +246,16 @@ class Context:
         self._check(self._L.paddi_get_exclusion(self._c, C.byref(v)), "paddi_get_exclusion")
         return {c: n for n, c in self.EXCLUSION.items()}[v.value]
 
-    def apply_tuned(self, minL, path=TUNED_PATH):
-        """Apply the autotuner's entry for this length range, if the table has one."""
+    def apply_tuned(self, minL, path=TUNED_PATH, nseries=1):
+        """Apply the autotuner's entry for this length range and call shape (one series or
+        several; the other shape's entry if only that one exists), if the table has one."""
         import json
         if not os.path.exists(path):
             return None
         with open(path) as f:
             table = json.load(f)
-        ent = table.get(tuned_key(minL))
+        ent = table.get(tuned_key(minL, nseries=nseries)) or \
+            table.get(tuned_key(minL, nseries=1 if nseries > 1 else 2))
         if ent:
             self.set_tunables(**ent["tunables"])
         return ent

@@ -49,7 +49,7 @@ def test_status_strings_without_device():
 
 def test_create_fails_cleanly_without_device():
     import torch
-    if torch.cuda.is_available():
+    if torch.cuda.device_count() > 0:  # counts without initialising a device
         return
     L = engine.load_library()
     p = ctypes.c_void_p()
