@@ -159,7 +159,7 @@ def test_learned_first_length_evaluates_fewer_pairs(ctx, name):
     8.4M-point series; c4: 65 lengths of one 2M-point series; c5: a seeded sample of 128 of its
     uneven series): the candidate blocks' exact top-k fixes the first range just below d_k, so
     the whole search evaluates fewer pairs than the probe descent from 2m (on c3 and c4 about
-    half at the first length, 4 % over all lengths), with the identical discord set."""
+    half at the first length, 1-4 % over all lengths), with the identical discord set."""
     w = _workload(name)["w"]
     series = w["series"] if name != "c5" else [w["series"][q] for q in _c5_sample(128, 9)]
     saved = ctx.get_tunables()
