@@ -296,6 +296,12 @@ def main():
                      "traffic": traffic, "avg_launch_ms": avg_ms, "launches": nl[dom],
                      "bytes_per_launch": bytes_per_launch},
         "tunables": tunables,
+        # where the length loop runs: on the device, every length after the first one
+        # replayed hipGraph (top-k and next range computed there), or on the host
+        "length_loop": ({"where": "device", "graph_per_length": True,
+                         "host_sync_every_lengths": tunables["device_lengths"]}
+                        if primary != "plain" and tunables.get("device_lengths", 0) > 0
+                        else {"where": "host", "graph_per_pass": bool(tunables.get("use_graph"))}),
         "valu": {"pairs_per_step": pairs, "ops_per_pair": OPS_PER_PAIR,
                  "frac_of_peak": valu, "live_row_share": useful,
                  "scan_pairs_per_step": {k: sum(s["scan_pairs"][k] for s in kstats) / len(kstats)
