@@ -117,6 +117,8 @@ typedef struct {
                                 is finished by the host */
     int32_t refine_rounds;   /* refine rounds per pass (3..4): cyclic offsets past the select
                                 window up to 2, 8 (and 32) x refine_chunk, then the rest */
+    int32_t seed_witnesses;  /* batched and learned forms, a call's first length: scan first the
+                                tile of a row with the same SAX word (0/1) */
 } paddi_tunables;
 
 const char *paddi_status_str(paddi_status s);

@@ -42,6 +42,7 @@ GRID = {
     "device_lengths": [0, 8, 32],
     "device_rounds": [1, 2, 3],
     "refine_rounds": [3, 4],
+    "seed_witnesses": [0, 1],
 }
 
 

@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpaddi_hip.so")
 
-HIP_SOURCES = ["kernels.hip", "mlp.hip", "scratch.hip", "engine.cpp", "learned.cpp"]  # the library
+HIP_SOURCES = ["kernels.hip", "mlp.hip", "scratch.hip", "sax.hip", "engine.cpp",
+               "learned.cpp"]  # the library
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall",
     # packed FP32 VALU instructions (v_pk_*_f32) are disabled: some of their forms have given

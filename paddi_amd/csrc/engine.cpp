@@ -1275,6 +1275,10 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
         const int m_next = (ahead && j + 1 < nlen) ? m + step : 0;
         att_mark(st);
         derive(B, m, st, m_next);
+        if (!wread_valid_ && seed_wit_ && m >= 8) {
+            seed_witnesses(B, m);
+            wread_valid_ = true;
+        }
         std::vector<double> r2((size_t)ns, 0.0);
         std::vector<char> from_prev((size_t)ns, 0);
         // r from the previous length's discords, evaluated exactly at this length.  A
@@ -1432,6 +1436,25 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
         witcur_ ^= 1;
     }
     for (int q = 0; q < ns; q++) out.insert(out.end(), res[(size_t)q].begin(), res[(size_t)q].end());
+}
+
+void Engine::seed_witnesses(Batch &B, int m)
+{
+    const int64_t rows = B.rows;
+    const size_t tb = sax_sort_bytes(rows);
+    if (tb == 0) throw HipError{PADDI_ERR_DEVICE, "radix sort sizing failed"};
+    wit_[0].reserve((size_t)rows * 4);
+    wit_[1].reserve((size_t)rows * 4);
+    sax_k_.reserve((size_t)rows * 8);
+    sax_k2_.reserve((size_t)rows * 8);
+    sax_v_.reserve((size_t)rows * 4);
+    sax_v2_.reserve((size_t)rows * 4);
+    sax_tmp_.reserve(tb);
+    HIPC(launch_sax_witnesses(B.x.as<float>(), B.ser.as<SerDev>(), B.bser.as<int>(),
+                              colv_.as<float4>(), rows, m, B.nser,
+                              sax_k_.as<unsigned long long>(), sax_k2_.as<unsigned long long>(),
+                              sax_v_.as<int>(), sax_v2_.as<int>(), sax_tmp_.p, sax_tmp_.bytes,
+                              wit_[witcur_ ^ 1].as<int>(), stream_));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1938,7 +1961,7 @@ paddi_status paddi_get_tunables(paddi_ctx *ctx, paddi_tunables *out)
                           E.witness_tiles_, E.spec_, E.use_graph_ ? 1 : 0, E.prev_shift_,
                           E.bootstrap_ ? 1 : 0, E.bound_cands_, E.direct_budget_,
                           E.direct_rows_, E.device_lengths_, E.device_rounds_,
-                          E.refine_rounds_};
+                          E.refine_rounds_, E.seed_wit_ ? 1 : 0};
     return PADDI_OK;
 }
 
@@ -1967,7 +1990,8 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
         t->bound_candidates < 0 || t->bound_candidates > 256 || t->direct_budget < 0 ||
         t->direct_rows < 0 || t->direct_rows > paddi::DIRECT_ROWS || t->device_lengths < 0 ||
         t->device_rounds < 1 || t->device_rounds > paddi::CHAIN_ROUNDS ||
-        t->refine_rounds < 3 || t->refine_rounds > paddi::REFINE_ROUNDS_MAX)
+        t->refine_rounds < 3 || t->refine_rounds > paddi::REFINE_ROUNDS_MAX ||
+        t->seed_witnesses < 0 || t->seed_witnesses > 1)
         return PADDI_ERR_INVALID;
     paddi::Engine &E = *ctx->eng;
     E.select_window_ = (t->select_window + 63) / 64 * 64;
@@ -1985,6 +2009,7 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
     E.device_lengths_ = t->device_lengths;
     E.device_rounds_ = t->device_rounds;
     E.refine_rounds_ = t->refine_rounds;
+    E.seed_wit_ = t->seed_witnesses != 0;
     return PADDI_OK;
 }
 

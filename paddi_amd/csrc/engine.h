@@ -189,6 +189,7 @@ public:
                                      // (0: the host runs every length)
     int device_rounds_ = 2;          // exact rounds per length on the device
     int refine_rounds_ = 3;          // refine rounds per pass (refine_bounds)
+    bool seed_wit_ = true;           // fast forms, first length: witnesses from SAX words
     bool excl_sym_ = false;          // exclusion rule: |i - j| >= m - 1 instead of |i - 1 - j| >= m
     bool exact_nn_ = true;           // k_nn for exact refinement (PADDI_EXACT_NN=0: k_exact)
     bool debug_ = false;             // per-pass trace on stderr (PADDI_DEBUG=1)
@@ -206,6 +207,9 @@ private:
                std::vector<paddi_discord> &out, paddi_stats *st);
     void fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opts &o,
               std::vector<paddi_discord> &out, paddi_stats *st);
+    // The read side of the witnesses (wit_[witcur_ ^ 1]) from SAX words at m (derive(m)
+    // current), for a call's first length (sax.hip).
+    void seed_witnesses(Batch &B, int m);
     // Lengths j0 .. jend-1 of fast() on the device (ChainArgs); returns the first one the
     // device did not finish (jend when all), whose discords and next-length values are
     // then in prev / prev_v, and its results appended to res.
@@ -279,6 +283,7 @@ private:
     hipEvent_t ser_ev_ = nullptr;
     hipEvent_t sync_ev_ = nullptr;  // wait(): completion of the work queued so far
     DevBuf colv_, st64_, alive_, best_, besth_, bestl_, wit_[2], blist_, par_, pout_;
+    DevBuf sax_k_, sax_k2_, sax_v_, sax_v2_, sax_tmp_;  // seed_witnesses' sort
     DevBuf tmax_, rho_;            // per global block: row scales, tile column factors
     DevBuf st64n_;                 // (mu, invn) at the last derive's second length
     int derive_m2_ = 0;            // that length (0: none)

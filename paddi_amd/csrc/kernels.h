@@ -416,4 +416,14 @@ inline int exact_max_u(int m)
     return u;
 }
 
+// First-length witnesses from SAX words (sax.hip): per global row the tile of a row of its
+// series with the same SAX word at least m + 1 away (-1: none), into wit.  Scratch: keys and
+// keys2 (rows x 8 B), vals and vals2 (rows x 4 B), tmp (sax_sort_bytes).
+size_t sax_sort_bytes(int64_t rows);
+hipError_t launch_sax_witnesses(const float *x, const SerDev *ser, const int *bser,
+                                const float4 *colv, int64_t rows, int m, int nser,
+                                unsigned long long *keys, unsigned long long *keys2, int *vals,
+                                int *vals2, void *tmp, size_t tmp_bytes, int *wit,
+                                hipStream_t st);
+
 }  // namespace paddi

@@ -31,6 +31,8 @@ SCHEDULES = [
     dict(direct_budget=1 << 20, direct_rows=4, refine_items=4096),
     dict(refine_rounds=4),
     dict(refine_rounds=4, refine_chunk=8192, min_chunk=512, use_graph=1),
+    dict(seed_witnesses=0),
+    dict(seed_witnesses=1, witness_tiles=48, select_window=256),
 ]
 
 
@@ -50,7 +52,8 @@ def test_tunables_roundtrip_and_validation(ctx, restore):
     for bad in (dict(select_window=0), dict(min_chunk=1 << 20, refine_chunk=1 << 16),
                 dict(witness_tiles=-1), dict(witness_tiles=65), dict(speculate=0),
                 dict(prev_shift=5), dict(bootstrap=2), dict(direct_budget=-1),
-                dict(direct_rows=5), dict(refine_rounds=2), dict(refine_rounds=5)):
+                dict(direct_rows=5), dict(refine_rounds=2), dict(refine_rounds=5),
+                dict(seed_witnesses=2)):
         with pytest.raises(PaddiError):
             ctx.set_tunables(**bad)
         assert ctx.get_tunables() == u
