@@ -1275,7 +1275,7 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
         const int m_next = (ahead && j + 1 < nlen) ? m + step : 0;
         att_mark(st);
         derive(B, m, st, m_next);
-        if (!wread_valid_ && seed_wit_ && m >= 8) {
+        if (!wread_valid_ && seed_wit_ && m >= 8 && B.rows >= SAX_MIN_ROWS) {
             seed_witnesses(B, m);
             wread_valid_ = true;
         }

@@ -54,6 +54,9 @@ constexpr size_t EXACT_NOALLOC_CHUNK = 4096;
 void *scratch_setup(int device);
 constexpr int64_t BOUND_WINDOW = 1024;  // columns behind the first-length candidates' bounds
 constexpr int BOUND_CANDIDATES = 32;    // default candidates per series refined for the first r
+// Calls with fewer rows skip the first length's SAX witnesses: their first pass is shorter
+// than the sort (c1, 65,536 points: 1.13 -> 1.20 ms with them)
+constexpr int64_t SAX_MIN_ROWS = 1 << 18;
 
 // Grow-only device buffer.  A borrowed buffer is a slice of memory the engine does not own
 // (the scratch array, ScratchScope): it is never freed and never grows.
