@@ -127,3 +127,23 @@ def test_next_length_range_with_large_groups(ctx, restore):
     assert [(g.index, g.length) for g in got] == [(i, m) for i, m, _ in exp]
     for g, (_, _, d) in zip(got, exp):
         assert abs(g.nn_dist2 - d) <= RTOL * d
+
+
+@pytest.mark.parametrize("mode", ["batched", "learned"])
+def test_first_length_sax_witnesses_change_nothing(ctx, restore, mode):
+    """The first length's witnesses from SAX words (calls of 2^18 rows or more) only order the
+    scans: one long series and a batch of uneven ones (whose words must stay within their
+    series) return the same records with and without them (the pairs each evaluated are
+    printed)."""
+    cases = [[synth.rw_planted(400_000, 71, 3, 96, 12.0)],
+             [synth.rw_planted(60_000 + 17_000 * q, 80 + q, 1, 64, 10.0) if q % 2 else
+              synth.ecg_like(50_000 + 13_000 * q, 80 + q, 1, 1) for q in range(6)]]
+    for series in cases:
+        out = {}
+        for sw in (0, 1):
+            ctx.set_tunables(seed_witnesses=sw)
+            recs, st = ctx.discords_batch(series, 64, 68, 1, 2, mode=mode, with_stats=True)
+            out[sw] = ([(r.series, r.index, r.length, r.nn_dist2, r.range2) for r in recs], st)
+        assert out[1][0] == out[0][0]
+        print(f"{len(series)} series: pairs without {out[0][1]['pairs']:.4g}, "
+              f"with {out[1][1]['pairs']:.4g}", flush=True)
