@@ -234,8 +234,9 @@ bool TopK::advance(std::vector<size_t> &need)
             }
             // Speculate: refine in the same launch the candidates likely to be accepted next,
             // up to `spec` per round: by descending upper bound, skipping any within m of T or
-            // of one already taken (those would be dropped as trivial matches).
-            if ((int)need.size() < spec) {
+            // of one already taken (those would be dropped as trivial matches).  Once U is
+            // exact its best is accepted, so with one acceptance left there is no next.
+            if ((int)need.size() < spec && (int)acc.size() + 1 < k) {
                 std::vector<char> in(cands.size(), 0);
                 for (size_t q : need) in[q] = 1;
                 std::vector<size_t> extra;

@@ -2590,7 +2590,8 @@ __global__ __launch_bounds__(64) void k_chain_topk(ChainArgs C, int round)
                 __syncthreads();
                 for (int r = 0; r < want; r++) {
                     const bool spec = r >= take;
-                    if (spec && nneed >= C.spec) break;  // TopK::advance speculates below spec only
+                    // TopK::advance speculates below spec only, and not for the last acceptance
+                    if (spec && (nneed >= C.spec || nacc + 1 >= C.k)) break;
                     TkKey b{-INFINITY, 0, -1};
                     for (int e = t; e < n; e += 64) {
                         if (s_gone[e] || !isnan(s_v[e])) continue;
