@@ -618,6 +618,7 @@ ScanPtrs Engine::fill_pass(Batch &B, int m, const std::vector<double> &r2, bool 
     const int64_t end = refine ? nmax + A.U0 : A.U0 + A.WA;
     refine_bounds(A.U0 + A.WA, refine_chunk_, end, refine_rounds_, R.bound);
     R.min_chunk = min_chunk_;
+    R.order = refine_order_;
     R.max_chunk = refine_chunk_;
     R.items = refine_waves_;
     return Q;
@@ -1583,6 +1584,7 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ah
     C.minL = minL;
     C.step = step;
     C.rrounds = refine_rounds_;
+    C.rorder = refine_order_;
     C.shift = shift;
     C.gr = gr;
     C.nur = nur;
@@ -1962,7 +1964,7 @@ paddi_status paddi_get_tunables(paddi_ctx *ctx, paddi_tunables *out)
                           E.witness_tiles_, E.spec_, E.use_graph_ ? 1 : 0, E.prev_shift_,
                           E.bootstrap_ ? 1 : 0, E.bound_cands_, E.direct_budget_,
                           E.direct_rows_, E.device_lengths_, E.device_rounds_,
-                          E.refine_rounds_, E.seed_wit_ ? 1 : 0};
+                          E.refine_rounds_, E.seed_wit_ ? 1 : 0, E.refine_order_};
     return PADDI_OK;
 }
 
@@ -1992,7 +1994,8 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
         t->direct_rows < 0 || t->direct_rows > paddi::DIRECT_ROWS || t->device_lengths < 0 ||
         t->device_rounds < 1 || t->device_rounds > paddi::CHAIN_ROUNDS ||
         t->refine_rounds < 3 || t->refine_rounds > paddi::REFINE_ROUNDS_MAX ||
-        t->seed_witnesses < 0 || t->seed_witnesses > 1)
+        t->seed_witnesses < 0 || t->seed_witnesses > 1 || t->refine_order < 0 ||
+        t->refine_order > 1)
         return PADDI_ERR_INVALID;
     paddi::Engine &E = *ctx->eng;
     E.select_window_ = (t->select_window + 63) / 64 * 64;
@@ -2011,6 +2014,7 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
     E.device_rounds_ = t->device_rounds;
     E.refine_rounds_ = t->refine_rounds;
     E.seed_wit_ = t->seed_witnesses != 0;
+    E.refine_order_ = t->refine_order;
     return PADDI_OK;
 }
 

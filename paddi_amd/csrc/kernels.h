@@ -66,6 +66,7 @@ constexpr int REFINE_ROUNDS_MAX = 4;
 struct RefineArgs {
     int64_t bound[REFINE_ROUNDS_MAX + 1];
     int64_t min_chunk, max_chunk, items;
+    int64_t order;  // work item order: 0 block-major, 1 chunk-major (k_scan_refine)
 };
 // Bounds of `rounds` refine rounds (3 or 4): offsets from b0 = U0 + select window up to 2, 8
 // (and 32) times the largest chunk, the last round to the end of the longest series (offset
@@ -351,6 +352,7 @@ struct ChainArgs {
     int64_t scap;              // survivors xv holds
     int64_t WA, max_chunk, min_chunk, items;   // pass scalars that stay fixed (Engine::fill_pass)
     int witness_max, dbudget, drows, sym, rrounds;  // rrounds: refine rounds per pass
+    int rorder;                // RefineArgs::order
     int nser, k, nu, spec, rounds;
     int minL, step;            // length j is minL + j step
     // The range of a length: shift = 0, from the previous discords' distances at this length

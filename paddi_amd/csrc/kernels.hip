@@ -1214,8 +1214,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     if (item >= nblk * nchunks) return;
     ScanArgs A = load_args(Q, P);
     A.CH = ch;
-    const int64_t gb = blist[item / nchunks];
-    const int64_t u0 = ub + (item % nchunks) * ch;
+    // block-major (a block's chunks dispatched together), or chunk-major (every block's
+    // chunk 0, then every block's chunk 1 ...: workgroups are dispatched in order, so a
+    // block's nearer chunks have mostly finished and pruned their rows before its farther
+    // ones start)
+    const int64_t bi = RA.order ? item % nblk : item / nchunks;
+    const int64_t gb = blist[bi];
+    const int64_t u0 = ub + (RA.order ? item / nblk : item % nchunks) * ch;
     const int64_t u1 = (u0 + ch < ue) ? u0 + ch : ue;
     refine_item(A, gb, u0, u1);
 }
@@ -2261,6 +2266,7 @@ __device__ __forceinline__ void chain_begin(const ChainArgs &C, const int j, int
         const int64_t end = (halted ? 0 : (int64_t)*s_nmax) + A.U0;
         refine_bounds(A.U0 + A.WA, C.max_chunk, end, C.rrounds, R.bound);
         R.min_chunk = C.min_chunk;
+        R.order = C.rorder;
         R.max_chunk = C.max_chunk;
         R.items = C.items;
         ctl->j = j;

@@ -69,11 +69,12 @@ class Tunables(C.Structure):
                 ("bootstrap", C.c_int32), ("bound_candidates", C.c_int32),
                 ("direct_budget", C.c_int32), ("direct_rows", C.c_int32),
                 ("device_lengths", C.c_int32), ("device_rounds", C.c_int32),
-                ("refine_rounds", C.c_int32), ("seed_witnesses", C.c_int32)]
+                ("refine_rounds", C.c_int32), ("seed_witnesses", C.c_int32),
+                ("refine_order", C.c_int32)]
     NAMES = ("select_window", "refine_chunk", "min_chunk", "refine_items", "witness_tiles",
              "speculate", "use_graph", "prev_shift", "bootstrap", "bound_candidates",
              "direct_budget", "direct_rows", "device_lengths", "device_rounds",
-             "refine_rounds", "seed_witnesses")
+             "refine_rounds", "seed_witnesses", "refine_order")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.NAMES}

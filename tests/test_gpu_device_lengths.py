@@ -98,12 +98,13 @@ def test_lengths_the_device_cannot_finish_go_back_to_the_host(ctx, restore, name
 @pytest.mark.parametrize("shift", [0, 2])
 @pytest.mark.parametrize("name", ["ecg_k2_step2", "uneven_k2_step3"])
 def test_four_refine_rounds(ctx, restore, name, shift):
-    """A fourth refine round (offsets split again at 32 x refine_chunk), on the host and on the
-    device loop (whose lengths set the rounds' bounds on the device): the same results."""
+    """A fourth refine round (offsets split again at 32 x refine_chunk) with chunk-major work
+    items, on the host and on the device loop (whose lengths set the rounds' bounds and order on
+    the device): the same results."""
     ref, _ = host(ctx, name, "learned", shift)
     for every in (0, 5):
         got, _ = run(ctx, name, "learned", shift, device_lengths=every, device_rounds=2,
-                     refine_rounds=4, refine_chunk=4096, min_chunk=512)
+                     refine_rounds=4, refine_chunk=4096, min_chunk=512, refine_order=1)
         assert key(got) == key(ref), (name, shift, every)
 
 

@@ -33,6 +33,8 @@ SCHEDULES = [
     dict(refine_rounds=4, refine_chunk=8192, min_chunk=512, use_graph=1),
     dict(seed_witnesses=0),
     dict(seed_witnesses=1, witness_tiles=48, select_window=256),
+    dict(refine_order=1),
+    dict(refine_order=1, refine_items=4096, min_chunk=512, use_graph=1),
 ]
 
 
@@ -53,7 +55,7 @@ def test_tunables_roundtrip_and_validation(ctx, restore):
                 dict(witness_tiles=-1), dict(witness_tiles=65), dict(speculate=0),
                 dict(prev_shift=5), dict(bootstrap=2), dict(direct_budget=-1),
                 dict(direct_rows=5), dict(refine_rounds=2), dict(refine_rounds=5),
-                dict(seed_witnesses=2)):
+                dict(seed_witnesses=2), dict(refine_order=2)):
         with pytest.raises(PaddiError):
             ctx.set_tunables(**bad)
         assert ctx.get_tunables() == u
