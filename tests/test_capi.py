@@ -55,3 +55,24 @@ def test_create_fails_cleanly_without_device():
     p = ctypes.c_void_p()
     st = L.paddi_create(0, ctypes.byref(p))
     assert st != 0 and not p
+
+
+def _header_struct(name):
+    """Fields (type, name, count) of `typedef struct { ... } name;` in include/paddi.h, in order."""
+    txt = open(os.path.join(ROOT, "include", "paddi.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    m = re.search(r"typedef struct\s*\{([^}]*)\}\s*" + name + r"\s*;", txt)
+    assert m, name
+    return [(t, n, int(k) if k else 1) for t, n, k in
+            re.findall(r"\b(u?int(?:32|64)_t|float|double)\s+([a-z_0-9]+)\s*(?:\[(\d+)\])?\s*;",
+                       m.group(1))]
+
+
+def test_python_structs_match_the_header():
+    """The ctypes mirrors of the ABI's structs have the header's fields, order and widths (a
+    field added to the header and not to the binding would shift every field after it)."""
+    width = {"int32_t": 4, "uint32_t": 4, "int64_t": 8, "uint64_t": 8, "float": 4, "double": 8}
+    for cname, py in (("paddi_tunables", engine.Tunables), ("paddi_stats", engine.Stats)):
+        fields = _header_struct(cname)
+        assert [n for _, n, _ in fields] == [f[0] for f in py._fields_], cname
+        assert [width[t] * k for t, _, k in fields] == [ctypes.sizeof(f[1]) for f in py._fields_], cname
