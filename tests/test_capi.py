@@ -58,21 +58,38 @@ def test_create_fails_cleanly_without_device():
 
 
 def _header_struct(name):
-    """Fields (type, name, count) of `typedef struct { ... } name;` in include/paddi.h, in order."""
+    """Fields (type, name, count) of `typedef struct { ... } name;` in include/paddi.h, in
+    order (pointers as type "ptr")."""
     txt = open(os.path.join(ROOT, "include", "paddi.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     m = re.search(r"typedef struct\s*\{([^}]*)\}\s*" + name + r"\s*;", txt)
     assert m, name
-    return [(t, n, int(k) if k else 1) for t, n, k in
-            re.findall(r"\b(u?int(?:32|64)_t|float|double)\s+([a-z_0-9]+)\s*(?:\[(\d+)\])?\s*;",
-                       m.group(1))]
+    out = []
+    for decl in m.group(1).split(";"):
+        decl = " ".join(decl.split())
+        if not decl:
+            continue
+        f = re.fullmatch(r"(?:const )?(u?int(?:32|64)_t|float|double|char)\s*(\*?)\s*"
+                         r"([a-z_0-9]+)(?:\[(\d+)\])?", decl)
+        assert f, (name, decl)
+        t, ptr, n, k = f.groups()
+        out.append(("ptr" if ptr else t, n, int(k) if k else 1))
+    return out
 
 
 def test_python_structs_match_the_header():
     """The ctypes mirrors of the ABI's structs have the header's fields, order and widths (a
     field added to the header and not to the binding would shift every field after it)."""
-    width = {"int32_t": 4, "uint32_t": 4, "int64_t": 8, "uint64_t": 8, "float": 4, "double": 8}
-    for cname, py in (("paddi_tunables", engine.Tunables), ("paddi_stats", engine.Stats)):
+    width = {"int32_t": 4, "uint32_t": 4, "int64_t": 8, "uint64_t": 8, "float": 4, "double": 8,
+             "ptr": 8}
+    for cname, py in (("paddi_discord", engine.Discord), ("paddi_opts", engine.Opts),
+                      ("paddi_stats", engine.Stats), ("paddi_tunables", engine.Tunables)):
         fields = _header_struct(cname)
         assert [n for _, n, _ in fields] == [f[0] for f in py._fields_], cname
         assert [width[t] * k for t, _, k in fields] == [ctypes.sizeof(f[1]) for f in py._fields_], cname
+        size, align = 0, 1
+        for t, _, k in fields:  # C layout: each field at a multiple of its width
+            w = width[t]
+            size = (size + w - 1) // w * w + w * k
+            align = max(align, w)
+        assert ctypes.sizeof(py) == (size + align - 1) // align * align, cname
