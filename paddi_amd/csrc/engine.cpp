@@ -1372,6 +1372,10 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
                 }
                 r2[(size_t)q] = r;
             }
+        // A series whose windows cannot hold k discords (k of them lie at least m apart)
+        // ends every descent at r = 0, after up to max_attempts passes: go there at once.
+        for (int q = 0; q < ns; q++)
+            if (B.n[(size_t)q] - m < (int64_t)(k - 1) * m) r2[(size_t)q] = 0.0;
         std::vector<char> active((size_t)ns, 1);
         std::vector<int> att((size_t)ns, 0);
         for (int npass = 1;; npass++) {

@@ -40,6 +40,12 @@ CASES = {
     "ar2_k3_step4": (lambda: [synth.ar2_mix(100_000, 7)], 128, 192, 4, 3),
     "uneven_k1": (lambda: uneven(40, 11), 40, 60, 1, 1),
     "uneven_k2_step3": (lambda: uneven(12, 12), 48, 84, 3, 2),
+    # a series too short for two discords at the longer lengths beside a long one: the
+    # lengths where it falls short go back to the host (three attempts per length: its
+    # descents would otherwise run to the default 64)
+    "short_series_k2": (lambda: [synth.rw_planted(60_000, 23, 2, 48, 8.0),
+                                 synth.rw_planted(190, 24, 0, 0, 0.0)], 56, 100, 2, 2,
+                        dict(max_attempts=3)),
 }
 _HOST = {}
 
@@ -56,9 +62,10 @@ def restore(ctx):
 
 
 def run(ctx, name, mode, shift=0, **tun):
-    series, minL, maxL, step, k = CASES[name]
+    series, minL, maxL, step, k, *call = CASES[name]
     ctx.set_tunables(prev_shift=shift, **tun)
-    return ctx.discords_batch(series(), minL, maxL, step, k, mode=mode, with_stats=True)
+    return ctx.discords_batch(series(), minL, maxL, step, k, mode=mode, with_stats=True,
+                              **(call[0] if call else {}))
 
 
 def host(ctx, name, mode, shift=0):
