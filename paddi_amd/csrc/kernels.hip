@@ -2057,7 +2057,9 @@ __global__ __launch_bounds__(64) void k_nn_fix(ExactArgs E)
     }
     double ub = (double)ord_val(hk);
     if (E.ub) ub = fmin(ub, E.ub[c]);
-    unsigned long long mask = __ballot((double)lo <= ub);
+    // lo is infinite for a super-tile without a valid column (and past the series): never
+    // evaluated, also when the bound is infinite (a candidate without any valid column)
+    unsigned long long mask = __ballot(lo < INFINITY && (double)lo <= ub);
     if (!mask) return;
     const int64_t i = E.cand[c];
     const double2 si = st64[i];
