@@ -535,19 +535,19 @@ void Engine::enqueue_pass(const ScanPtrs &Q, size_t lds, size_t parb, int64_t ro
     if (timed) HIPC(hipEventRecord(ev_[0], stream_));
     launch_scan_select(Q, P, (unsigned)nwords, lds, stream_);
     if (timed) HIPC(hipEventRecord(ev_[1], stream_));
-    hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
+    hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256 * LIST_WPT)), dim3(256), 0, stream_,
                        (const uint64_t *)alive_.p, nwords, blist_.as<int>(), &ctl->list[0]);
     if (timed) HIPC(hipEventRecord(ev_[2], stream_));
     for (int r = 0; r < refine_rounds_; r++) {
         if (r > 0)
-            hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256)), dim3(256), 0, stream_,
+            hipLaunchKernelGGL(k_list_blocks, dim3(grid1(nwords, 256 * LIST_WPT)), dim3(256), 0, stream_,
                                (const uint64_t *)alive_.p, nwords, blist_.as<int>(),
                                &ctl->list[r]);
         launch_scan_refine(Q, P, blist_.as<int>(), ctl, r, grid, lds, stream_);
     }
     if (timed) HIPC(hipEventRecord(ev_[3], stream_));
-    hipLaunchKernelGGL(k_survivors, dim3(grid1(rows, 256)), dim3(256), 0, stream_, Q, P, rows,
-                       ctl, surv, rows);
+    hipLaunchKernelGGL(k_survivors, dim3(grid1(nwords, 256)), dim3(256), 0, stream_, Q, P,
+                       rows, ctl, surv, rows);
     if (host_io)
         HIPC(hipMemcpyAsync(pin_out_, pout_.p, PASS_OUT_HEAD + (size_t)scap * sizeof(Surv),
                             hipMemcpyDeviceToHost, stream_));

@@ -206,6 +206,7 @@ void launch_scan_refine(const ScanPtrs &Q, const PassDev *P, const int *blist, P
                         int round, unsigned grid, size_t lds, hipStream_t st);
 __global__ void k_pass_init(ScanPtrs Q, const PassDev *__restrict__ P, int64_t rows,
                             int64_t nwords, PassCtl *__restrict__ ctl);
+constexpr int LIST_WPT = 16;  // alive words per k_list_blocks thread
 __global__ void k_list_blocks(const uint64_t *__restrict__ alive, int64_t nwords,
                               int *__restrict__ list, int *__restrict__ count);
 // Restrict a pass to the global blocks flagged in keep (the others' alive words cleared).
