@@ -119,8 +119,9 @@ typedef struct {
                                 window up to 2, 8 (and 32) x refine_chunk, then the rest */
     int32_t seed_witnesses;  /* batched and learned forms, a call's first length: scan first the
                                 tile of a row with the same SAX word (0/1) */
-    int32_t refine_order;    /* refine work items block-major (0) or chunk-major (1: every
-                                block's nearest chunk first) */
+    int32_t refine_order;    /* refine work items block-major (0), chunk-major (1: every
+                                block's nearest chunk first), or chunk-major at a call's first
+                                length only (2) */
 } paddi_tunables;
 
 const char *paddi_status_str(paddi_status s);

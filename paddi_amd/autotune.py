@@ -43,7 +43,7 @@ GRID = {
     "device_rounds": [1, 2, 3],
     "refine_rounds": [3, 4],
     "seed_witnesses": [0, 1],
-    "refine_order": [0, 1],
+    "refine_order": [0, 1, 2],
 }
 
 

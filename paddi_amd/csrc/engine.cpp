@@ -618,7 +618,7 @@ ScanPtrs Engine::fill_pass(Batch &B, int m, const std::vector<double> &r2, bool 
     const int64_t end = refine ? nmax + A.U0 : A.U0 + A.WA;
     refine_bounds(A.U0 + A.WA, refine_chunk_, end, refine_rounds_, R.bound);
     R.min_chunk = min_chunk_;
-    R.order = refine_order_;
+    R.order = refine_order_ == 2 ? (first_len_ ? 1 : 0) : refine_order_;
     R.max_chunk = refine_chunk_;
     R.items = refine_waves_;
     return Q;
@@ -1277,6 +1277,7 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
         const int m_next = (ahead && j + 1 < nlen) ? m + step : 0;
         att_mark(st);
         derive(B, m, st, m_next);
+        first_len_ = j == 0;
         if (!wread_valid_ && seed_wit_ && m >= 8 && B.rows >= SAX_MIN_ROWS) {
             seed_witnesses(B, m);
             wread_valid_ = true;
@@ -1441,6 +1442,7 @@ void Engine::fast(Batch &B, int minL, int maxL, int step, int k, const paddi_opt
         wread_valid_ = true;
         witcur_ ^= 1;
     }
+    first_len_ = false;
     for (int q = 0; q < ns; q++) out.insert(out.end(), res[(size_t)q].begin(), res[(size_t)q].end());
 }
 
@@ -1588,7 +1590,7 @@ int Engine::chain(Batch &B, int minL, int step, int k, int j0, int jend, bool ah
     C.minL = minL;
     C.step = step;
     C.rrounds = refine_rounds_;
-    C.rorder = refine_order_;
+    C.rorder = refine_order_ == 1 ? 1 : 0;  // the loop never runs a call's first length
     C.shift = shift;
     C.gr = gr;
     C.nur = nur;
@@ -1999,7 +2001,7 @@ paddi_status paddi_set_tunables(paddi_ctx *ctx, const paddi_tunables *t)
         t->device_rounds < 1 || t->device_rounds > paddi::CHAIN_ROUNDS ||
         t->refine_rounds < 3 || t->refine_rounds > paddi::REFINE_ROUNDS_MAX ||
         t->seed_witnesses < 0 || t->seed_witnesses > 1 || t->refine_order < 0 ||
-        t->refine_order > 1)
+        t->refine_order > 2)
         return PADDI_ERR_INVALID;
     paddi::Engine &E = *ctx->eng;
     E.select_window_ = (t->select_window + 63) / 64 * 64;

@@ -193,7 +193,9 @@ public:
     int device_rounds_ = 2;          // exact rounds per length on the device
     int refine_rounds_ = 3;          // refine rounds per pass (refine_bounds)
     bool seed_wit_ = true;           // fast forms, first length: witnesses from SAX words
-    int refine_order_ = 0;           // refine work items block-major (0) or chunk-major (1)
+    int refine_order_ = 0;           // refine work items block-major (0), chunk-major (1), or
+                                     // chunk-major at a call's first length only (2)
+    bool first_len_ = false;         // fast(): the passes of a call's first length
     bool excl_sym_ = false;          // exclusion rule: |i - j| >= m - 1 instead of |i - 1 - j| >= m
     bool exact_nn_ = true;           // k_nn for exact refinement (PADDI_EXACT_NN=0: k_exact)
     bool debug_ = false;             // per-pass trace on stderr (PADDI_DEBUG=1)

@@ -35,6 +35,7 @@ SCHEDULES = [
     dict(seed_witnesses=1, witness_tiles=48, select_window=256),
     dict(refine_order=1),
     dict(refine_order=1, refine_items=4096, min_chunk=512, use_graph=1),
+    dict(refine_order=2),
 ]
 
 
@@ -55,7 +56,7 @@ def test_tunables_roundtrip_and_validation(ctx, restore):
                 dict(witness_tiles=-1), dict(witness_tiles=65), dict(speculate=0),
                 dict(prev_shift=5), dict(bootstrap=2), dict(direct_budget=-1),
                 dict(direct_rows=5), dict(refine_rounds=2), dict(refine_rounds=5),
-                dict(seed_witnesses=2), dict(refine_order=2)):
+                dict(seed_witnesses=2), dict(refine_order=3)):
         with pytest.raises(PaddiError):
             ctx.set_tunables(**bad)
         assert ctx.get_tunables() == u
