@@ -93,7 +93,7 @@ void Engine::features(Batch &B, int m, int k, std::vector<float> &X)
     hipLaunchKernelGGL(k_feat_pairs, dim3((unsigned)((waves * 64 + 255) / 256)), dim3(256), 0,
                        stream_, B.x.as<float>(), B.ser.as<SerDev>(), st64_.as<double2>(), ns, m,
                        FEAT_SEED, hist);
-    hipLaunchKernelGGL(k_feat_std, dim3((unsigned)((ns + 63) / 64)), dim3(64), 0, stream_,
+    hipLaunchKernelGGL(k_feat_std, dim3((unsigned)ns), dim3(64), 0, stream_,
                        B.ser.as<SerDev>(), B.P.as<double2>(), B.refs.as<double4>(), ns, gstd);
     std::vector<float> h((size_t)ns * 33);
     HIPC(hipMemcpyAsync(h.data(), hist, h.size() * 4, hipMemcpyDeviceToHost, stream_));

@@ -16,6 +16,7 @@ constexpr uint64_t FEAT_SEED = 0x5eed5eed1234ull;
 __global__ void k_feat_pairs(const float *__restrict__ x, const SerDev *__restrict__ ser,
                              const double2 *__restrict__ st64, int nser, int m, uint64_t seed,
                              float *__restrict__ hist);
+// one wave (64 threads) per series
 __global__ void k_feat_std(const SerDev *__restrict__ ser, const double2 *__restrict__ P,
                            const double4 *__restrict__ refs, int nser, float *__restrict__ gstd);
 __global__ void k_mlp_forward(const float *__restrict__ X, int B, const float *__restrict__ prm,

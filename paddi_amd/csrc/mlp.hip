@@ -66,22 +66,31 @@ __global__ __launch_bounds__(256) void k_feat_pairs(const float *__restrict__ x,
 }
 
 // Global standard deviation of each series from its prefix blocks (one thread per series).
-__global__ void k_feat_std(const SerDev *__restrict__ ser, const double2 *__restrict__ P,
-                           const double4 *__restrict__ refs, int nser, float *__restrict__ gstd)
+__global__ __launch_bounds__(64) void k_feat_std(const SerDev *__restrict__ ser,
+                                                 const double2 *__restrict__ P,
+                                                 const double4 *__restrict__ refs, int nser,
+                                                 float *__restrict__ gstd)
 {
-    const int sid = blockIdx.x * blockDim.x + threadIdx.x;
+    // one wave per series, its lanes over the series' prefix blocks (one thread per series
+    // read c3's 2,048 blocks one after another: 830 us)
+    const int sid = blockIdx.x;
     if (sid >= nser) return;
     const SerDev S = ser[sid];
     const int64_t nb = (S.n + PB - 1) / PB;
     double s1 = 0.0, s2 = 0.0;
     const double c0 = refs[S.pboff].x;
-    for (int64_t b = 0; b < nb; b++) {
+    for (int64_t b = threadIdx.x; b < nb; b += 64) {
         const double2 t = P[(S.pboff + b) * (PB + 1) + PB];
         const double cnt = (double)((b + 1) * PB <= S.n ? PB : S.n - b * PB);
         const double D = refs[S.pboff + b].x - c0;  // shift block sums to the reference c0
         s1 += t.x + cnt * D;
         s2 += t.y + 2.0 * D * t.x + cnt * D * D;
     }
+    for (int o = 32; o >= 1; o >>= 1) {
+        s1 += __shfl_xor(s1, o, 64);
+        s2 += __shfl_xor(s2, o, 64);
+    }
+    if (threadIdx.x != 0) return;
     const double mu = s1 / S.n;
     const double var = s2 / S.n - mu * mu;
     gstd[sid] = (float)sqrt(var > 0.0 ? var : 0.0);
